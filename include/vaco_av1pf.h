@@ -1,0 +1,284 @@
+/*
+ * vaco_av1pf.h - C ABI of the MI355X (gfx950) AV1 post-reconstruction filter
+ * stage: CDEF -> superres -> loop restoration, on 8/10/12-bit planes.
+ *
+ * This is the drop-in boundary for the stage that matteopolak/vaco runs as
+ * three private per-frame calls in
+ *   crates/codec/vaco-codec-av1/src/decode.rs:554-559
+ *     (copy_picture, apply_cdef, apply_superres, apply_restoration),
+ * and for the public kernel functions of the same crate
+ *   src/cdef.rs:79-230, src/superres.rs:163-274, src/restoration.rs:108-207.
+ *
+ * Conventions
+ *  - Plain C types and pointers only; no torch or HIP types in signatures
+ *    (streams are passed as `void*` holding a hipStream_t, NULL = the
+ *    context's own stream).
+ *  - Planes are u16 row-major, exactly like framebuf::Plane
+ *    (framebuf.rs:14-20), with an explicit stride in samples (>= width).
+ *  - Every entry point returns a vaco_av1pf_status; the human-readable
+ *    message of the last failure on a context is vaco_av1pf_last_error().
+ *    Messages equal the reference's `Error::InvalidData(&'static str)` text
+ *    where the reference has one.
+ *  - Nothing aborts or throws across this boundary.
+ *  - A context is not internally synchronised: use one per host thread
+ *    (the reference threads `&mut Budget` through a single thread).
+ */
+#ifndef VACO_AV1PF_H
+#define VACO_AV1PF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VACO_AV1PF_ABI_VERSION 1
+
+/* vaco_core::Error classes (crates/core/vaco-core/src/error.rs:14-59). */
+typedef enum vaco_av1pf_status {
+    VACO_AV1PF_OK = 0,
+    VACO_AV1PF_INVALID_DATA = 1,    /* Error::InvalidData(msg)                 */
+    VACO_AV1PF_UNSUPPORTED = 2,     /* Error::Unsupported(msg)                 */
+    VACO_AV1PF_LIMIT_EXCEEDED = 3,  /* Error::LimitExceeded (fuel, capacity)   */
+    VACO_AV1PF_DEVICE_ERROR = 4,    /* HIP runtime failure (no reference peer) */
+    VACO_AV1PF_INVALID_ARGUMENT = 5 /* NULL pointer / bad handle (Rust
+                                       references cannot be null, so the
+                                       reference has no peer for this)        */
+} vaco_av1pf_status;
+
+typedef struct vaco_av1pf_ctx vaco_av1pf_ctx;
+
+/* ---------------------------------------------------------------- CDEF --- */
+
+/* cdef::Strength, cdef.rs:14-20 (8-bit units; secondary in {0,1,2,4}). */
+typedef struct vaco_av1pf_strength {
+    uint8_t primary;
+    uint8_t secondary;
+} vaco_av1pf_strength;
+
+/* cdef::CdefParams, cdef.rs:23-33. `damping` is CdefDamping (includes the
+ * syntax's +3). `enabled` = 0 disables CDEF for the frame. */
+typedef struct vaco_av1pf_cdef_params {
+    uint8_t enabled;
+    uint8_t damping;
+    vaco_av1pf_strength y[8];
+    vaco_av1pf_strength uv[8];
+} vaco_av1pf_cdef_params;
+
+/* cdef::FilterParams, cdef.rs:35-54. */
+typedef struct vaco_av1pf_filter_params {
+    uint32_t width;      /* 4 or 8 */
+    uint32_t height;     /* 4 or 8 */
+    uint8_t edges;       /* left=1 right=2 top=4 bottom=8 */
+    uint8_t bit_depth;   /* 8, 10, 12 */
+    uint8_t direction;   /* 0..7 */
+    uint8_t damping;     /* plane damping incl. bit-depth and chroma adjust */
+    uint16_t primary;    /* after variance adjustment and <<(bd-8) */
+    uint16_t secondary;  /* after <<(bd-8) */
+} vaco_av1pf_filter_params;
+
+/* ---------------------------------------------------------- superres --- */
+
+/* superres::PlaneConfig, superres.rs:93-103. */
+typedef struct vaco_av1pf_sr_plane_config {
+    uint32_t visible_width;
+    uint32_t output_width;
+    uint32_t height;
+    uint8_t bit_depth;
+} vaco_av1pf_sr_plane_config;
+
+/* ------------------------------------------------------- restoration --- */
+
+/* restoration::RestorationType, restoration.rs:15-27 (frame-level mode). */
+enum {
+    VACO_AV1PF_RESTORE_NONE = 0,
+    VACO_AV1PF_RESTORE_SWITCHABLE = 1,
+    VACO_AV1PF_RESTORE_WIENER = 2,
+    VACO_AV1PF_RESTORE_SELF_GUIDED = 3
+};
+
+/* restoration::RestorationUnit discriminant, restoration.rs:46-65. */
+enum {
+    VACO_AV1PF_UNIT_NONE = 0,
+    VACO_AV1PF_UNIT_WIENER = 1,
+    VACO_AV1PF_UNIT_SELF_GUIDED = 2
+};
+
+/* restoration::RestorationUnit, restoration.rs:46-65. Fields not used by
+ * `kind` are ignored. Coefficient ranges are validated exactly as
+ * restoration.rs:67-91 does. */
+typedef struct vaco_av1pf_lr_unit {
+    uint8_t kind;           /* VACO_AV1PF_UNIT_* */
+    uint8_t set;            /* self-guided: lr_sgr_set, 0..15 */
+    int16_t xqd[2];         /* self-guided: LrSgrXqd */
+    int16_t vertical[3];    /* Wiener: outermost first */
+    int16_t horizontal[3];  /* Wiener: outermost first */
+} vaco_av1pf_lr_unit;
+
+/* restoration::PlaneConfig, restoration.rs:93-106. */
+typedef struct vaco_av1pf_lr_plane_config {
+    uint32_t width;
+    uint32_t height;
+    uint32_t unit_size;     /* 32, 64, 128, 256 */
+    uint8_t bit_depth;
+    uint8_t subsampling_y;
+} vaco_av1pf_lr_plane_config;
+
+/* ------------------------------------------------------------ planes --- */
+
+/* framebuf::Plane, framebuf.rs:14-20, with an explicit stride. `data` is a
+ * host pointer for the host entry points and a device pointer for the
+ * *_device entry points. */
+typedef struct vaco_av1pf_plane {
+    uint16_t *data;
+    uint32_t width;
+    uint32_t height;
+    uint32_t stride;        /* samples between rows, >= width */
+} vaco_av1pf_plane;
+
+/* framebuf::Picture, framebuf.rs:78-83. plane[1], plane[2] unused (data may
+ * be NULL) when the frame is monochrome. */
+typedef struct vaco_av1pf_picture {
+    vaco_av1pf_plane plane[3];
+} vaco_av1pf_picture;
+
+/* Everything decode_frame() holds for the filter stage (decode.rs:506-561,
+ * FrameCtx decode.rs:185-214, FrameSize frame_header.rs:71-85).
+ *
+ * Input picture geometry is the Mi-padded reconstruction (decode.rs:518-530):
+ *   mi_cols = 2*((coded_width+7)>>3), mi_rows = 2*((coded_height+7)>>3),
+ *   luma mi_cols*4 x mi_rows*4, chroma (luma_w>>ss_x) x (luma_h>>ss_y).
+ * Output picture geometry is the reference's final ctx.pic
+ * (vaco_av1pf_frame_output_geometry computes it). */
+typedef struct vaco_av1pf_frame_desc {
+    uint32_t coded_width;
+    uint32_t upscaled_width;    /* == coded_width unless use_superres */
+    uint32_t coded_height;
+    uint8_t use_superres;
+    uint8_t bit_depth;          /* 8, 10, 12 */
+    uint8_t subsampling_x;
+    uint8_t subsampling_y;
+    uint8_t monochrome;
+    vaco_av1pf_cdef_params cdef;
+    /* cdef_idx per 64x64 luma unit, ceil(mi_rows/16) rows of cdef_stride
+     * entries (cdef_stride = ceil(mi_cols/16)); -1 = not read (decode.rs:543,
+     * 774-776). May be NULL when cdef.enabled == 0. */
+    const int8_t *cdef_idx;
+    uint32_t cdef_stride;
+    /* One byte per 8x8 luma block, (mi_rows/2) rows of (mi_cols/2): nonzero
+     * when all four 4x4 MiCells of the block are skip (decode.rs:892-896;
+     * vaco_av1pf_skip8_from_mi derives it from a MiCell skip grid). */
+    const uint8_t *skip8;
+    /* FrameRestoration, restoration.rs:29-44. */
+    uint8_t lr_type[3];         /* VACO_AV1PF_RESTORE_* */
+    uint32_t lr_unit_size[3];
+    /* Unit maps per plane, row-major, unit_counts() long (decode.rs:612-638) */
+    const vaco_av1pf_lr_unit *lr_units[3];
+    uint32_t lr_unit_count[3];
+} vaco_av1pf_frame_desc;
+
+/* ------------------------------------------------------ context/errors --- */
+
+/* Create a context on HIP device `device`. Device buffers for frames up to
+ * max_width x max_height luma samples (padded or upscaled, whichever is
+ * larger) are allocated here, so the per-frame calls allocate nothing.
+ * No reference peer: the reference allocates per stage through Budget. */
+vaco_av1pf_status vaco_av1pf_create(int device, uint32_t max_width,
+                                    uint32_t max_height, vaco_av1pf_ctx **out);
+void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx);
+/* Static string describing the last failure on ctx ("" if none). */
+const char *vaco_av1pf_last_error(const vaco_av1pf_ctx *ctx);
+/* ABI version of the loaded library (VACO_AV1PF_ABI_VERSION). */
+uint32_t vaco_av1pf_abi_version(void);
+
+/* ------------------------------------------------- whole-stage entries --- */
+
+/* Output plane geometry of the stage for `desc`: the reference's final
+ * ctx.pic after apply_cdef/apply_superres/apply_restoration
+ * (decode.rs:734, 857, 887). Host-only, no GPU. */
+vaco_av1pf_status vaco_av1pf_frame_output_geometry(
+    const vaco_av1pf_frame_desc *desc, uint32_t width[3], uint32_t height[3]);
+
+/* Input (Mi-padded) plane geometry for `desc` (decode.rs:518-530). */
+vaco_av1pf_status vaco_av1pf_frame_input_geometry(
+    const vaco_av1pf_frame_desc *desc, uint32_t width[3], uint32_t height[3]);
+
+/* The whole stage on host planes: replaces decode.rs:554-559 as a unit
+ * (copy_picture + apply_cdef + apply_superres + apply_restoration).
+ * `in` is read only; `out` receives the final picture. `fuel`, if not NULL,
+ * is the Budget fuel still available (budget.rs:229-237): on return it holds
+ * the remaining fuel; exhaustion returns VACO_AV1PF_LIMIT_EXCEEDED at the
+ * same point the reference would. Synchronous. */
+vaco_av1pf_status vaco_av1pf_frame(vaco_av1pf_ctx *ctx,
+                                   const vaco_av1pf_frame_desc *desc,
+                                   const vaco_av1pf_picture *in,
+                                   vaco_av1pf_picture *out, uint64_t *fuel);
+
+/* The same stage on device-resident planes, enqueued on `stream`
+ * (hipStream_t; NULL = the context's stream). Host-side validation errors
+ * return at once; sample-range errors found by the kernels are reported by
+ * the next vaco_av1pf_finish() on this context. */
+vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
+                                          const vaco_av1pf_frame_desc *desc,
+                                          const vaco_av1pf_picture *in,
+                                          vaco_av1pf_picture *out,
+                                          void *stream);
+
+/* Wait for the context's enqueued work and report deferred errors. */
+vaco_av1pf_status vaco_av1pf_finish(vaco_av1pf_ctx *ctx);
+
+/* ------------------------------------------- kernel-level GPU entries --- */
+
+/* cdef::find_direction (cdef.rs:79-124) and cdef::filter_block
+ * (cdef.rs:142-205) for `n` independent cases on the GPU. inputs: n*144
+ * samples (12x12, block at (2,2)); the direction search reads the 8x8 at
+ * (2,2). outputs: dirs[n], vars[n], out[n*64] (stride 8, zero outside
+ * width x height). Host buffers. */
+vaco_av1pf_status vaco_av1pf_cdef_blocks(vaco_av1pf_ctx *ctx, uint32_t n,
+                                         const uint16_t *inputs,
+                                         const vaco_av1pf_filter_params *params,
+                                         uint8_t *dirs, uint32_t *vars,
+                                         uint16_t *out);
+
+/* superres::upscale_plane, superres.rs:163-216, on the GPU. Host planes;
+ * in->width is the padded width used for edge clamping. out must be
+ * output_width x height. */
+vaco_av1pf_status vaco_av1pf_upscale_plane(vaco_av1pf_ctx *ctx,
+                                           const vaco_av1pf_plane *in,
+                                           const vaco_av1pf_sr_plane_config *cfg,
+                                           vaco_av1pf_plane *out);
+
+/* restoration::restore_plane, restoration.rs:178-207, on the GPU. Host
+ * planes; out must be cfg->width x cfg->height. */
+vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
+                                           const vaco_av1pf_plane *before_cdef,
+                                           const vaco_av1pf_plane *after_cdef,
+                                           const vaco_av1pf_lr_plane_config *cfg,
+                                           const vaco_av1pf_lr_unit *units,
+                                           uint32_t n_units,
+                                           vaco_av1pf_plane *out);
+
+/* ------------------------------------------------ host-only helpers --- */
+
+/* cdef::adjust_strength, cdef.rs:208-220. */
+uint16_t vaco_av1pf_adjust_strength(uint16_t primary, uint32_t variance);
+/* cdef::chroma_direction, cdef.rs:222-230. */
+uint8_t vaco_av1pf_chroma_direction(uint8_t direction, uint8_t sub_x,
+                                    uint8_t sub_y);
+/* restoration::PlaneConfig::unit_counts, restoration.rs:111-120. */
+void vaco_av1pf_lr_unit_counts(const vaco_av1pf_lr_plane_config *cfg,
+                               uint32_t *cols, uint32_t *rows);
+/* restoration::PlaneConfig::unit_index, restoration.rs:124-133. */
+uint32_t vaco_av1pf_lr_unit_index(const vaco_av1pf_lr_plane_config *cfg,
+                                  uint32_t x, uint32_t y);
+/* decode.rs:892-896 cdef_block_skipped over a MiCell skip grid
+ * (mi_rows x mi_cols, nonzero = skip) into the 8x8 map of the frame desc. */
+void vaco_av1pf_skip8_from_mi(const uint8_t *mi_skip, uint32_t mi_rows,
+                              uint32_t mi_cols, uint8_t *skip8);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VACO_AV1PF_H */
