@@ -1,0 +1,94 @@
+"""Hand-built frames of arbitrary geometry for parity tests.
+
+``random_frame`` covers what the seeded workload generator does not: odd
+coded sizes, 4:4:4 / 4:2:2 / 4:4:0 / monochrome, 12-bit, superres with
+loop restoration, small restoration units, every CDEF index pattern.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from vaco_amd import abi
+from vaco_amd.av1pf import Frame
+
+
+def padded_geometry(coded_w, coded_h, ssx, ssy, mono):
+    mi_cols = 2 * ((coded_w + 7) >> 3)
+    mi_rows = 2 * ((coded_h + 7) >> 3)
+    w0, h0 = mi_cols * 4, mi_rows * 4
+    geo = [(h0, w0)]
+    if not mono:
+        geo += [(h0 >> ssy, w0 >> ssx)] * 2
+    return mi_cols, mi_rows, geo
+
+
+def textured_plane(rng, h, w, bd, edges=True):
+    mx = (1 << bd) - 1
+    ys, xs = np.mgrid[0:h, 0:w]
+    base = (xs * 97 // max(w, 1) + ys * 61 // max(h, 1) + 40) << (bd - 8)
+    v = base + rng.normal(0, 5 << (bd - 8), size=(h, w))
+    if edges:
+        for _ in range(3):
+            a = rng.uniform(0, np.pi)
+            px, py = rng.uniform(0, w), rng.uniform(0, h)
+            side = (xs - px) * np.cos(a) + (ys - py) * np.sin(a) > 0
+            v = v + side * rng.choice([-1, 1]) * (40 << (bd - 8))
+    return np.clip(np.rint(v), 0, mx).astype(np.uint16)
+
+
+def random_frame(seed, coded_w, coded_h, bd=8, ssx=1, ssy=1, mono=False, upscaled_w=None, lr=True,
+                 lr_types=(1, 1, 1), unit=64, uv_unit=None, cdef=True, skip_p=0.2, idx_neg_p=0.1,
+                 zero_strengths=False, unit_kinds=(0, 1, 2)):
+    rng = np.random.default_rng(seed)
+    mi_cols, mi_rows, geo = padded_geometry(coded_w, coded_h, ssx, ssy, mono)
+    planes = [textured_plane(rng, h, w, bd) for (h, w) in geo]
+    d = abi.FrameDesc()
+    d.coded_width, d.coded_height = coded_w, coded_h
+    d.upscaled_width = upscaled_w if upscaled_w else coded_w
+    d.use_superres = 1 if upscaled_w else 0
+    d.bit_depth = bd
+    d.subsampling_x, d.subsampling_y = (0, 0) if mono else (ssx, ssy)
+    d.monochrome = 1 if mono else 0
+    d.cdef.enabled = 1 if cdef else 0
+    d.cdef.damping = int(rng.integers(3, 7))
+    for i in range(8):
+        for s in (d.cdef.y[i], d.cdef.uv[i]):
+            s.primary = 0 if zero_strengths else int(rng.integers(0, 16))
+            s.secondary = 0 if zero_strengths else int(rng.choice([0, 1, 2, 4]))
+    stride = (mi_cols + 15) // 16
+    idx = rng.integers(0, 8, size=((mi_rows + 15) // 16) * stride).astype(np.int8)
+    idx[rng.random(idx.shape) < idx_neg_p] = -1
+    skip = (rng.random((mi_rows // 2) * (mi_cols // 2)) < skip_p).astype(np.uint8)
+    d.cdef_idx = idx.ctypes.data_as(abi.C.POINTER(abi.C.c_int8))
+    d.cdef_stride = stride
+    d.skip8 = skip.ctypes.data_as(abi.C.POINTER(abi.C.c_uint8))
+    keep = [idx, skip]
+    if lr:
+        np_ = 1 if mono else 3
+        for p in range(np_):
+            sx = ssx if p and not mono else 0
+            sy = ssy if p and not mono else 0
+            w = d.upscaled_width >> sx
+            h = coded_h >> sy
+            u = unit if p == 0 else (uv_unit or unit)
+            d.lr_type[p] = lr_types[p]
+            d.lr_unit_size[p] = u
+            cols = max(1, (w + u // 2) // u)
+            rows = max(1, (h + u // 2) // u)
+            units = []
+            for _ in range(cols * rows):
+                kind = int(rng.choice(unit_kinds))
+                if kind == abi.UNIT_WIENER:
+                    taps = lambda: (0 if p else int(rng.integers(-5, 11)), int(rng.integers(-23, 9)),
+                                    int(rng.integers(-17, 47)))
+                    units.append(abi.make_unit(kind, vertical=taps(), horizontal=taps()))
+                elif kind == abi.UNIT_SELF_GUIDED:
+                    units.append(abi.make_unit(kind, set=int(rng.integers(0, 16)),
+                                               xqd=(int(rng.integers(-96, 32)), int(rng.integers(-32, 96)))))
+                else:
+                    units.append(abi.make_unit())
+            arr = abi.units_array(units)
+            keep.append(arr)
+            d.lr_units[p] = abi.C.cast(arr, abi.C.POINTER(abi.LrUnit))
+            d.lr_unit_count[p] = len(units)
+    return Frame(planes, d, keep=keep)

@@ -1,0 +1,182 @@
+"""GPU parity: the HIP path against the reference's dav1d golden vectors and
+against the oracle on seeded frames. Bit-exact (tolerance 0) everywhere:
+the stage is integer arithmetic.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import frames as F
+import golden_inputs as G
+from oracle import oracle as O
+from vaco_amd import abi, av1pf, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = av1pf.Context(3840, 2176)
+    yield c
+    c.close()
+
+
+def assert_frame_equal(got, want, label):
+    assert len(got) == len(want), label
+    for p, (g, w) in enumerate(zip(got, want)):
+        assert g.shape == w.shape, (label, p, g.shape, w.shape)
+        if not np.array_equal(g, w):
+            ys, xs = np.nonzero(g != w)
+            raise AssertionError(f"{label} plane {p}: {len(ys)} samples differ, first at y={ys[0]} x={xs[0]} "
+                                 f"got {g[ys[0], xs[0]]} want {w[ys[0], xs[0]]}")
+
+
+# ------------------------------------------------------ golden vectors ---
+
+def test_gpu_cdef_blocks_match_dav1d_corpus(ctx):
+    say("cdef corpus: 7680 cases")
+    inputs, params, exp_dir, exp_var, exp_out = G.cdef_cases()
+    dirs, vars_, outs = ctx.cdef_blocks(inputs, params)
+    assert np.array_equal(dirs, exp_dir.astype(np.uint8))
+    assert np.array_equal(vars_, exp_var)
+    assert np.array_equal(outs, exp_out)
+
+
+def test_gpu_superres_matches_dav1d_corpus(ctx):
+    for case_id, (bd, vis, padded, out_w, h, inp, exp) in enumerate(G.superres_cases()):
+        say(f"superres case {case_id} bd={bd} {vis}/{padded}->{out_w}x{h}")
+        got = ctx.upscale_plane(inp, vis, out_w, h, bd)
+        assert np.array_equal(got, exp), f"superres case {case_id}"
+
+
+def test_gpu_restoration_matches_dav1d_corpus(ctx):
+    n = 0
+    for (bd, w, h, ssy, unit, mode, before, after, units, cfg, exp) in G.restoration_cases():
+        if mode == 0:
+            say(f"restoration bd={bd} {w}x{h} unit={unit}")
+        got = ctx.restore_plane(before, after, cfg, units)
+        assert np.array_equal(got, exp), f"depth={bd} {w}x{h} mode={mode}"
+        n += 1
+    assert n == 300
+
+
+def test_gpu_restoration_constant_planes_match_dav1d(ctx):
+    say("restoration constant planes")
+    for bd, s, plane, exp in G.constant_cases():
+        cfg = abi.LrPlaneConfig(7, 9, 64, bd, 0)
+        sgr = abi.make_unit(abi.UNIT_SELF_GUIDED, set=s, xqd=(-32, 31))
+        assert np.array_equal(ctx.restore_plane(plane, plane, cfg, [sgr]), exp), (bd, s)
+        wiener = abi.make_unit(abi.UNIT_WIENER, vertical=(-5, -23, -17), horizontal=(10, 8, 46))
+        assert np.array_equal(ctx.restore_plane(plane, plane, cfg, [wiener]), plane)
+
+
+# ------------------------------------------------------- seeded frames ---
+
+@pytest.mark.parametrize("frame_index", [0, 1, 2, 3])
+def test_gpu_frame_cfg0_matches_oracle(ctx, frame_index):
+    say(f"cfg0 frame {frame_index}")
+    f = av1pf.generate(workloads.cfg0(), frame_index)
+    assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), f"cfg0/{frame_index}")
+
+
+def test_gpu_frame_cfg1_matches_oracle(ctx):
+    say("cfg1 frame 0")
+    f = av1pf.generate(workloads.cfg1(), 0)
+    assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), "cfg1/0")
+
+
+RANDOM_CASES = [
+    dict(seed=1, coded_w=64, coded_h=64),
+    dict(seed=2, coded_w=100, coded_h=70),
+    dict(seed=3, coded_w=131, coded_h=97, bd=10),
+    dict(seed=4, coded_w=90, coded_h=66, bd=12),
+    dict(seed=5, coded_w=72, coded_h=80, ssx=0, ssy=0),
+    dict(seed=6, coded_w=72, coded_h=80, ssx=1, ssy=0),
+    dict(seed=7, coded_w=72, coded_h=80, ssx=0, ssy=1, unit=32),
+    dict(seed=8, coded_w=77, coded_h=45, mono=True),
+    dict(seed=9, coded_w=150, coded_h=140, unit=32),
+    dict(seed=10, coded_w=200, coded_h=130, unit=128, uv_unit=64),
+    dict(seed=11, coded_w=93, coded_h=75, upscaled_w=160, lr=False),
+    dict(seed=12, coded_w=93, coded_h=75, upscaled_w=160),
+    dict(seed=13, coded_w=171, coded_h=130, upscaled_w=300, bd=10, unit=32),
+    dict(seed=14, coded_w=64, coded_h=64, lr_types=(0, 1, 0)),
+    dict(seed=15, coded_w=80, coded_h=72, cdef=False),
+    dict(seed=16, coded_w=80, coded_h=72, cdef=False, lr=False),
+    dict(seed=17, coded_w=80, coded_h=72, zero_strengths=True, upscaled_w=128),
+    dict(seed=18, coded_w=260, coded_h=200, unit_kinds=(2,), unit=64),
+    dict(seed=19, coded_w=260, coded_h=200, unit_kinds=(1,), unit=256),
+]
+
+
+@pytest.mark.parametrize("case", RANDOM_CASES, ids=lambda c: "s%d" % c["seed"])
+def test_gpu_frame_random_geometry_matches_oracle(ctx, case):
+    say(f"random frame {case}")
+    f = F.random_frame(**case)
+    assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), str(case))
+
+
+def test_gpu_frame_cfg2_4k_superres_matches_oracle(ctx):
+    say("cfg2 frame 0 (4K 10-bit superres)")
+    f = av1pf.generate(workloads.cfg2(), 0)
+    assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), "cfg2/0")
+
+
+def test_gpu_frame_device_entry_matches_host_entry(ctx):
+    import torch
+    say("device entry vs host entry, cfg1 frames 0-1")
+    for k in range(2):
+        f = av1pf.generate(workloads.cfg1(), k)
+        want = ctx.frame(f)
+        dev_in = [torch.from_numpy(p.view(np.int16)).cuda() for p in f.planes]
+        dev_out = [torch.empty(g, dtype=torch.int16, device="cuda") for g in f.output_geometry()]
+        torch.cuda.synchronize()
+        ctx.frame_device(f, [t.data_ptr() for t in dev_in], [t.data_ptr() for t in dev_out])
+        ctx.finish()
+        got = [t.cpu().numpy().view(np.uint16) for t in dev_out]
+        assert_frame_equal(got, want, f"device/{k}")
+
+
+# --------------------------------------------------------------- errors ---
+
+def test_gpu_out_of_range_samples_report_the_reference_error(ctx):
+    say("error: out-of-range samples")
+    f = F.random_frame(40, 64, 64, skip_p=0.0, idx_neg_p=0.0)
+    f.planes[0][10, 10] = 256
+    with pytest.raises(O.OracleError) as want:
+        O.frame(f.desc, f.planes)
+    with pytest.raises(av1pf.Av1Error) as got:
+        ctx.frame(f)
+    assert (got.value.status, got.value.message) == (want.value.status, want.value.message)
+    # a bad sample no stage reads is not an error (CDEF skips it, LR off)
+    g = F.random_frame(41, 64, 64, skip_p=1.0, lr=False)
+    g.planes[1][5, 5] = 4000
+    assert_frame_equal(ctx.frame(g), O.frame(g.desc, g.planes), "unread bad sample")
+
+
+def test_gpu_invalid_parameters_report_the_reference_error(ctx):
+    say("error: invalid parameters")
+    f = F.random_frame(42, 64, 64, skip_p=0.0, idx_neg_p=0.0)
+    f.desc.cdef.y[3].primary = 16
+    f._keep[0][:] = 3
+    with pytest.raises(O.OracleError) as want:
+        O.frame(f.desc, f.planes)
+    with pytest.raises(av1pf.Av1Error) as got:
+        ctx.frame(f)
+    assert got.value.message == want.value.message
+
+
+def test_gpu_fuel_exhaustion_matches_reference_accounting(ctx):
+    say("error: fuel")
+    f = F.random_frame(43, 64, 32, lr=False, skip_p=0.0, idx_neg_p=0.0, upscaled_w=80)
+    need = 32 + 80 * 32 + 2 * 40 * 16
+    out, left = ctx.frame(f, fuel=need + 5)
+    assert left == 5
+    with pytest.raises(av1pf.Av1Error) as e:
+        ctx.frame(f, fuel=need - 1)
+    assert e.value.status == abi.LIMIT_EXCEEDED and e.value.message == "fuel"

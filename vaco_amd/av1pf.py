@@ -1,0 +1,298 @@
+"""Python mirror of the stage's operator interface over the C ABI.
+
+The functions here keep the reference's names, argument meaning and error
+behaviour (crates/codec/vaco-codec-av1/src/ of matteopolak/vaco):
+
+* :func:`find_direction_filter_blocks` - ``cdef::find_direction`` +
+  ``cdef::filter_block`` (cdef.rs:79-205) on the GPU, batched
+* :func:`upscale_plane` - ``superres::upscale_plane`` (superres.rs:163-216)
+* :func:`restore_plane` - ``restoration::restore_plane`` (restoration.rs:178-207)
+* :meth:`Context.frame` - ``decode.rs:554-559`` (pre-CDEF copy, apply_cdef,
+  apply_superres, apply_restoration) on host planes
+* :meth:`Context.frame_device` - the same stage on device-resident planes
+
+Failures raise :class:`Av1Error` carrying the reference's error class
+(``InvalidData``, ``LimitExceeded``, ...) and message. The HIP library is
+mandatory: importing this module on a machine without it raises at once;
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvaco_av1pf.so")
+
+_lib = None
+
+
+class Av1Error(Exception):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{abi.STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+class GenConfig(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("superres_denom", C.c_uint32),
+        ("bit_depth", C.c_uint8),
+        ("hard_edges", C.c_uint8),
+        ("monochrome", C.c_uint8),
+        ("lr_enabled", C.c_uint8),
+        ("noise_sigma", C.c_uint16),
+        ("lr_unit_size", C.c_uint16),
+        ("skip_permille", C.c_uint16),
+        ("pad", C.c_uint16),
+        ("seed", C.c_uint64),
+    ]
+
+
+class GenLayout(C.Structure):
+    _fields_ = [
+        ("coded_width", C.c_uint32),
+        ("upscaled_width", C.c_uint32),
+        ("coded_height", C.c_uint32),
+        ("superres_denom", C.c_uint32),
+        ("plane_width", C.c_uint32 * 3),
+        ("plane_height", C.c_uint32 * 3),
+        ("cdef_stride", C.c_uint32),
+        ("cdef_idx_count", C.c_uint32),
+        ("skip8_count", C.c_uint32),
+        ("lr_unit_count", C.c_uint32 * 3),
+    ]
+
+
+def lib():
+    """The loaded HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        ctxp = C.c_void_p
+        L.vaco_av1pf_create.argtypes = [C.c_int, C.c_uint32, C.c_uint32, P(ctxp)]
+        L.vaco_av1pf_destroy.argtypes = [ctxp]
+        L.vaco_av1pf_destroy.restype = None
+        L.vaco_av1pf_last_error.argtypes = [ctxp]
+        L.vaco_av1pf_last_error.restype = C.c_char_p
+        L.vaco_av1pf_abi_version.restype = C.c_uint32
+        L.vaco_av1pf_frame_output_geometry.argtypes = [P(abi.FrameDesc), P(C.c_uint32), P(C.c_uint32)]
+        L.vaco_av1pf_frame_input_geometry.argtypes = [P(abi.FrameDesc), P(C.c_uint32), P(C.c_uint32)]
+        L.vaco_av1pf_frame.argtypes = [ctxp, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture), P(C.c_uint64)]
+        L.vaco_av1pf_frame_device.argtypes = [ctxp, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture), C.c_void_p]
+        L.vaco_av1pf_finish.argtypes = [ctxp]
+        L.vaco_av1pf_cdef_blocks.argtypes = [ctxp, C.c_uint32, P(C.c_uint16), P(abi.FilterParams),
+                                             P(C.c_uint8), P(C.c_uint32), P(C.c_uint16)]
+        L.vaco_av1pf_upscale_plane.argtypes = [ctxp, P(abi.Plane), P(abi.SrPlaneConfig), P(abi.Plane)]
+        L.vaco_av1pf_restore_plane.argtypes = [ctxp, P(abi.Plane), P(abi.Plane), P(abi.LrPlaneConfig),
+                                               P(abi.LrUnit), C.c_uint32, P(abi.Plane)]
+        L.vaco_av1pf_adjust_strength.argtypes = [C.c_uint16, C.c_uint32]
+        L.vaco_av1pf_adjust_strength.restype = C.c_uint16
+        L.vaco_av1pf_chroma_direction.argtypes = [C.c_uint8, C.c_uint8, C.c_uint8]
+        L.vaco_av1pf_chroma_direction.restype = C.c_uint8
+        L.vaco_av1pf_lr_unit_counts.argtypes = [P(abi.LrPlaneConfig), P(C.c_uint32), P(C.c_uint32)]
+        L.vaco_av1pf_lr_unit_counts.restype = None
+        L.vaco_av1pf_lr_unit_index.argtypes = [P(abi.LrPlaneConfig), C.c_uint32, C.c_uint32]
+        L.vaco_av1pf_lr_unit_index.restype = C.c_uint32
+        L.vaco_av1pf_skip8_from_mi.argtypes = [P(C.c_uint8), C.c_uint32, C.c_uint32, P(C.c_uint8)]
+        L.vaco_av1pf_skip8_from_mi.restype = None
+        L.vaco_av1pf_gen_get_layout.argtypes = [P(GenConfig), C.c_uint32, P(GenLayout)]
+        L.vaco_av1pf_gen_frame.argtypes = [P(GenConfig), C.c_uint32, P(abi.FrameDesc), P(abi.Picture),
+                                           P(C.c_int8), P(C.c_uint8), P(abi.LrUnit), P(abi.LrUnit),
+                                           P(abi.LrUnit)]
+        for name in ("vaco_av1pf_create", "vaco_av1pf_frame", "vaco_av1pf_frame_device", "vaco_av1pf_finish",
+                     "vaco_av1pf_cdef_blocks", "vaco_av1pf_upscale_plane", "vaco_av1pf_restore_plane",
+                     "vaco_av1pf_frame_output_geometry", "vaco_av1pf_frame_input_geometry",
+                     "vaco_av1pf_gen_get_layout", "vaco_av1pf_gen_frame"):
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+# --------------------------------------------------------------- frames --
+
+class Frame:
+    """A frame's input planes and side information (host memory).
+
+    ``desc`` is the ``vaco_av1pf_frame_desc`` pointing into the arrays held
+    here, so the object must outlive every call that uses ``desc``.
+    """
+
+    def __init__(self, planes, desc: abi.FrameDesc, keep=()):
+        self.planes = planes
+        self.desc = desc
+        self._keep = list(keep)
+
+    @property
+    def nplanes(self) -> int:
+        return 1 if self.desc.monochrome else 3
+
+    def output_geometry(self):
+        w, h = (C.c_uint32 * 3)(), (C.c_uint32 * 3)()
+        lib().vaco_av1pf_frame_output_geometry(C.byref(self.desc), w, h)
+        return [(h[i], w[i]) for i in range(self.nplanes)]
+
+    @property
+    def output_luma_pixels(self) -> int:
+        return int(self.desc.upscaled_width) * int(self.desc.coded_height)
+
+
+def generate(cfg: GenConfig, frame_index: int = 0) -> Frame:
+    """Seeded synthetic frame `frame_index` of workload `cfg`."""
+    L = lib()
+    lay = GenLayout()
+    st = L.vaco_av1pf_gen_get_layout(C.byref(cfg), frame_index, C.byref(lay))
+    if st != abi.OK:
+        raise Av1Error(st, "invalid generator configuration")
+    np_ = 1 if cfg.monochrome else 3
+    planes = [np.zeros((lay.plane_height[p], lay.plane_width[p]), np.uint16) for p in range(np_)]
+    idx = np.zeros(max(1, lay.cdef_idx_count), np.int8)
+    skip = np.zeros(max(1, lay.skip8_count), np.uint8)
+    units = [(abi.LrUnit * max(1, lay.lr_unit_count[p]))() for p in range(3)]
+    desc = abi.FrameDesc()
+    pic = abi.picture_of(planes)
+    st = L.vaco_av1pf_gen_frame(C.byref(cfg), frame_index, C.byref(desc), C.byref(pic),
+                                idx.ctypes.data_as(C.POINTER(C.c_int8)),
+                                skip.ctypes.data_as(C.POINTER(C.c_uint8)), units[0], units[1], units[2])
+    if st != abi.OK:
+        raise Av1Error(st, "generator failed")
+    return Frame(planes, desc, keep=(idx, skip, units))
+
+
+# -------------------------------------------------------------- context --
+
+class Context:
+    """A device context (``vaco_av1pf_ctx``) sized for frames up to max_w x max_h."""
+
+    def __init__(self, max_width: int = 3840, max_height: int = 2160, device: int = 0):
+        self._lib = lib()
+        h = C.c_void_p()
+        st = self._lib.vaco_av1pf_create(device, max_width, max_height, C.byref(h))
+        if st != abi.OK:
+            raise Av1Error(st, "vaco_av1pf_create failed")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self._lib.vaco_av1pf_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int):
+        if st != abi.OK:
+            msg = self._lib.vaco_av1pf_last_error(self.handle)
+            raise Av1Error(st, msg.decode() if msg else "")
+
+    # ---- whole stage
+
+    def frame(self, frame: Frame, fuel: int | None = None):
+        """The stage on host planes; returns the output planes (and fuel left)."""
+        outs = [np.zeros(g, np.uint16) for g in frame.output_geometry()]
+        pin = abi.picture_of([np.ascontiguousarray(p) for p in frame.planes])
+        pout = abi.picture_of(outs)
+        f = C.c_uint64(fuel) if fuel is not None else None
+        self._check(self._lib.vaco_av1pf_frame(self.handle, C.byref(frame.desc), C.byref(pin), C.byref(pout),
+                                               C.byref(f) if f is not None else None))
+        if fuel is not None:
+            return outs, f.value
+        return outs
+
+    def frame_device(self, frame: Frame, in_ptrs, out_ptrs, stream: int | None = None):
+        """The stage on device planes (integer device addresses, dense rows)."""
+        geo_in = [(p.shape[0], p.shape[1]) for p in frame.planes]
+        geo_out = frame.output_geometry()
+        pin, pout = abi.Picture(), abi.Picture()
+        for i in range(frame.nplanes):
+            pin.plane[i] = abi.device_plane(in_ptrs[i], geo_in[i][1], geo_in[i][0])
+            pout.plane[i] = abi.device_plane(out_ptrs[i], geo_out[i][1], geo_out[i][0])
+        self._check(self._lib.vaco_av1pf_frame_device(self.handle, C.byref(frame.desc), C.byref(pin),
+                                                      C.byref(pout), C.c_void_p(stream or 0)))
+
+    def finish(self):
+        self._check(self._lib.vaco_av1pf_finish(self.handle))
+
+    # ---- kernel-level entries
+
+    def cdef_blocks(self, inputs: np.ndarray, params):
+        """find_direction + filter_block for n cases: -> (dirs, vars, outs[n, 64])."""
+        n = len(params)
+        inp = np.ascontiguousarray(inputs, dtype=np.uint16).reshape(n, 144)
+        par = (abi.FilterParams * max(1, n))(*params)
+        dirs = np.zeros(n, np.uint8)
+        vars_ = np.zeros(n, np.uint32)
+        outs = np.zeros((n, 64), np.uint16)
+        self._check(self._lib.vaco_av1pf_cdef_blocks(
+            self.handle, n, abi.u16_ptr(inp), par, dirs.ctypes.data_as(C.POINTER(C.c_uint8)),
+            vars_.ctypes.data_as(C.POINTER(C.c_uint32)), abi.u16_ptr(outs)))
+        return dirs, vars_, outs
+
+    def upscale_plane(self, inp: np.ndarray, visible_width: int, output_width: int, height: int,
+                      bit_depth: int) -> np.ndarray:
+        i = np.ascontiguousarray(inp, dtype=np.uint16)
+        out = np.zeros((max(height, 1), max(output_width, 1)), np.uint16)
+        cfg = abi.SrPlaneConfig(visible_width, output_width, height, bit_depth)
+        pi, po = abi.plane_of(i), abi.plane_of(out)
+        po.width, po.height = output_width, height
+        self._check(self._lib.vaco_av1pf_upscale_plane(self.handle, C.byref(pi), C.byref(cfg), C.byref(po)))
+        return out[:height, :output_width]
+
+    def restore_plane(self, before: np.ndarray, after: np.ndarray, cfg: abi.LrPlaneConfig, units) -> np.ndarray:
+        b = np.ascontiguousarray(before, dtype=np.uint16)
+        a = np.ascontiguousarray(after, dtype=np.uint16)
+        out = np.zeros((max(cfg.height, 1), max(cfg.width, 1)), np.uint16)
+        po = abi.plane_of(out)
+        po.width, po.height = cfg.width, cfg.height
+        arr = abi.units_array(units)
+        self._check(self._lib.vaco_av1pf_restore_plane(self.handle, C.byref(abi.plane_of(b)),
+                                                       C.byref(abi.plane_of(a)), C.byref(cfg), arr,
+                                                       len(units), C.byref(po)))
+        return out[: cfg.height, : cfg.width]
+
+
+# --------------------------------------------------------- host helpers --
+
+def adjust_strength(primary: int, variance: int) -> int:
+    return lib().vaco_av1pf_adjust_strength(primary, variance)
+
+
+def chroma_direction(direction: int, sub_x: bool, sub_y: bool) -> int:
+    return lib().vaco_av1pf_chroma_direction(direction, int(sub_x), int(sub_y))
+
+
+def lr_unit_counts(cfg: abi.LrPlaneConfig):
+    c, r = C.c_uint32(), C.c_uint32()
+    lib().vaco_av1pf_lr_unit_counts(C.byref(cfg), C.byref(c), C.byref(r))
+    return c.value, r.value
+
+
+def lr_unit_index(cfg: abi.LrPlaneConfig, x: int, y: int) -> int:
+    return lib().vaco_av1pf_lr_unit_index(C.byref(cfg), x, y)
+
+
+def skip8_from_mi(mi_skip: np.ndarray) -> np.ndarray:
+    m = np.ascontiguousarray(mi_skip, dtype=np.uint8)
+    rows, cols = m.shape
+    out = np.zeros((max(1, rows // 2), max(1, cols // 2)), np.uint8)
+    lib().vaco_av1pf_skip8_from_mi(m.ctypes.data_as(C.POINTER(C.c_uint8)), rows, cols,
+                                   out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out

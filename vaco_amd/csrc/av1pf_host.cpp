@@ -1,0 +1,1027 @@
+// av1pf_host.cpp - host runtime and C ABI (include/vaco_av1pf.h) of the
+// MI355X AV1 post-reconstruction filter stage.
+//
+// The per-frame entry replaces decode.rs:554-559 of matteopolak/vaco
+// (crates/codec/vaco-codec-av1/src/): pre-CDEF copy, apply_cdef,
+// apply_superres, apply_restoration. Device buffers are sized once at
+// vaco_av1pf_create; a frame call only fills a pinned staging arena with
+// work descriptors and side information, uploads it with one copy and
+// launches at most three kernels (CDEF, superres, restoration).
+//
+// Errors: the fast path validates parameters on the host and lets the
+// kernels flag any sample above the bit depth. Whenever a parameter is out
+// of range, fuel is limited, or a kernel raised the flag, the host replays
+// the reference's checks in the reference's order (av1pf_replay.cpp) so the
+// status and message are exactly the reference's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/vaco_av1pf.h"
+#include "av1pf_device.h"
+#include "av1pf_host.h"
+
+namespace av1pf {
+hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, hipStream_t s);
+hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s);
+hipError_t launch_lr(const LrPlane *planes, int n, uint32_t blocks, hipStream_t s);
+hipError_t launch_cdef_cases(const uint16_t *inputs, const void *params, uint8_t *dirs,
+                             uint32_t *vars, uint16_t *outs, uint32_t n, hipStream_t s);
+}  // namespace av1pf
+
+using namespace av1pf;
+
+// ------------------------------------------------------------ staging --
+
+namespace {
+
+constexpr int kSlots = 4;
+
+struct Arena {
+    uint8_t *h = nullptr;  // pinned host
+    uint8_t *d = nullptr;  // device mirror
+    size_t cap = 0, used = 0;
+    hipEvent_t done = nullptr;  // last upload from this slot finished
+    bool pending = false;
+
+    template <class T>
+    T *take(size_t n, T **dev)
+    {
+        size_t a = (used + 255) & ~size_t(255);
+        size_t bytes = n * sizeof(T);
+        if (a + bytes > cap)
+            return nullptr;
+        used = a + bytes;
+        *dev = reinterpret_cast<T *>(d + a);
+        return reinterpret_cast<T *>(h + a);
+    }
+};
+
+}  // namespace
+
+struct vaco_av1pf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t max_w = 0, max_h = 0;
+    size_t plane_cap = 0;          // samples per plane buffer
+    uint16_t *pool = nullptr;      // 5 images x 3 planes
+    uint16_t *img[5][3] = {};      // 0 input, 1 cdef, 2 upscaled, 3 upscaled pre-CDEF, 4 output
+    Arena slot[kSlots];
+    int next_slot = 0;
+    uint32_t *h_status = nullptr;  // pinned readback of frame status words
+    const char *err = "";
+    // deferred (device entry) frames awaiting status check
+    struct Pending {
+        const uint32_t *d_status;
+        vaco_av1pf_frame_desc desc;
+        vaco_av1pf_picture in;
+        std::vector<int8_t> idx;
+        std::vector<uint8_t> skip;
+        std::vector<vaco_av1pf_lr_unit> units[3];
+    };
+    std::vector<Pending> pending;
+};
+
+static vaco_av1pf_status fail(vaco_av1pf_ctx *ctx, vaco_av1pf_status st, const char *msg)
+{
+    if (ctx)
+        ctx->err = msg;
+    return st;
+}
+
+static vaco_av1pf_status hip_fail(vaco_av1pf_ctx *ctx, hipError_t e)
+{
+    return fail(ctx, VACO_AV1PF_DEVICE_ERROR, hipGetErrorString(e));
+}
+
+#define HIPCHECK(expr)                       \
+    do {                                     \
+        hipError_t e_ = (expr);              \
+        if (e_ != hipSuccess)                \
+            return hip_fail(ctx, e_);        \
+    } while (0)
+
+static vaco_av1pf_status take_slot(vaco_av1pf_ctx *ctx, Arena **out)
+{
+    Arena &a = ctx->slot[ctx->next_slot];
+    ctx->next_slot = (ctx->next_slot + 1) % kSlots;
+    if (a.pending) {
+        HIPCHECK(hipEventSynchronize(a.done));
+        a.pending = false;
+    }
+    a.used = 0;
+    *out = &a;
+    return VACO_AV1PF_OK;
+}
+
+static vaco_av1pf_status upload_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t s)
+{
+    if (a->used == 0)
+        return VACO_AV1PF_OK;
+    HIPCHECK(hipMemcpyAsync(a->d, a->h, a->used, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipEventRecord(a->done, s));
+    a->pending = true;
+    return VACO_AV1PF_OK;
+}
+
+// ------------------------------------------------------------ create --
+
+extern "C" uint32_t vaco_av1pf_abi_version(void) { return VACO_AV1PF_ABI_VERSION; }
+
+extern "C" const char *vaco_av1pf_last_error(const vaco_av1pf_ctx *ctx)
+{
+    return ctx ? ctx->err : "null context";
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_create(int device, uint32_t max_width, uint32_t max_height,
+                                               vaco_av1pf_ctx **out)
+{
+    if (!out || max_width == 0 || max_height == 0 || max_width > 65536 || max_height > 65536)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    *out = nullptr;
+    vaco_av1pf_ctx *ctx = new (std::nothrow) vaco_av1pf_ctx();
+    if (!ctx)
+        return VACO_AV1PF_DEVICE_ERROR;
+    auto bail = [&](hipError_t e) {
+        vaco_av1pf_destroy(ctx);
+        (void)e;
+        return VACO_AV1PF_DEVICE_ERROR;
+    };
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess)
+        return bail(e);
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess)
+        return bail(e);
+    // Mi-padded (multiple of 8) maximum geometry.
+    ctx->max_w = (max_width + 7) & ~7u;
+    ctx->max_h = (max_height + 7) & ~7u;
+    ctx->plane_cap = (size_t)ctx->max_w * ctx->max_h;
+    e = hipMalloc(&ctx->pool, ctx->plane_cap * 15 * sizeof(uint16_t));
+    if (e != hipSuccess)
+        return bail(e);
+    for (int i = 0; i < 5; i++)
+        for (int p = 0; p < 3; p++)
+            ctx->img[i][p] = ctx->pool + ctx->plane_cap * (size_t)(i * 3 + p);
+    const size_t blocks8 = (size_t)(ctx->max_w / 8 + 2) * (ctx->max_h / 8 + 2);
+    const size_t units = (size_t)(ctx->max_w / 32 + 2) * (ctx->max_h / 32 + 2) * 3;
+    const size_t cap = (1u << 20) + blocks8 * 2 + units * sizeof(LrUnit) + ctx->max_h * 64;
+    for (int i = 0; i < kSlots; i++) {
+        Arena &a = ctx->slot[i];
+        a.cap = cap;
+        e = hipHostMalloc((void **)&a.h, cap, hipHostMallocDefault);
+        if (e != hipSuccess)
+            return bail(e);
+        e = hipMalloc((void **)&a.d, cap);
+        if (e != hipSuccess)
+            return bail(e);
+        e = hipEventCreateWithFlags(&a.done, hipEventDisableTiming);
+        if (e != hipSuccess)
+            return bail(e);
+    }
+    e = hipHostMalloc((void **)&ctx->h_status, 64 * sizeof(uint32_t), hipHostMallocDefault);
+    if (e != hipSuccess)
+        return bail(e);
+    *out = ctx;
+    return VACO_AV1PF_OK;
+}
+
+extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
+{
+    if (!ctx)
+        return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream)
+        hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < kSlots; i++) {
+        if (ctx->slot[i].h)
+            hipHostFree(ctx->slot[i].h);
+        if (ctx->slot[i].d)
+            hipFree(ctx->slot[i].d);
+        if (ctx->slot[i].done)
+            hipEventDestroy(ctx->slot[i].done);
+    }
+    if (ctx->h_status)
+        hipHostFree(ctx->h_status);
+    if (ctx->pool)
+        hipFree(ctx->pool);
+    if (ctx->stream)
+        hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+// ------------------------------------------------------- host helpers --
+
+extern "C" uint16_t vaco_av1pf_adjust_strength(uint16_t primary, uint32_t variance)
+{
+    return host_adjust_strength(primary, variance);
+}
+
+extern "C" uint8_t vaco_av1pf_chroma_direction(uint8_t direction, uint8_t sub_x, uint8_t sub_y)
+{
+    return host_chroma_direction(direction, sub_x, sub_y);
+}
+
+extern "C" void vaco_av1pf_lr_unit_counts(const vaco_av1pf_lr_plane_config *cfg, uint32_t *cols,
+                                          uint32_t *rows)
+{
+    uint32_t c = 1, r = 1;
+    if (cfg)
+        lr_unit_counts(cfg->width, cfg->height, cfg->unit_size, &c, &r);
+    if (cols)
+        *cols = c;
+    if (rows)
+        *rows = r;
+}
+
+extern "C" uint32_t vaco_av1pf_lr_unit_index(const vaco_av1pf_lr_plane_config *cfg, uint32_t x,
+                                             uint32_t y)
+{
+    if (!cfg)
+        return 0;
+    uint32_t cols, rows;
+    lr_unit_counts(cfg->width, cfg->height, cfg->unit_size, &cols, &rows);
+    const uint32_t u = cfg->unit_size;
+    uint32_t row = u ? (y + (8u >> (cfg->subsampling_y ? 1 : 0))) / u : 0;
+    uint32_t col = u ? x / u : 0;
+    row = std::min(row, rows - 1);
+    col = std::min(col, cols - 1);
+    return row * cols + col;
+}
+
+extern "C" void vaco_av1pf_skip8_from_mi(const uint8_t *mi_skip, uint32_t mi_rows, uint32_t mi_cols,
+                                         uint8_t *skip8)
+{
+    if (!mi_skip || !skip8)
+        return;
+    for (uint32_t r = 0; r + 1 < mi_rows + 1 && r < mi_rows; r += 2)
+        for (uint32_t c = 0; c < mi_cols; c += 2) {
+            // decode.rs:892-896: all four cells present and skip (mi_at is
+            // None outside the grid, which counts as not skipped).
+            bool all = true;
+            for (uint32_t dr = 0; dr < 2; dr++)
+                for (uint32_t dc = 0; dc < 2; dc++) {
+                    const uint32_t rr = r + dr, cc = c + dc;
+                    if (rr >= mi_rows || cc >= mi_cols || !mi_skip[(size_t)rr * mi_cols + cc])
+                        all = false;
+                }
+            skip8[(size_t)(r / 2) * (mi_cols / 2 ? mi_cols / 2 : 1) + c / 2] = all ? 1 : 0;
+        }
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_frame_input_geometry(const vaco_av1pf_frame_desc *d,
+                                                             uint32_t width[3], uint32_t height[3])
+{
+    if (!d || !width || !height)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    FrameGeo g;
+    frame_geometry(*d, &g);
+    for (int p = 0; p < 3; p++) {
+        width[p] = g.in_w[p];
+        height[p] = g.in_h[p];
+    }
+    return VACO_AV1PF_OK;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_frame_output_geometry(const vaco_av1pf_frame_desc *d,
+                                                              uint32_t width[3],
+                                                              uint32_t height[3])
+{
+    if (!d || !width || !height)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    FrameGeo g;
+    frame_geometry(*d, &g);
+    for (int p = 0; p < 3; p++) {
+        width[p] = g.out_w[p];
+        height[p] = g.out_h[p];
+    }
+    return VACO_AV1PF_OK;
+}
+
+// ------------------------------------------------- per-frame planning --
+
+namespace {
+
+struct DevPlane {
+    uint16_t *data;
+    uint32_t stride;
+};
+
+// Row segments: intersections of restoration stripes (restoration.rs:218-231)
+// with unit rows (restoration.rs:124-133), clipped to the plane.
+void lr_row_segments(uint32_t height, uint32_t unit, bool ss_y, std::vector<LrRowSeg> &out)
+{
+    out.clear();
+    uint32_t ucols_dummy, urows;
+    lr_unit_counts(1, height, unit, &ucols_dummy, &urows);
+    const int sh = 64 >> (ss_y ? 1 : 0), off = 8 >> (ss_y ? 1 : 0);
+    std::vector<int> cuts;
+    cuts.push_back(0);
+    for (int y = sh - off; y < (int)height; y += sh)
+        cuts.push_back(y);
+    for (uint32_t r = 1; r < urows; r++)
+        cuts.push_back((int)(r * unit) - off);
+    cuts.push_back((int)height);
+    std::sort(cuts.begin(), cuts.end());
+    cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+    for (size_t i = 0; i + 1 < cuts.size(); i++) {
+        const int y0 = cuts[i], y1 = cuts[i + 1];
+        if (y0 < 0 || y1 <= y0)
+            continue;
+        LrRowSeg s;
+        s.y0 = y0;
+        s.h = y1 - y0;
+        const int stripe = (y0 + off) / sh;
+        s.stripe_start = stripe * sh - off;
+        s.stripe_end = s.stripe_start + sh - 1;
+        s.unit_row = std::min<uint32_t>(urows - 1, (uint32_t)(y0 + off) / unit);
+        out.push_back(s);
+    }
+}
+
+void lr_col_segments(uint32_t width, uint32_t unit, std::vector<LrColSeg> &out)
+{
+    out.clear();
+    uint32_t ucols, urows_dummy;
+    lr_unit_counts(width, 1, unit, &ucols, &urows_dummy);
+    for (uint32_t c = 0; c < ucols; c++) {
+        const uint32_t x0 = c * unit, x1 = c + 1 == ucols ? width : std::min(width, (c + 1) * unit);
+        for (uint32_t x = x0; x < x1; x += 64) {
+            LrColSeg s;
+            s.x0 = (int)x;
+            s.w = (int)std::min<uint32_t>(64, x1 - x);
+            s.unit_col = c;
+            out.push_back(s);
+        }
+    }
+}
+
+// Pre-CDEF rows that Source::get can read at stripe borders
+// (restoration.rs:236-241), for the upscaled pre-CDEF image.
+void lr_border_rows(const std::vector<LrRowSeg> &rows, uint32_t height, std::vector<uint32_t> &out)
+{
+    out.clear();
+    const int H = (int)height;
+    for (const LrRowSeg &s : rows) {
+        for (int y = s.y0 - 3; y < s.y0 + s.h + 3; y++) {
+            const int yc = std::min(std::max(y, 0), H - 1);
+            if (yc < s.stripe_start)
+                out.push_back((uint32_t)std::max(yc, s.stripe_start - 2));
+            else if (yc > s.stripe_end)
+                out.push_back((uint32_t)std::min(yc, s.stripe_end + 2));
+        }
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+}
+
+}  // namespace
+
+// Build descriptors for one frame into `a` and launch the stage on stream
+// `s`. `in`/`out` are device planes. `status` is the device status word.
+static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_desc &d,
+                                       const FrameGeo &g, const DevPlane in[3],
+                                       const DevPlane out[3], Arena *a, uint32_t *d_status,
+                                       hipStream_t s)
+{
+    const int np = g.nplanes;
+    DevPlane cdef_out[3], cur[3], pre[3];
+    for (int p = 0; p < np; p++) {
+        cdef_out[p] = in[p];
+        cur[p] = in[p];
+        pre[p] = in[p];
+    }
+    const bool final_cdef = !g.superres && !g.lr;
+
+    CdefFrame *hc = nullptr, *dc = nullptr;
+    SrJob jobs[6], *dj = nullptr;
+    int njobs = 0;
+    uint32_t sr_blocks = 0;
+    LrPlane lrp[3], *dl = nullptr;
+    int nlr = 0;
+    uint32_t lr_blocks = 0;
+    uint32_t cdef_blocks = 0;
+
+    // ---- CDEF (decode.rs:739-861)
+    if (g.cdef) {
+        hc = a->take<CdefFrame>(1, &dc);
+        int8_t *didx;
+        uint8_t *dskip;
+        const size_t nidx = (size_t)g.idx_rows * d.cdef_stride;
+        const size_t nskip = (size_t)(g.mi_rows / 2) * (g.mi_cols / 2);
+        int8_t *hidx = a->take<int8_t>(nidx, &didx);
+        uint8_t *hskip = a->take<uint8_t>(nskip, &dskip);
+        if (!hc || !hidx || !hskip)
+            return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
+        memcpy(hidx, d.cdef_idx, nidx);
+        memcpy(hskip, d.skip8, nskip);
+        CdefFrame &f = *hc;
+        memset(&f, 0, sizeof f);
+        for (int p = 0; p < np; p++) {
+            if (final_cdef)
+                cdef_out[p] = out[p];
+            else
+                cdef_out[p] = DevPlane{ctx->img[1][p], g.in_w[p]};
+            f.in[p] = in[p].data;
+            f.out[p] = cdef_out[p].data;
+            f.width[p] = g.in_w[p];
+            f.height[p] = g.in_h[p];
+            f.in_stride[p] = in[p].stride;
+            f.out_stride[p] = cdef_out[p].stride;
+        }
+        f.cdef_idx = didx;
+        f.skip8 = dskip;
+        f.idx_stride = d.cdef_stride;
+        f.skip_stride = g.mi_cols / 2;
+        f.sb_cols = (g.in_w[0] + 63) / 64;
+        f.sb_rows = (g.in_h[0] + 63) / 64;
+        f.status = d_status;
+        f.block_begin = 0;
+        f.bit_depth = d.bit_depth;
+        f.ss_x = d.subsampling_x;
+        f.ss_y = d.subsampling_y;
+        f.nplanes = (uint8_t)np;
+        f.damping = d.cdef.damping;
+        for (int i = 0; i < 8; i++) {
+            f.y_pri[i] = d.cdef.y[i].primary;
+            f.y_sec[i] = d.cdef.y[i].secondary;
+            f.uv_pri[i] = d.cdef.uv[i].primary;
+            f.uv_sec[i] = d.cdef.uv[i].secondary;
+        }
+        cdef_blocks = f.sb_cols * f.sb_rows;
+        for (int p = 0; p < np; p++)
+            cur[p] = cdef_out[p];
+    }
+
+    // ---- superres (decode.rs:865-890, superres.rs:226-274)
+    std::vector<LrRowSeg> lr_rows[3];
+    std::vector<LrColSeg> lr_cols[3];
+    if (g.lr) {
+        for (int p = 0; p < np; p++) {
+            if (d.lr_type[p] == VACO_AV1PF_RESTORE_NONE)
+                continue;
+            lr_row_segments(g.lr_h[p], d.lr_unit_size[p], p && d.subsampling_y, lr_rows[p]);
+            lr_col_segments(g.lr_w[p], d.lr_unit_size[p], lr_cols[p]);
+        }
+    }
+    if (g.superres) {
+        for (int img = 0; img < (g.lr ? 2 : 1); img++) {
+            for (int p = 0; p < np; p++) {
+                // img 0: the CDEF image; img 1: the pre-CDEF image, only at
+                // rows restoration reads across stripe borders.
+                const uint32_t *drows = nullptr;
+                uint32_t nrows = g.sr_h[p];
+                if (img == 1) {
+                    if (d.lr_type[p] == VACO_AV1PF_RESTORE_NONE)
+                        continue;
+                    std::vector<uint32_t> rows;
+                    lr_border_rows(lr_rows[p], g.lr_h[p], rows);
+                    if (rows.empty())
+                        continue;
+                    uint32_t *dr;
+                    uint32_t *hr = a->take<uint32_t>(rows.size(), &dr);
+                    if (!hr)
+                        return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
+                    memcpy(hr, rows.data(), rows.size() * 4);
+                    drows = dr;
+                    nrows = (uint32_t)rows.size();
+                }
+                DevPlane src = img == 0 ? cur[p] : in[p];
+                DevPlane dst;
+                if (img == 0)
+                    dst = g.lr ? DevPlane{ctx->img[2][p], g.sr_out_w[p]} : out[p];
+                else
+                    dst = DevPlane{ctx->img[3][p], g.sr_out_w[p]};
+                SrJob &j = jobs[njobs++];
+                memset(&j, 0, sizeof j);
+                j.in = src.data;
+                j.out = dst.data;
+                j.status = d_status;
+                j.rows = drows;
+                j.in_width = g.in_w[p];
+                j.in_stride = src.stride;
+                j.out_stride = dst.stride;
+                j.out_width = g.sr_out_w[p];
+                j.n_rows = nrows;
+                j.segs = (g.sr_out_w[p] + kSrSeg - 1) / kSrSeg;
+                j.step = g.sr_step[p];
+                j.start = g.sr_start[p];
+                j.block_begin = sr_blocks;
+                j.max_sample = (uint16_t)((1u << d.bit_depth) - 1);
+                sr_blocks += j.n_rows * j.segs;
+                if (img == 0)
+                    cur[p] = dst;
+                else
+                    pre[p] = dst;
+            }
+        }
+        if (njobs) {
+            SrJob *hj = a->take<SrJob>(njobs, &dj);
+            if (!hj)
+                return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
+            memcpy(hj, jobs, sizeof(SrJob) * njobs);
+        }
+    }
+
+    // ---- loop restoration (decode.rs:683-737, restoration.rs:178-207)
+    if (g.lr) {
+        for (int p = 0; p < np; p++) {
+            LrPlane &L = lrp[nlr];
+            memset(&L, 0, sizeof L);
+            const bool none = d.lr_type[p] == VACO_AV1PF_RESTORE_NONE;
+            std::vector<LrRowSeg> rows_none;
+            std::vector<LrColSeg> cols_none;
+            const std::vector<LrRowSeg> *rows = &lr_rows[p];
+            const std::vector<LrColSeg> *cols = &lr_cols[p];
+            if (none) {
+                // copy_visible_plane (decode.rs:673-681): 64x64 copy tiles
+                for (uint32_t y = 0; y < g.lr_h[p]; y += 64) {
+                    LrRowSeg r{(int)y, (int)std::min<uint32_t>(64, g.lr_h[p] - y), 0, 0, 0};
+                    rows_none.push_back(r);
+                }
+                for (uint32_t x = 0; x < g.lr_w[p]; x += 64) {
+                    LrColSeg c{(int)x, (int)std::min<uint32_t>(64, g.lr_w[p] - x), 0};
+                    cols_none.push_back(c);
+                }
+                rows = &rows_none;
+                cols = &cols_none;
+            }
+            LrRowSeg *drs;
+            LrColSeg *dcs;
+            LrRowSeg *hrs = a->take<LrRowSeg>(rows->size(), &drs);
+            LrColSeg *hcs = a->take<LrColSeg>(cols->size(), &dcs);
+            if (!hrs || !hcs)
+                return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
+            memcpy(hrs, rows->data(), rows->size() * sizeof(LrRowSeg));
+            memcpy(hcs, cols->data(), cols->size() * sizeof(LrColSeg));
+            L.units = nullptr;
+            if (!none) {
+                LrUnit *du;
+                LrUnit *hu = a->take<LrUnit>(d.lr_unit_count[p], &du);
+                if (!hu)
+                    return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
+                static_assert(sizeof(LrUnit) == sizeof(vaco_av1pf_lr_unit), "unit layout");
+                memcpy(hu, d.lr_units[p], sizeof(LrUnit) * d.lr_unit_count[p]);
+                L.units = du;
+                uint32_t ucols, urows;
+                lr_unit_counts(g.lr_w[p], g.lr_h[p], d.lr_unit_size[p], &ucols, &urows);
+                L.unit_cols = ucols;
+            }
+            L.before = pre[p].data;
+            L.after = cur[p].data;
+            L.out = out[p].data;
+            L.rows = drs;
+            L.cols = dcs;
+            L.status = d_status;
+            L.before_stride = pre[p].stride;
+            L.after_stride = cur[p].stride;
+            L.out_stride = out[p].stride;
+            L.width = g.lr_w[p];
+            L.height = g.lr_h[p];
+            L.n_rows = (uint32_t)rows->size();
+            L.n_cols = (uint32_t)cols->size();
+            L.block_begin = lr_blocks;
+            L.bit_depth = d.bit_depth;
+            L.ss_y = (uint8_t)(p && d.subsampling_y);
+            lr_blocks += L.n_rows * L.n_cols;
+            nlr++;
+        }
+        LrPlane *hl = a->take<LrPlane>(nlr, &dl);
+        if (!hl)
+            return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
+        memcpy(hl, lrp, sizeof(LrPlane) * nlr);
+    }
+
+    vaco_av1pf_status st = upload_slot(ctx, a, s);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    if (!g.cdef && !g.superres && !g.lr) {
+        // Nothing runs: the stage's output is its input (decode.rs:741-748).
+        for (int p = 0; p < np; p++)
+            HIPCHECK(hipMemcpy2DAsync(out[p].data, out[p].stride * 2, in[p].data, in[p].stride * 2,
+                                      g.in_w[p] * 2, g.in_h[p], hipMemcpyDeviceToDevice, s));
+        return VACO_AV1PF_OK;
+    }
+    if (g.cdef)
+        HIPCHECK(launch_cdef(dc, 1, cdef_blocks, s));
+    if (njobs)
+        HIPCHECK(launch_superres(dj, njobs, sr_blocks, s));
+    if (nlr)
+        HIPCHECK(launch_lr(dl, nlr, lr_blocks, s));
+    return VACO_AV1PF_OK;
+}
+
+// Validate the caller's planes against the geometry the reference implies.
+static vaco_av1pf_status check_planes(vaco_av1pf_ctx *ctx, const FrameGeo &g, uint32_t cdef_stride,
+                                      const vaco_av1pf_picture *in, const vaco_av1pf_picture *out)
+{
+    auto d_stride_short = [&](const FrameGeo &gg) { return cdef_stride < (gg.mi_cols + 15) / 16; };
+    for (int p = 0; p < g.nplanes; p++) {
+        const vaco_av1pf_plane &i = in->plane[p], &o = out->plane[p];
+        if (!i.data || !o.data)
+            return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "missing plane");
+        if (i.width != g.in_w[p] || i.height != g.in_h[p] || i.stride < i.width)
+            return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "input plane geometry differs from the Mi-padded frame");
+        if (o.width != g.out_w[p] || o.height != g.out_h[p] || o.stride < o.width)
+            return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "output plane geometry differs from the stage output");
+        if ((size_t)g.in_w[p] > ctx->max_w || (size_t)g.in_h[p] > ctx->max_h ||
+            (size_t)g.out_w[p] > ctx->max_w || (size_t)g.out_h[p] > ctx->max_h)
+            return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "frame exceeds the context's capacity");
+    }
+    if (g.cdef && (!g.has_idx || !g.has_skip || d_stride_short(g)))
+        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "CDEF side information missing");
+    return VACO_AV1PF_OK;
+}
+
+static void host_planes_of(const vaco_av1pf_picture *pic, int np, HostPlane hp[3])
+{
+    for (int p = 0; p < 3; p++) {
+        hp[p].data = p < np ? pic->plane[p].data : nullptr;
+        hp[p].width = p < np ? pic->plane[p].width : 0;
+        hp[p].height = p < np ? pic->plane[p].height : 0;
+        hp[p].stride = p < np ? pic->plane[p].stride : 0;
+    }
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_desc *desc,
+                                              const vaco_av1pf_picture *in, vaco_av1pf_picture *out,
+                                              uint64_t *fuel)
+{
+    if (!ctx || !desc || !in || !out)
+        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "null argument");
+    ctx->err = "";
+    FrameGeo g;
+    frame_geometry(*desc, &g);
+    vaco_av1pf_status st = check_planes(ctx, g, desc->cdef_stride, in, out);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    HIPCHECK(hipSetDevice(ctx->device));
+    HostPlane hin[3];
+    host_planes_of(in, g.nplanes, hin);
+    // Slow path first when the reference's checks could fail on parameters
+    // or fuel alone: replay them in the reference's order.
+    const bool fuel_short = fuel && required_fuel(*desc, g) > *fuel;
+    if (!params_clean(*desc, g) || fuel_short) {
+        const char *msg = "";
+        vaco_av1pf_status r = replay_checks(*desc, g, hin, fuel, &msg);
+        if (r != VACO_AV1PF_OK)
+            return fail(ctx, r, msg);
+        fuel = nullptr;  // replay already charged it
+    }
+    Arena *a;
+    if ((st = take_slot(ctx, &a)) != VACO_AV1PF_OK)
+        return st;
+    uint32_t *dstat;
+    uint32_t *hstat = a->take<uint32_t>(1, &dstat);
+    *hstat = 0;
+    DevPlane din[3], dout[3];
+    for (int p = 0; p < g.nplanes; p++) {
+        din[p] = DevPlane{ctx->img[0][p], g.in_w[p]};
+        dout[p] = DevPlane{ctx->img[4][p], g.out_w[p]};
+        HIPCHECK(hipMemcpy2DAsync(din[p].data, din[p].stride * 2, in->plane[p].data,
+                                  in->plane[p].stride * 2, g.in_w[p] * 2, g.in_h[p],
+                                  hipMemcpyHostToDevice, ctx->stream));
+    }
+    st = enqueue_frame(ctx, *desc, g, din, dout, a, dstat, ctx->stream);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    for (int p = 0; p < g.nplanes; p++)
+        HIPCHECK(hipMemcpy2DAsync(out->plane[p].data, out->plane[p].stride * 2, dout[p].data,
+                                  dout[p].stride * 2, g.out_w[p] * 2, g.out_h[p],
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(ctx->h_status, dstat, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_status[0] & kFlagSampleRange) {
+        const char *msg = "";
+        vaco_av1pf_status r = replay_checks(*desc, g, hin, fuel, &msg);
+        if (r != VACO_AV1PF_OK)
+            return fail(ctx, r, msg);
+    } else if (fuel) {
+        *fuel -= required_fuel(*desc, g);
+    }
+    return VACO_AV1PF_OK;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
+                                                     const vaco_av1pf_frame_desc *desc,
+                                                     const vaco_av1pf_picture *in,
+                                                     vaco_av1pf_picture *out, void *stream)
+{
+    if (!ctx || !desc || !in || !out)
+        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "null argument");
+    ctx->err = "";
+    FrameGeo g;
+    frame_geometry(*desc, &g);
+    vaco_av1pf_status st = check_planes(ctx, g, desc->cdef_stride, in, out);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    if (!params_clean(*desc, g)) {
+        // Parameter errors need the samples to order them exactly.
+        std::vector<uint16_t> host[3];
+        vaco_av1pf_picture hpic = *in;
+        HIPCHECK(hipStreamSynchronize(s));
+        for (int p = 0; p < g.nplanes; p++) {
+            host[p].resize((size_t)g.in_w[p] * g.in_h[p]);
+            HIPCHECK(hipMemcpy2D(host[p].data(), g.in_w[p] * 2, in->plane[p].data,
+                                 in->plane[p].stride * 2, g.in_w[p] * 2, g.in_h[p],
+                                 hipMemcpyDeviceToHost));
+            hpic.plane[p].data = host[p].data();
+            hpic.plane[p].stride = g.in_w[p];
+        }
+        HostPlane hin[3];
+        host_planes_of(&hpic, g.nplanes, hin);
+        const char *msg = "";
+        vaco_av1pf_status r = replay_checks(*desc, g, hin, nullptr, &msg);
+        if (r != VACO_AV1PF_OK)
+            return fail(ctx, r, msg);
+    }
+    Arena *a;
+    if ((st = take_slot(ctx, &a)) != VACO_AV1PF_OK)
+        return st;
+    uint32_t *dstat;
+    uint32_t *hstat = a->take<uint32_t>(1, &dstat);
+    *hstat = 0;
+    DevPlane din[3], dout[3];
+    for (int p = 0; p < g.nplanes; p++) {
+        din[p] = DevPlane{in->plane[p].data, in->plane[p].stride};
+        dout[p] = DevPlane{out->plane[p].data, out->plane[p].stride};
+    }
+    st = enqueue_frame(ctx, *desc, g, din, dout, a, dstat, s);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    // Keep what the deferred check needs.
+    vaco_av1pf_ctx::Pending pd;
+    pd.d_status = dstat;
+    pd.desc = *desc;
+    pd.in = *in;
+    if (g.cdef) {
+        pd.idx.assign(desc->cdef_idx, desc->cdef_idx + (size_t)g.idx_rows * desc->cdef_stride);
+        pd.skip.assign(desc->skip8, desc->skip8 + (size_t)(g.mi_rows / 2) * (g.mi_cols / 2));
+        pd.desc.cdef_idx = nullptr;
+        pd.desc.skip8 = nullptr;
+    }
+    for (int p = 0; p < 3; p++) {
+        if (desc->lr_units[p] && desc->lr_unit_count[p])
+            pd.units[p].assign(desc->lr_units[p], desc->lr_units[p] + desc->lr_unit_count[p]);
+        pd.desc.lr_units[p] = nullptr;
+    }
+    if (ctx->pending.size() >= 48) {
+        st = vaco_av1pf_finish(ctx);
+        if (st != VACO_AV1PF_OK)
+            return st;
+    }
+    ctx->pending.push_back(std::move(pd));
+    (void)s;
+    return VACO_AV1PF_OK;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_finish(vaco_av1pf_ctx *ctx)
+{
+    if (!ctx)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(hipDeviceSynchronize());
+    vaco_av1pf_status result = VACO_AV1PF_OK;
+    for (auto &pd : ctx->pending) {
+        uint32_t flag = 0;
+        HIPCHECK(hipMemcpy(&flag, pd.d_status, 4, hipMemcpyDeviceToHost));
+        if (!(flag & kFlagSampleRange) || result != VACO_AV1PF_OK)
+            continue;
+        vaco_av1pf_frame_desc d = pd.desc;
+        d.cdef_idx = pd.idx.empty() ? nullptr : pd.idx.data();
+        d.skip8 = pd.skip.empty() ? nullptr : pd.skip.data();
+        for (int p = 0; p < 3; p++)
+            d.lr_units[p] = pd.units[p].empty() ? nullptr : pd.units[p].data();
+        FrameGeo g;
+        frame_geometry(d, &g);
+        std::vector<uint16_t> host[3];
+        vaco_av1pf_picture hpic = pd.in;
+        for (int p = 0; p < g.nplanes; p++) {
+            host[p].resize((size_t)g.in_w[p] * g.in_h[p]);
+            HIPCHECK(hipMemcpy2D(host[p].data(), g.in_w[p] * 2, pd.in.plane[p].data,
+                                 pd.in.plane[p].stride * 2, g.in_w[p] * 2, g.in_h[p],
+                                 hipMemcpyDeviceToHost));
+            hpic.plane[p].data = host[p].data();
+            hpic.plane[p].stride = g.in_w[p];
+        }
+        HostPlane hin[3];
+        host_planes_of(&hpic, g.nplanes, hin);
+        const char *msg = "";
+        vaco_av1pf_status r = replay_checks(d, g, hin, nullptr, &msg);
+        if (r != VACO_AV1PF_OK)
+            result = fail(ctx, r, msg);
+    }
+    ctx->pending.clear();
+    return result;
+}
+
+// ------------------------------------------------- kernel-level entries --
+
+extern "C" vaco_av1pf_status vaco_av1pf_cdef_blocks(vaco_av1pf_ctx *ctx, uint32_t n,
+                                                    const uint16_t *inputs,
+                                                    const vaco_av1pf_filter_params *params,
+                                                    uint8_t *dirs, uint32_t *vars, uint16_t *out)
+{
+    if (!ctx || (n && (!inputs || !params || !dirs || !vars || !out)))
+        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "null argument");
+    ctx->err = "";
+    // cdef.rs:79-87 and 142-154, per case in order.
+    for (uint32_t i = 0; i < n; i++) {
+        const char *msg = nullptr;
+        if (!check_find_direction(inputs + (size_t)i * 144, params[i].bit_depth, &msg) ||
+            !check_filter_block(inputs + (size_t)i * 144, params[i], &msg))
+            return fail(ctx, VACO_AV1PF_INVALID_DATA, msg);
+    }
+    if (n == 0)
+        return VACO_AV1PF_OK;
+    HIPCHECK(hipSetDevice(ctx->device));
+    static_assert(sizeof(vaco_av1pf_filter_params) == 16, "filter params layout");
+    const size_t bin = (size_t)n * 144 * 2, bpar = (size_t)n * 16, bout = (size_t)n * 64 * 2;
+    uint8_t *buf = nullptr;
+    HIPCHECK(hipMalloc(&buf, bin + bpar + bout + n * 4 + n));
+    uint16_t *din = (uint16_t *)buf;
+    void *dpar = buf + bin;
+    uint16_t *dout = (uint16_t *)(buf + bin + bpar);
+    uint32_t *dvar = (uint32_t *)(buf + bin + bpar + bout);
+    uint8_t *ddir = buf + bin + bpar + bout + n * 4;
+    hipError_t e = hipMemcpyAsync(din, inputs, bin, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dpar, params, bpar, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = launch_cdef_cases(din, dpar, ddir, dvar, dout, n, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dout, bout, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(vars, dvar, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dirs, ddir, n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    hipFree(buf);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e);
+    return VACO_AV1PF_OK;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_upscale_plane(vaco_av1pf_ctx *ctx, const vaco_av1pf_plane *in,
+                                                      const vaco_av1pf_sr_plane_config *cfg,
+                                                      vaco_av1pf_plane *out)
+{
+    if (!ctx || !in || !cfg || !out || !in->data || !out->data)
+        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "null argument");
+    ctx->err = "";
+    const char *msg = nullptr;
+    if (!check_superres_plane(*cfg, in->width, in->height, &msg))  // superres.rs:128-143
+        return fail(ctx, VACO_AV1PF_INVALID_DATA, msg);
+    if (out->width != cfg->output_width || out->height != cfg->height || out->stride < out->width ||
+        in->stride < in->width)
+        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "plane geometry");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const size_t nin = (size_t)in->width * cfg->height, nout = (size_t)cfg->output_width * cfg->height;
+    uint8_t *buf = nullptr;
+    HIPCHECK(hipMalloc(&buf, (nin + nout) * 2 + 512));
+    uint16_t *din = (uint16_t *)buf, *dout = din + nin;
+    uint8_t *tail = (uint8_t *)(dout + nout);
+    SrJob *dj = (SrJob *)(((uintptr_t)tail + 255) & ~(uintptr_t)255);
+    uint32_t *dst = (uint32_t *)(dj + 1);
+    SrJob j;
+    memset(&j, 0, sizeof j);
+    int32_t step, start;
+    superres_phase(cfg->visible_width, cfg->output_width, &step, &start);
+    j.in = din;
+    j.out = dout;
+    j.status = dst;
+    j.rows = nullptr;
+    j.in_width = in->width;
+    j.in_stride = in->width;
+    j.out_stride = cfg->output_width;
+    j.out_width = cfg->output_width;
+    j.n_rows = cfg->height;
+    j.segs = (cfg->output_width + kSrSeg - 1) / kSrSeg;
+    j.step = step;
+    j.start = start;
+    j.block_begin = 0;
+    j.max_sample = (uint16_t)((1u << cfg->bit_depth) - 1);
+    uint32_t zero = 0, flag = 0;
+    hipError_t e = hipMemcpy2DAsync(din, in->width * 2, in->data, in->stride * 2, in->width * 2,
+                                    cfg->height, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dj, &j, sizeof j, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = launch_superres(dj, 1, j.n_rows * j.segs, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(out->data, out->stride * 2, dout, cfg->output_width * 2,
+                             cfg->output_width * 2, cfg->height, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&flag, dst, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    hipFree(buf);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e);
+    if (flag & kFlagSampleRange)  // superres.rs:201-203
+        return fail(ctx, VACO_AV1PF_INVALID_DATA, "AV1 superres sample exceeds bit depth");
+    return VACO_AV1PF_OK;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
+                                                      const vaco_av1pf_plane *before,
+                                                      const vaco_av1pf_plane *after,
+                                                      const vaco_av1pf_lr_plane_config *cfg,
+                                                      const vaco_av1pf_lr_unit *units,
+                                                      uint32_t n_units, vaco_av1pf_plane *out)
+{
+    if (!ctx || !before || !after || !cfg || !out || !before->data || !after->data || !out->data ||
+        (n_units && !units))
+        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "null argument");
+    ctx->err = "";
+    HostPlane hb{before->data, before->width, before->height, before->stride};
+    HostPlane ha{after->data, after->width, after->height, after->stride};
+    const char *msg = nullptr;
+    if (!check_restore_plane(*cfg, hb, ha, units, n_units, &msg))  // restoration.rs:135-168
+        return fail(ctx, VACO_AV1PF_INVALID_DATA, msg);
+    if (out->width != cfg->width || out->height != cfg->height || out->stride < out->width)
+        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "plane geometry");
+    HIPCHECK(hipSetDevice(ctx->device));
+    std::vector<LrRowSeg> rows;
+    std::vector<LrColSeg> cols;
+    lr_row_segments(cfg->height, cfg->unit_size, cfg->subsampling_y != 0, rows);
+    lr_col_segments(cfg->width, cfg->unit_size, cols);
+    const size_t nb = (size_t)before->height * before->width, na = (size_t)after->height * after->width;
+    const size_t no = (size_t)cfg->width * cfg->height;
+    const size_t bytes_tail = rows.size() * sizeof(LrRowSeg) + cols.size() * sizeof(LrColSeg) +
+                              n_units * sizeof(LrUnit) + sizeof(LrPlane) + 8 * 256;
+    uint8_t *buf = nullptr;
+    HIPCHECK(hipMalloc(&buf, (nb + na + no) * 2 + bytes_tail));
+    uint16_t *db = (uint16_t *)buf, *da = db + nb, *dout = da + na;
+    uint8_t *p8 = (uint8_t *)(((uintptr_t)(dout + no) + 255) & ~(uintptr_t)255);
+    LrPlane *dl = (LrPlane *)p8;
+    p8 += (sizeof(LrPlane) + 255) & ~size_t(255);
+    LrRowSeg *drs = (LrRowSeg *)p8;
+    p8 += (rows.size() * sizeof(LrRowSeg) + 255) & ~size_t(255);
+    LrColSeg *dcs = (LrColSeg *)p8;
+    p8 += (cols.size() * sizeof(LrColSeg) + 255) & ~size_t(255);
+    LrUnit *du = (LrUnit *)p8;
+    p8 += (n_units * sizeof(LrUnit) + 255) & ~size_t(255);
+    uint32_t *dst = (uint32_t *)p8;
+    LrPlane L;
+    memset(&L, 0, sizeof L);
+    L.before = db;
+    L.after = da;
+    L.out = dout;
+    L.rows = drs;
+    L.cols = dcs;
+    L.units = du;
+    L.status = dst;
+    L.before_stride = before->width;
+    L.after_stride = after->width;
+    L.out_stride = cfg->width;
+    L.width = cfg->width;
+    L.height = cfg->height;
+    L.n_rows = (uint32_t)rows.size();
+    L.n_cols = (uint32_t)cols.size();
+    uint32_t ucols, urows;
+    lr_unit_counts(cfg->width, cfg->height, cfg->unit_size, &ucols, &urows);
+    L.unit_cols = ucols;
+    L.block_begin = 0;
+    L.bit_depth = cfg->bit_depth;
+    L.ss_y = cfg->subsampling_y ? 1 : 0;
+    uint32_t zero = 0;
+    hipError_t e = hipMemcpy2DAsync(db, before->width * 2, before->data, before->stride * 2,
+                                    before->width * 2, before->height, hipMemcpyHostToDevice,
+                                    ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(da, after->width * 2, after->data, after->stride * 2, after->width * 2,
+                             after->height, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dl, &L, sizeof L, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(drs, rows.data(), rows.size() * sizeof(LrRowSeg), hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dcs, cols.data(), cols.size() * sizeof(LrColSeg), hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess && n_units)
+        e = hipMemcpyAsync(du, units, n_units * sizeof(LrUnit), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = launch_lr(dl, 1, L.n_rows * L.n_cols, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(out->data, out->stride * 2, dout, cfg->width * 2, cfg->width * 2,
+                             cfg->height, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    hipFree(buf);
+    if (e != hipSuccess)
+        return hip_fail(ctx, e);
+    return VACO_AV1PF_OK;
+}
