@@ -259,6 +259,27 @@ vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
                                            uint32_t n_units,
                                            vaco_av1pf_plane *out);
 
+/* --------------------------------------------------------- profiling --- */
+
+/* Kernel identifiers of the profiling counters. */
+enum {
+    VACO_AV1PF_KERNEL_CDEF = 0,
+    VACO_AV1PF_KERNEL_SUPERRES = 1,
+    VACO_AV1PF_KERNEL_RESTORATION = 2,
+    VACO_AV1PF_KERNEL_FUSED = 3,
+    VACO_AV1PF_KERNEL_COUNT = 4
+};
+
+/* No reference counterpart (measurement hook). While enabled, every kernel
+ * launch of the frame entries is bracketed by HIP events on the stream it
+ * runs on; vaco_av1pf_profile_read() synchronises the context and returns,
+ * per kernel id, the summed launch durations in milliseconds and the launch
+ * counts since the last read. Enabling allocates the event pool. */
+vaco_av1pf_status vaco_av1pf_profile(vaco_av1pf_ctx *ctx, int enable);
+vaco_av1pf_status vaco_av1pf_profile_read(vaco_av1pf_ctx *ctx,
+                                          double ms[VACO_AV1PF_KERNEL_COUNT],
+                                          uint64_t launches[VACO_AV1PF_KERNEL_COUNT]);
+
 /* ------------------------------------------------ host-only helpers --- */
 
 /* cdef::adjust_strength, cdef.rs:208-220. */

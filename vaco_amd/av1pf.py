@@ -109,7 +109,9 @@ def lib():
         L.vaco_av1pf_gen_frame.argtypes = [P(GenConfig), C.c_uint32, P(abi.FrameDesc), P(abi.Picture),
                                            P(C.c_int8), P(C.c_uint8), P(abi.LrUnit), P(abi.LrUnit),
                                            P(abi.LrUnit)]
-        for name in ("vaco_av1pf_create", "vaco_av1pf_frame", "vaco_av1pf_frame_device", "vaco_av1pf_finish",
+        L.vaco_av1pf_profile.argtypes = [ctxp, C.c_int]
+        L.vaco_av1pf_profile_read.argtypes = [ctxp, P(C.c_double), P(C.c_uint64)]
+        for name in ("vaco_av1pf_profile", "vaco_av1pf_profile_read", "vaco_av1pf_create", "vaco_av1pf_frame", "vaco_av1pf_frame_device", "vaco_av1pf_finish",
                      "vaco_av1pf_cdef_blocks", "vaco_av1pf_upscale_plane", "vaco_av1pf_restore_plane",
                      "vaco_av1pf_frame_output_geometry", "vaco_av1pf_frame_input_geometry",
                      "vaco_av1pf_gen_get_layout", "vaco_av1pf_gen_frame"):
@@ -230,6 +232,20 @@ class Context:
 
     def finish(self):
         self._check(self._lib.vaco_av1pf_finish(self.handle))
+
+    # ---- kernel timing
+
+    KERNELS = ("cdef", "superres", "restoration", "fused")
+
+    def profile(self, enable: bool = True):
+        self._check(self._lib.vaco_av1pf_profile(self.handle, 1 if enable else 0))
+
+    def profile_read(self):
+        """{kernel: (total_ms, launches)} since the last read (synchronises)."""
+        ms = (C.c_double * 4)()
+        n = (C.c_uint64 * 4)()
+        self._check(self._lib.vaco_av1pf_profile_read(self.handle, ms, n))
+        return {k: (ms[i], n[i]) for i, k in enumerate(self.KERNELS)}
 
     # ---- kernel-level entries
 

@@ -84,7 +84,22 @@ struct vaco_av1pf_ctx {
         std::vector<vaco_av1pf_lr_unit> units[3];
     };
     std::vector<Pending> pending;
+    // kernel timing (vaco_av1pf_profile)
+    struct ProfRec {
+        int kernel;
+        uint32_t a, b;
+    };
+    bool prof = false;
+    std::vector<hipEvent_t> prof_pool;
+    uint32_t prof_next = 0;
+    std::vector<ProfRec> prof_recs;
+    double prof_ms[VACO_AV1PF_KERNEL_COUNT] = {};
+    uint64_t prof_n[VACO_AV1PF_KERNEL_COUNT] = {};
 };
+
+static hipError_t profile_collect(vaco_av1pf_ctx *ctx);
+template <class F>
+static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, hipStream_t s, F launch);
 
 static vaco_av1pf_status fail(vaco_av1pf_ctx *ctx, vaco_av1pf_status st, const char *msg)
 {
@@ -208,6 +223,8 @@ extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
     }
     if (ctx->h_status)
         hipHostFree(ctx->h_status);
+    for (auto ev : ctx->prof_pool)
+        hipEventDestroy(ev);
     if (ctx->pool)
         hipFree(ctx->pool);
     if (ctx->stream)
@@ -608,11 +625,91 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
         return VACO_AV1PF_OK;
     }
     if (g.cdef)
-        HIPCHECK(launch_cdef(dc, 1, cdef_blocks, s));
+        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, s,
+                              [&] { return launch_cdef(dc, 1, cdef_blocks, s); }));
     if (njobs)
-        HIPCHECK(launch_superres(dj, njobs, sr_blocks, s));
+        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES, s,
+                              [&] { return launch_superres(dj, njobs, sr_blocks, s); }));
     if (nlr)
-        HIPCHECK(launch_lr(dl, nlr, lr_blocks, s));
+        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION, s,
+                              [&] { return launch_lr(dl, nlr, lr_blocks, s); }));
+    return VACO_AV1PF_OK;
+}
+
+// ---------------------------------------------------------- profiling --
+
+static hipError_t profile_collect(vaco_av1pf_ctx *ctx)
+{
+    if (ctx->prof_recs.empty())
+        return hipSuccess;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess)
+        return e;
+    for (const auto &r : ctx->prof_recs) {
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, ctx->prof_pool[r.a], ctx->prof_pool[r.b]);
+        if (e != hipSuccess)
+            return e;
+        ctx->prof_ms[r.kernel] += ms;
+        ctx->prof_n[r.kernel] += 1;
+    }
+    ctx->prof_recs.clear();
+    ctx->prof_next = 0;
+    return hipSuccess;
+}
+
+template <class F>
+static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, hipStream_t s, F launch)
+{
+    if (!ctx->prof)
+        return launch();
+    if (ctx->prof_next + 2 > ctx->prof_pool.size()) {
+        hipError_t e = profile_collect(ctx);
+        if (e != hipSuccess)
+            return e;
+    }
+    const uint32_t a = ctx->prof_next++, b = ctx->prof_next++;
+    hipError_t e = hipEventRecord(ctx->prof_pool[a], s);
+    if (e == hipSuccess)
+        e = launch();
+    if (e == hipSuccess)
+        e = hipEventRecord(ctx->prof_pool[b], s);
+    if (e == hipSuccess)
+        ctx->prof_recs.push_back({kernel, a, b});
+    return e;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_profile(vaco_av1pf_ctx *ctx, int enable)
+{
+    if (!ctx)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    HIPCHECK(hipSetDevice(ctx->device));
+    if (enable && ctx->prof_pool.empty()) {
+        ctx->prof_pool.resize(2048);
+        for (auto &ev : ctx->prof_pool)
+            HIPCHECK(hipEventCreate(&ev));
+    }
+    if (!enable)
+        HIPCHECK(profile_collect(ctx));
+    ctx->prof = enable != 0;
+    return VACO_AV1PF_OK;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_profile_read(vaco_av1pf_ctx *ctx, double ms[VACO_AV1PF_KERNEL_COUNT],
+                                                     uint64_t launches[VACO_AV1PF_KERNEL_COUNT])
+{
+    if (!ctx)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(profile_collect(ctx));
+    for (int k = 0; k < VACO_AV1PF_KERNEL_COUNT; k++) {
+        if (ms)
+            ms[k] = ctx->prof_ms[k];
+        if (launches)
+            launches[k] = ctx->prof_n[k];
+        ctx->prof_ms[k] = 0;
+        ctx->prof_n[k] = 0;
+    }
     return VACO_AV1PF_OK;
 }
 
