@@ -50,9 +50,7 @@ def main():
                                    "stage shards by frame with no collectives (see DESIGN.md); run --gpus 1"}))
         sys.exit(2)
 
-    import torch
-
-    from vaco_amd import av1pf, workloads
+    from vaco_amd import av1pf, hip, workloads
 
     cfg = workloads.cfg1()
     frames = [av1pf.generate(cfg, k) for k in range(args.frames)]
@@ -60,11 +58,11 @@ def main():
     ctx = av1pf.Context(1920, 1088)
     dev_in, dev_out = [], []
     for f in frames:
-        dev_in.append([torch.from_numpy(p.view(np.int16)).cuda() for p in f.planes])
-        dev_out.append([torch.empty(g, dtype=torch.int16, device="cuda") for g in geo_out])
-    in_ptrs = [[t.data_ptr() for t in ts] for ts in dev_in]
-    out_ptrs = [[t.data_ptr() for t in ts] for ts in dev_out]
-    torch.cuda.synchronize()
+        dev_in.append([hip.DeviceArray.from_host(p) for p in f.planes])
+        dev_out.append([hip.DeviceArray(g) for g in geo_out])
+    in_ptrs = [[t.ptr for t in ts] for ts in dev_in]
+    out_ptrs = [[t.ptr for t in ts] for ts in dev_out]
+    hip.synchronize()
 
     def step(i):
         k = i % len(frames)
@@ -73,14 +71,14 @@ def main():
     for i in range(args.warmup):
         step(i)
     ctx.finish()
-    torch.cuda.synchronize()
+    hip.synchronize()
 
     # Timed region: K steps, device synchronised on both sides.
     t0 = time.perf_counter()
     for i in range(args.steps):
         step(i)
     ctx.finish()
-    torch.cuda.synchronize()
+    hip.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     mpix = frames[0].output_luma_pixels / 1e6
@@ -100,7 +98,7 @@ def main():
     exact = True
     cpu_times = []
     for k, f in enumerate(frames):
-        got = [t.cpu().numpy().view(np.uint16) for t in dev_out[k]]
+        got = [t.download() for t in dev_out[k]]
         t = time.perf_counter()
         want = O.frame(f.desc, f.planes)
         cpu_times.append(time.perf_counter() - t)

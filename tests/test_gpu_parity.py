@@ -128,18 +128,31 @@ def test_gpu_frame_cfg2_4k_superres_matches_oracle(ctx):
 
 
 def test_gpu_frame_device_entry_matches_host_entry(ctx):
-    import torch
+    from vaco_amd import hip
     say("device entry vs host entry, cfg1 frames 0-1")
     for k in range(2):
         f = av1pf.generate(workloads.cfg1(), k)
         want = ctx.frame(f)
-        dev_in = [torch.from_numpy(p.view(np.int16)).cuda() for p in f.planes]
-        dev_out = [torch.empty(g, dtype=torch.int16, device="cuda") for g in f.output_geometry()]
-        torch.cuda.synchronize()
-        ctx.frame_device(f, [t.data_ptr() for t in dev_in], [t.data_ptr() for t in dev_out])
+        dev_in = [hip.DeviceArray.from_host(p) for p in f.planes]
+        dev_out = [hip.DeviceArray(g) for g in f.output_geometry()]
+        ctx.frame_device(f, [t.ptr for t in dev_in], [t.ptr for t in dev_out])
         ctx.finish()
-        got = [t.cpu().numpy().view(np.uint16) for t in dev_out]
+        got = [t.download() for t in dev_out]
         assert_frame_equal(got, want, f"device/{k}")
+
+
+def test_gpu_device_entry_defers_sample_range_errors_to_finish(ctx):
+    from vaco_amd import hip
+    say("device entry: deferred error")
+    f = F.random_frame(44, 64, 64, skip_p=0.0, idx_neg_p=0.0)
+    f.planes[0][20, 20] = 999
+    dev_in = [hip.DeviceArray.from_host(p) for p in f.planes]
+    dev_out = [hip.DeviceArray(g) for g in f.output_geometry()]
+    ctx.frame_device(f, [t.ptr for t in dev_in], [t.ptr for t in dev_out])
+    with pytest.raises(av1pf.Av1Error) as e:
+        ctx.finish()
+    assert e.value.message == "CDEF sample exceeds bit depth"
+    ctx.finish()  # the error is reported once
 
 
 # --------------------------------------------------------------- errors ---
