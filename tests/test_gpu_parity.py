@@ -172,16 +172,78 @@ def test_gpu_out_of_range_samples_report_the_reference_error(ctx):
     assert_frame_equal(ctx.frame(g), O.frame(g.desc, g.planes), "unread bad sample")
 
 
-def test_gpu_invalid_parameters_report_the_reference_error(ctx):
-    say("error: invalid parameters")
-    f = F.random_frame(42, 64, 64, skip_p=0.0, idx_neg_p=0.0)
-    f.desc.cdef.y[3].primary = 16
-    f._keep[0][:] = 3
-    with pytest.raises(O.OracleError) as want:
-        O.frame(f.desc, f.planes)
-    with pytest.raises(av1pf.Av1Error) as got:
-        ctx.frame(f)
-    assert got.value.message == want.value.message
+def outcome_oracle(f, **kw):
+    try:
+        return ("ok", O.frame(f.desc, f.planes, **kw))
+    except O.OracleError as e:
+        return ("err", e.status, e.message)
+
+
+def outcome_gpu(ctx, f, **kw):
+    try:
+        return ("ok", ctx.frame(f, **kw))
+    except av1pf.Av1Error as e:
+        return ("err", e.status, e.message)
+
+
+def same_outcome(a, b, label):
+    assert a[0] == b[0], (label, a[:3] if a[0] == "err" else "ok", b[:3] if b[0] == "err" else "ok")
+    if a[0] == "ok":
+        assert_frame_equal(a[1], b[1], label)
+    else:
+        assert a[1:] == b[1:], label
+
+
+def _mutations():
+    def luma_primary_16(f):  # adjusted by variance: an error only when var >= 2^18
+        f.desc.cdef.y[3].primary = 16
+        f._keep[0][:] = 3
+
+    def chroma_primary_16(f):  # chroma primary is not adjusted: always an error
+        f.desc.cdef.uv[2].primary = 16
+        f._keep[0][:] = 2
+
+    def secondary_5(f):
+        f.desc.cdef.y[1].secondary = 5
+        f._keep[0][:] = 1
+
+    def index_8(f):
+        f._keep[0][1] = 8
+
+    def damping_13(f):
+        f.desc.cdef.damping = 13
+
+    def bad_depth(f):
+        f.desc.bit_depth = 9
+
+    def unit_count(f):
+        f.desc.lr_unit_count[1] -= 1
+
+    def wiener_tap(f):
+        f.desc.lr_units[0][0].kind = abi.UNIT_WIENER
+        f.desc.lr_units[0][0].vertical[0] = 11
+
+    def sgr_xqd(f):
+        f.desc.lr_units[2][0].kind = abi.UNIT_SELF_GUIDED
+        f.desc.lr_units[2][0].xqd[1] = 96
+
+    def no_unit_size(f):
+        f.desc.lr_unit_size[1] = 0
+
+    def superres_not_wider(f):
+        f.desc.use_superres = 1
+
+    return [luma_primary_16, chroma_primary_16, secondary_5, index_8, damping_13, bad_depth, unit_count,
+            wiener_tap, sgr_xqd, no_unit_size, superres_not_wider]
+
+
+@pytest.mark.parametrize("mutate", _mutations(), ids=lambda m: m.__name__)
+def test_gpu_invalid_parameters_match_reference_outcome(ctx, mutate):
+    say(f"error: {mutate.__name__}")
+    for seed in (42, 43, 44):
+        f = F.random_frame(seed, 64, 64, skip_p=0.0, idx_neg_p=0.0)
+        mutate(f)
+        same_outcome(outcome_oracle(f), outcome_gpu(ctx, f), f"{mutate.__name__}/{seed}")
 
 
 def test_gpu_fuel_exhaustion_matches_reference_accounting(ctx):
