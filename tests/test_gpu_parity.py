@@ -208,7 +208,7 @@ def _mutations():
         f._keep[0][:] = 1
 
     def index_8(f):
-        f._keep[0][1] = 8
+        f._keep[0][0] = 8
 
     def damping_13(f):
         f.desc.cdef.damping = 13
