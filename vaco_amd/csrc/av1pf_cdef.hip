@@ -1,0 +1,425 @@
+// av1pf_cdef.hip - CDEF (AV1 7.15) for gfx950.
+//
+// Reference semantics (matteopolak/vaco, crates/codec/vaco-codec-av1/src/):
+// find_direction cdef.rs:79-124, constrain 126-133, filter_block 142-205,
+// adjust_strength 208-220, chroma_direction 222-230, frame walk
+// decode.rs:739-861 with cdef_block_skipped 892-896.
+//
+// One workgroup of 1024 threads (16 wave64) per 64x64 luma superblock and
+// its co-located chroma. The superblock plus a 2-sample border is staged in
+// LDS once; the direction search keeps each block's 15 line sums in
+// registers (one wave per direction, one lane per 8x8 block, 16-byte LDS
+// row loads); then 16 lanes filter each 8x8 luma block (4 samples each)
+// with the block's parameters held in registers. Only superblocks that
+// touch a plane edge pay for tap-availability checks.
+#include "av1pf_common.h"
+
+namespace av1pf {
+
+// DIRECTIONS, cdef.rs:56-66, as (row, column).
+__constant__ int8_t kDirDy[8][2] = {{-1, -2}, {0, -1}, {0, 0}, {0, 1},
+                                    {1, 2},   {1, 2},  {1, 2}, {1, 2}};
+__constant__ int8_t kDirDx[8][2] = {{1, 2}, {1, 2}, {1, 2}, {1, 2},
+                                    {1, 2}, {0, 1}, {0, 0}, {0, -1}};
+
+// Line-partition bin of pixel (i, j) for direction d (cdef.rs:88-97).
+constexpr int bin_of(int d, int i, int j)
+{
+    return d == 0   ? i + j
+           : d == 1 ? i + j / 2
+           : d == 2 ? i
+           : d == 3 ? 3 + i - j / 2
+           : d == 4 ? 7 + i - j
+           : d == 5 ? 3 + j - i / 2
+           : d == 6 ? j
+                    : i / 2 + j;
+}
+
+constexpr int bin_count(int d, int b)
+{
+    int c = 0;
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++)
+            if (bin_of(d, i, j) == b)
+                c++;
+    return c;
+}
+
+constexpr int kRow = 80;         // LDS tile row: columns px0-8 .. px0+71
+constexpr int kThreads = 1024;
+
+// Cost of direction D (cdef.rs:104-113) of the 8x8 block whose first row
+// starts at `p` (16-byte aligned LDS rows of kRow samples), with samples
+// taken >> shift. Rows stream into the 15 line sums, which stay in
+// registers; the -128 of cdef.rs:98 is applied once per bin as -128 *
+// count. For in-range samples every cost is below 840*64*128^2 < 2^31.
+template <int D>
+__device__ __forceinline__ uint32_t dir_cost(const uint16_t *p, int shift)
+{
+    int part[15];
+#pragma unroll
+    for (int b = 0; b < 15; b++)
+        part[b] = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(p + i * kRow);
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t s = (j & 1) ? (w[j >> 1] >> 16) : (w[j >> 1] & 0xffffu);
+            part[bin_of(D, i, j)] += (int)(s >> shift);
+        }
+    }
+    uint32_t cost = 0;
+#pragma unroll
+    for (int b = 0; b < 15; b++) {
+        const int cnt = bin_count(D, b);
+        if (cnt != 0) {
+            const int s = part[b] - 128 * cnt;
+            cost += (uint32_t)(s * s) * (uint32_t)(840 / cnt);
+        }
+    }
+    return cost;
+}
+
+__device__ __forceinline__ uint32_t dir_cost_any(int d, const uint16_t *p, int shift)
+{
+    switch (d) {
+    case 0: return dir_cost<0>(p, shift);
+    case 1: return dir_cost<1>(p, shift);
+    case 2: return dir_cost<2>(p, shift);
+    case 3: return dir_cost<3>(p, shift);
+    case 4: return dir_cost<4>(p, shift);
+    case 5: return dir_cost<5>(p, shift);
+    case 6: return dir_cost<6>(p, shift);
+    default: return dir_cost<7>(p, shift);
+    }
+}
+
+// Filter parameters of one 8x8 block (one plane class), cdef.rs:35-54.
+struct __align__(16) Blk {
+    int16_t off[6];      // LDS tap offsets: dir d1, d2; dir+2 d1, d2; dir+6 d1, d2
+    uint8_t pri, sec;    // strengths after scaling / variance adjustment
+    uint8_t shifts;      // damping - ilog2(strength), primary | secondary << 4
+    uint8_t flags;       // 1: filter this block, 2: primary taps are (3, 3)
+};
+
+__device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int damping,
+                                        int shift, int row)
+{
+    Blk B;
+    B.pri = (uint8_t)min(pri, 255u);
+    B.sec = (uint8_t)min(sec, 255u);
+    const int ps = pri ? max(0, damping - ilog2u(pri)) : 0;  // constrain, cdef.rs:130
+    const int ss = sec ? max(0, damping - ilog2u(sec)) : 0;
+    B.shifts = (uint8_t)(min(ps, 15) | (min(ss, 15) << 4));
+    B.flags = 1 | (((pri >> shift) & 1) ? 2 : 0);  // cdef.rs:156-160
+    const int d2 = (dir + 2) & 7, d6 = (dir + 6) & 7;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        B.off[k] = (int16_t)(kDirDy[dir][k] * row + kDirDx[dir][k]);
+        B.off[2 + k] = (int16_t)(kDirDy[d2][k] * row + kDirDx[d2][k]);
+        B.off[4 + k] = (int16_t)(kDirDy[d6][k] * row + kDirDx[d6][k]);
+    }
+    return B;
+}
+
+// Both taps of one direction/distance, cdef.rs:171-198. constrain() is
+// min(|d|, max(0, S - (|d| >> shift))) with the sign of d.
+template <bool BORDER>
+__device__ __forceinline__ void tap2(const uint16_t *t, int ci, int off, int x, int S, int sh,
+                                     int w, int &sum, int &mn, int &mx)
+{
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const int p = t[s ? ci + off : ci - off];
+        const int d = p - x;
+        const int ad = abs(d);
+        const int m = min(max(S - (ad >> sh), 0), ad);
+        const int c = d < 0 ? -m : m;
+        if (BORDER) {  // unavailable taps are skipped (cdef.rs:189-191)
+            const bool valid = p != kUnavailable;
+            sum += valid ? w * c : 0;
+            mx = max(mx, valid ? p : x);
+        } else {
+            sum += w * c;
+            mx = max(mx, p);
+        }
+        mn = min(mn, p);
+    }
+}
+
+template <bool BORDER>
+__device__ __forceinline__ int filter_px(const uint16_t *t, int ci, const Blk &B)
+{
+    const int x = t[ci];
+    int sum = 0, mn = x, mx = x;
+    const int ps = B.shifts & 15, ss = B.shifts >> 4;
+    const int t0 = (B.flags & 2) ? 3 : 4;
+    tap2<BORDER>(t, ci, B.off[0], x, B.pri, ps, t0, sum, mn, mx);
+    tap2<BORDER>(t, ci, B.off[1], x, B.pri, ps, 6 - t0, sum, mn, mx);
+    tap2<BORDER>(t, ci, B.off[2], x, B.sec, ss, 2, sum, mn, mx);
+    tap2<BORDER>(t, ci, B.off[3], x, B.sec, ss, 1, sum, mn, mx);
+    tap2<BORDER>(t, ci, B.off[4], x, B.sec, ss, 2, sum, mn, mx);
+    tap2<BORDER>(t, ci, B.off[5], x, B.sec, ss, 1, sum, mn, mx);
+    const int y = x + ((8 + sum - (sum < 0)) >> 4);  // cdef.rs:200-201
+    return min(max(y, mn), mx);
+}
+
+// Stage rows ys .. ys+ROWS-1, columns xs .. xs+COLS-1 of a plane into an LDS
+// tile at column offset 6; positions outside the plane hold kUnavailable.
+template <int ROWS, int COLS>
+__device__ __forceinline__ bool stage(uint16_t *dst, const uint16_t *src, int W, int H, int st,
+                                      int xs, int ys, bool border, int maxv)
+{
+    bool bad = false;
+#pragma unroll 2
+    for (int e = threadIdx.x; e < ROWS * COLS; e += kThreads) {
+        const int r = e / COLS, c = e - r * COLS;
+        const int x = xs + c, y = ys + r;
+        uint16_t v = kUnavailable;
+        if (!border || (x >= 0 && y >= 0 && x < W && y < H)) {
+            v = src[(size_t)y * st + x];
+            bad |= v > maxv;
+        }
+        dst[r * kRow + c + 6] = v;
+    }
+    return bad;
+}
+
+// Filter (or pass through) every sample of a staged superblock and store it.
+// Luma: 16 lanes per 8x8 block, rows i0, i0+2, i0+4, i0+6 of column j.
+// Chroma: 16 lanes per chroma block, BW*BH/16 samples each, both planes.
+template <bool BORDER, int SX, int SY>
+__device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*lt)[68 * kRow],
+                                          const Blk (*blk)[64], const uint8_t *bact, bool any,
+                                          int x0, int y0)
+{
+    constexpr int BW = 8 >> SX, BH = 8 >> SY, CPX = BW * BH / 16;
+    const int tid = threadIdx.x;
+    const int b = tid >> 4, q = tid & 15;
+    const int bi = b >> 3, bj = b & 7;
+    const bool act = any && bact[b];
+    {
+        const int j = q & 7, i0 = q >> 3;
+        Blk B = {};
+        if (act)
+            B = blk[0][b];
+        const uint16_t *t = lt[0];
+        uint16_t *dst = f.out[0];
+        const int ost = f.out_stride[0], W0 = f.width[0], H0 = f.height[0];
+#pragma unroll 2
+        for (int k = 0; k < 4; k++) {
+            const int i = i0 + 2 * k;
+            const int x = x0 + bj * 8 + j, y = y0 + bi * 8 + i;
+            const int ci = (2 + bi * 8 + i) * kRow + 8 + bj * 8 + j;
+            const int res = act ? filter_px<BORDER>(t, ci, B) : t[ci];
+            if (!BORDER || (x < W0 && y < H0))
+                dst[(size_t)y * ost + x] = (uint16_t)res;
+        }
+    }
+    if (f.nplanes > 1) {
+        Blk B = {};
+        if (act)
+            B = blk[1][b];
+        const int W1 = f.width[1], H1 = f.height[1];
+        const int cx0 = x0 >> SX, cy0 = y0 >> SY;
+#pragma unroll
+        for (int k = 0; k < CPX; k++) {
+            const int pi = q + 16 * k, i = pi / BW, j = pi % BW;
+            const int x = cx0 + bj * BW + j, y = cy0 + bi * BH + i;
+            const int ci = (2 + bi * BH + i) * kRow + 8 + bj * BW + j;
+            const bool store = !BORDER || (x < W1 && y < H1);
+#pragma unroll
+            for (int p = 1; p < 3; p++) {
+                const uint16_t *t = lt[p];
+                const int res = act ? filter_px<BORDER>(t, ci, B) : t[ci];
+                if (store)
+                    f.out[p][(size_t)y * f.out_stride[p] + x] = (uint16_t)res;
+            }
+        }
+    }
+}
+
+// One 64x64 luma superblock of one frame, all planes (decode.rs:772-856).
+// SX, SY: chroma subsampling (monochrome frames use <0, 0> with 1 plane).
+template <int SX, int SY>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cdef(const CdefFrame *__restrict__ frames, int n_frames)
+{
+    constexpr int CW = 64 >> SX, CH = 64 >> SY, BW = 8 >> SX, BH = 8 >> SY;
+    constexpr int CPX = BW * BH / 16;  // chroma samples per thread per plane
+    __shared__ __align__(16) uint16_t lt[3][68 * kRow];
+    __shared__ Blk blk[2][64];
+    __shared__ uint32_t cost[8][64];
+    __shared__ uint8_t bact[64];
+
+    const CdefFrame &f = find_item(frames, n_frames, blockIdx.x);
+    const uint32_t sb = blockIdx.x - f.block_begin;
+    const int sby = (int)(sb / f.sb_cols), sbx = (int)(sb - (uint32_t)sby * f.sb_cols);
+    const int tid = threadIdx.x;
+    const int bd = f.bit_depth, shift = bd - 8, maxv = (1 << bd) - 1;
+    const int nplanes = f.nplanes;
+    const int x0 = sbx * 64, y0 = sby * 64;
+    const int W0 = f.width[0], H0 = f.height[0];
+    const int idx = f.cdef_idx[sby * f.idx_stride + sbx];
+    const bool border = x0 == 0 || y0 == 0 || x0 + 64 >= W0 || y0 + 64 >= H0;
+
+    if (tid < 64) {  // decode.rs:774-777, 892-896
+        const int bx = x0 + (tid & 7) * 8, by = y0 + (tid >> 3) * 8;
+        bact[tid] = idx >= 0 && bx < W0 && by < H0 &&
+                    f.skip8[(size_t)(by >> 3) * f.skip_stride + (bx >> 3)] == 0;
+    }
+    bool bad = stage<68, 68>(lt[0], f.in[0], W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border, maxv);
+    if (nplanes > 1) {
+        const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
+        bad |= stage<CH + 4, CW + 4>(lt[1], f.in[1], f.width[1], f.height[1], f.in_stride[1], cx,
+                                     cy, border, maxv);
+        bad |= stage<CH + 4, CW + 4>(lt[2], f.in[2], f.width[2], f.height[2], f.in_stride[2], cx,
+                                     cy, border, maxv);
+    }
+    if (__syncthreads_or(bad) && tid == 0)
+        atomicOr(f.status, kFlagSampleRange);
+
+    if (idx >= 0) {
+        // Direction search: wave w < 8 computes direction w, lane = block.
+        if (tid < 512) {
+            const int b = tid & 63;
+            if (bact[b])
+                cost[tid >> 6][b] =
+                    dir_cost_any(tid >> 6, &lt[0][(2 + (b >> 3) * 8) * kRow + 8 + (b & 7) * 8], shift);
+        }
+        __syncthreads();
+        if (tid < 128) {
+            const int b = tid & 63, cls = tid >> 6;
+            Blk B = {};
+            if (bact[b]) {
+                int best = 0;
+                uint32_t bc = cost[0][b];
+#pragma unroll
+                for (int d = 1; d < 8; d++) {
+                    const uint32_t c = cost[d][b];
+                    if (c > bc) {  // first maximum wins, cdef.rs:114-119
+                        bc = c;
+                        best = d;
+                    }
+                }
+                const uint32_t var = (bc - cost[(best + 4) & 7][b]) >> 10;
+                const int ix = idx & 7;
+                if (cls == 0) {  // decode.rs:806-818
+                    const uint32_t pri = (uint32_t)f.y_pri[ix] << shift;
+                    uint32_t adj = 0;
+                    if (var != 0) {  // adjust_strength, cdef.rs:208-220
+                        const uint32_t sc = var >> 6;
+                        const uint32_t lg = sc ? min(ilog2u(sc), 12) : 0;
+                        adj = (pri * (4 + lg) + 8) >> 4;
+                    }
+                    B = make_blk(pri ? best : 0, adj, (uint32_t)f.y_sec[ix] << shift,
+                                 (uint8_t)(f.damping + shift), shift, kRow);
+                } else {
+                    const uint32_t pri = (uint32_t)f.uv_pri[ix] << shift;
+                    // chroma_direction, cdef.rs:222-230: nibble `best` of
+                    // {7,0,2,4,5,6,6,6} (4:2:2) or {1,2,2,2,3,4,6,0} (4:4:0).
+                    int cdir = best;
+                    if (SX && !SY)
+                        cdir = (0x66654207u >> (4 * best)) & 7;
+                    else if (!SX && SY)
+                        cdir = (0x06432221u >> (4 * best)) & 7;
+                    B = make_blk(pri ? cdir : 0, pri, (uint32_t)f.uv_sec[ix] << shift,
+                                 (uint8_t)(f.damping + shift - 1), shift, kRow);
+                }
+            }
+            blk[cls][b] = B;
+        }
+        __syncthreads();
+    }
+
+    const bool any = idx >= 0;
+    if (border)
+        filter_sb<true, SX, SY>(f, lt, blk, bact, any, x0, y0);
+    else
+        filter_sb<false, SX, SY>(f, lt, blk, bact, any, x0, y0);
+}
+
+// Kernel-level cases (cdef::find_direction + cdef::filter_block on one
+// 12x12 neighbourhood each, av1_cdef.rs:85-103): one wave per case.
+struct CdefCase {
+    uint32_t width, height;
+    uint8_t edges, bit_depth, direction, damping;
+    uint16_t primary, secondary;
+};
+
+__global__ __launch_bounds__(64) void k_cdef_cases(const uint16_t *__restrict__ inputs,
+                                                   const CdefCase *__restrict__ params,
+                                                   uint8_t *__restrict__ dirs,
+                                                   uint32_t *__restrict__ vars,
+                                                   uint16_t *__restrict__ outs, uint32_t n)
+{
+    __shared__ __align__(16) uint16_t t[12 * kRow];
+    __shared__ __align__(16) uint16_t raw[8 * kRow];
+    __shared__ uint32_t cost[8];
+    const uint32_t cs = blockIdx.x;
+    if (cs >= n)
+        return;
+    const int lane = threadIdx.x;
+    const CdefCase P = params[cs];
+    const int shift = P.bit_depth - 8;
+    const uint16_t *in = inputs + (size_t)cs * 144;
+    // Taps outside the edge mask become unavailable, exactly like plane-exterior
+    // samples in k_cdef (cdef.rs:161-166, 189-191).
+    for (int e = lane; e < 144; e += 64) {
+        const int r = e / 12, c = e % 12;
+        const int ny = r - 2, nx = c - 2;
+        const bool ok = !((nx < 0 && !(P.edges & 1)) || (nx >= (int)P.width && !(P.edges & 2)) ||
+                          (ny < 0 && !(P.edges & 4)) || (ny >= (int)P.height && !(P.edges & 8)));
+        t[r * kRow + c + 6] = ok ? in[e] : kUnavailable;
+    }
+    // The direction search reads the raw 8x8 at (2,2) (av1_cdef.rs:90-95).
+    raw[(lane >> 3) * kRow + (lane & 7)] = in[((lane >> 3) + 2) * 12 + (lane & 7) + 2];
+    __syncthreads();
+    if (lane < 8)
+        cost[lane] = dir_cost_any(lane, raw, shift);
+    __syncthreads();
+    if (lane == 0) {
+        int best = 0;
+        uint32_t bc = cost[0];
+        for (int d = 1; d < 8; d++)
+            if (cost[d] > bc) {
+                bc = cost[d];
+                best = d;
+            }
+        dirs[cs] = (uint8_t)best;
+        vars[cs] = (bc - cost[(best + 4) & 7]) >> 10;
+    }
+    const Blk B = make_blk(P.direction, P.primary, P.secondary, P.damping, shift, kRow);
+    const int r = lane >> 3, c = lane & 7;
+    uint16_t res = 0;
+    if (r < (int)P.height && c < (int)P.width)
+        res = (uint16_t)filter_px<true>(t, (r + 2) * kRow + c + 8, B);
+    outs[(size_t)cs * 64 + lane] = res;
+}
+
+// mode: 0 = 4:2:0, 1 = 4:4:4 or monochrome, 2 = 4:2:2, 3 = 4:4:0
+hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, hipStream_t s)
+{
+    if (blocks == 0)
+        return hipSuccess;
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((k_cdef<1, 1>), dim3(blocks), dim3(kThreads), 0, s, frames, n); break;
+    case 1: hipLaunchKernelGGL((k_cdef<0, 0>), dim3(blocks), dim3(kThreads), 0, s, frames, n); break;
+    case 2: hipLaunchKernelGGL((k_cdef<1, 0>), dim3(blocks), dim3(kThreads), 0, s, frames, n); break;
+    default: hipLaunchKernelGGL((k_cdef<0, 1>), dim3(blocks), dim3(kThreads), 0, s, frames, n); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_cdef_cases(const uint16_t *inputs, const void *params, uint8_t *dirs,
+                             uint32_t *vars, uint16_t *outs, uint32_t n, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_cdef_cases, dim3(n), dim3(64), 0, s, inputs, (const CdefCase *)params,
+                       dirs, vars, outs, n);
+    return hipGetLastError();
+}
+
+}  // namespace av1pf

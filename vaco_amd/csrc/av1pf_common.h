@@ -1,0 +1,33 @@
+// av1pf_common.h - device helpers shared by the stage's kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "av1pf_device.h"
+
+namespace av1pf {
+
+__device__ __forceinline__ int ilog2u(uint32_t v) { return 31 - __clz(v); }
+
+// Round2 of the AV1 specification (restoration.rs:251-257) for int32.
+__device__ __forceinline__ int round2i(int v, int s)
+{
+    return s == 0 ? v : (v + (1 << (s - 1))) >> s;
+}
+
+// Work item owning workgroup `b`: the last item whose block_begin <= b.
+template <class T>
+__device__ __forceinline__ const T &find_item(const T *items, int n, uint32_t b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].block_begin <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return items[lo];
+}
+
+}  // namespace av1pf

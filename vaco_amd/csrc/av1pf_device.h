@@ -17,6 +17,9 @@ namespace av1pf {
 // most 4095, so the value never collides with in-range data.
 constexpr uint16_t kUnavailable = 0xFFFF;
 
+// RestorationUnit discriminants (restoration.rs:46-65), as in the C ABI.
+constexpr uint8_t kUnitNone = 0, kUnitWiener = 1, kUnitSelfGuided = 2;
+
 // Bit set in a frame's status word when a kernel read a sample above
 // (1 << bit_depth) - 1. The host then replays the reference's checks in
 // order to report the exact error (or none).

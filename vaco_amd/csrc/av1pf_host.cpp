@@ -26,7 +26,7 @@
 #include "av1pf_host.h"
 
 namespace av1pf {
-hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, hipStream_t s);
+hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, hipStream_t s);
 hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s);
 hipError_t launch_lr(const LrPlane *planes, int n, uint32_t blocks, hipStream_t s);
 hipError_t launch_cdef_cases(const uint16_t *inputs, const void *params, uint8_t *dirs,
@@ -399,6 +399,18 @@ void lr_border_rows(const std::vector<LrRowSeg> &rows, uint32_t height, std::vec
 
 }  // namespace
 
+// Template instance of the CDEF kernel for a frame's chroma layout.
+static int cdef_mode(const vaco_av1pf_frame_desc &d)
+{
+    if (d.monochrome)
+        return 1;
+    if (d.subsampling_x && d.subsampling_y)
+        return 0;
+    if (!d.subsampling_x && !d.subsampling_y)
+        return 1;
+    return d.subsampling_x ? 2 : 3;
+}
+
 // Build descriptors for one frame into `a` and launch the stage on stream
 // `s`. `in`/`out` are device planes. `status` is the device status word.
 static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_desc &d,
@@ -626,7 +638,7 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
     }
     if (g.cdef)
         HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, s,
-                              [&] { return launch_cdef(dc, 1, cdef_blocks, s); }));
+                              [&] { return launch_cdef(dc, 1, cdef_blocks, cdef_mode(d), s); }));
     if (njobs)
         HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES, s,
                               [&] { return launch_superres(dj, njobs, sr_blocks, s); }));

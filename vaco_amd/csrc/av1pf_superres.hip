@@ -1,0 +1,86 @@
+// av1pf_superres.hip - super-resolution upscaling (AV1 7.16) for gfx950.
+//
+// Reference semantics (matteopolak/vaco, crates/codec/vaco-codec-av1/src/
+// superres.rs): subpel_start 145-152, upscale_plane 163-216. One workgroup
+// per kSrSeg output samples of one row; the eight taps are read from the
+// padded source row (edge-clamped to the padded width).
+#include "av1pf_common.h"
+
+namespace av1pf {
+
+// UPSCALE_FILTER, superres.rs:25-90 (AV1 spec Upscale_Filter[64][8]).
+__constant__ int16_t kUpscale[64][8] = {
+    {0, 0, 0, 128, 0, 0, 0, 0},       {0, 0, -1, 128, 2, -1, 0, 0},
+    {0, 1, -3, 127, 4, -2, 1, 0},     {0, 1, -4, 127, 6, -3, 1, 0},
+    {0, 2, -6, 126, 8, -3, 1, 0},     {0, 2, -7, 125, 11, -4, 1, 0},
+    {-1, 2, -8, 125, 13, -5, 2, 0},   {-1, 3, -9, 124, 15, -6, 2, 0},
+    {-1, 3, -10, 123, 18, -6, 2, -1}, {-1, 3, -11, 122, 20, -7, 3, -1},
+    {-1, 4, -12, 121, 22, -8, 3, -1}, {-1, 4, -13, 120, 25, -9, 3, -1},
+    {-1, 4, -14, 118, 28, -9, 3, -1}, {-1, 4, -15, 117, 30, -10, 4, -1},
+    {-1, 5, -16, 116, 32, -11, 4, -1}, {-1, 5, -16, 114, 35, -12, 4, -1},
+    {-1, 5, -17, 112, 38, -12, 4, -1}, {-1, 5, -18, 111, 40, -13, 5, -1},
+    {-1, 5, -18, 109, 43, -14, 5, -1}, {-1, 6, -19, 107, 45, -14, 5, -1},
+    {-1, 6, -19, 105, 48, -15, 5, -1}, {-1, 6, -19, 103, 51, -16, 5, -1},
+    {-1, 6, -20, 101, 53, -16, 6, -1}, {-1, 6, -20, 99, 56, -17, 6, -1},
+    {-1, 6, -20, 97, 58, -17, 6, -1}, {-1, 6, -20, 95, 61, -18, 6, -1},
+    {-2, 7, -20, 93, 64, -18, 6, -2}, {-2, 7, -20, 91, 66, -19, 6, -1},
+    {-2, 7, -20, 88, 69, -19, 6, -1}, {-2, 7, -20, 86, 71, -19, 6, -1},
+    {-2, 7, -20, 84, 74, -20, 7, -2}, {-2, 7, -20, 81, 76, -20, 7, -1},
+    {-2, 7, -20, 79, 79, -20, 7, -2}, {-1, 7, -20, 76, 81, -20, 7, -2},
+    {-2, 7, -20, 74, 84, -20, 7, -2}, {-1, 6, -19, 71, 86, -20, 7, -2},
+    {-1, 6, -19, 69, 88, -20, 7, -2}, {-1, 6, -19, 66, 91, -20, 7, -2},
+    {-2, 6, -18, 64, 93, -20, 7, -2}, {-1, 6, -18, 61, 95, -20, 6, -1},
+    {-1, 6, -17, 58, 97, -20, 6, -1}, {-1, 6, -17, 56, 99, -20, 6, -1},
+    {-1, 6, -16, 53, 101, -20, 6, -1}, {-1, 5, -16, 51, 103, -19, 6, -1},
+    {-1, 5, -15, 48, 105, -19, 6, -1}, {-1, 5, -14, 45, 107, -19, 6, -1},
+    {-1, 5, -14, 43, 109, -18, 5, -1}, {-1, 5, -13, 40, 111, -18, 5, -1},
+    {-1, 4, -12, 38, 112, -17, 5, -1}, {-1, 4, -12, 35, 114, -16, 5, -1},
+    {-1, 4, -11, 32, 116, -16, 5, -1}, {-1, 4, -10, 30, 117, -15, 4, -1},
+    {-1, 3, -9, 28, 118, -14, 4, -1}, {-1, 3, -9, 25, 120, -13, 4, -1},
+    {-1, 3, -8, 22, 121, -12, 4, -1}, {-1, 3, -7, 20, 122, -11, 3, -1},
+    {-1, 2, -6, 18, 123, -10, 3, -1}, {0, 2, -6, 15, 124, -9, 3, -1},
+    {0, 2, -5, 13, 125, -8, 2, -1},   {0, 1, -4, 11, 125, -7, 2, 0},
+    {0, 1, -3, 8, 126, -6, 2, 0},     {0, 1, -3, 6, 127, -4, 1, 0},
+    {0, 1, -2, 4, 127, -3, 1, 0},     {0, 0, -1, 2, 128, -1, 0, 0},
+};
+
+// One row segment of kSrSeg output samples (superres.rs:173-214).
+__global__ __launch_bounds__(kSrSeg) void k_superres(const SrJob *__restrict__ jobs, int n_jobs)
+{
+    const SrJob &j = find_item(jobs, n_jobs, blockIdx.x);
+    const uint32_t local = blockIdx.x - j.block_begin;
+    const uint32_t ri = local / j.segs;
+    const uint32_t seg = local - ri * j.segs;
+    const uint32_t y = j.rows ? j.rows[ri] : ri;
+    const int x = (int)(seg * kSrSeg + threadIdx.x);
+    if (x >= (int)j.out_width)
+        return;
+    const int src_x = -(1 << 14) + j.start + x * j.step;
+    const int px = src_x >> 14;
+    const int phase = (src_x & 0x3fff) >> 8;
+    const uint16_t *row = j.in + (size_t)y * j.in_stride;
+    const int last = (int)j.in_width - 1;
+    int sum = 0;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int sx = min(max(px + k - 3, 0), last);
+        const int v = row[sx];
+        bad |= v > j.max_sample;
+        sum += v * kUpscale[phase][k];
+    }
+    if (bad)
+        atomicOr(j.status, kFlagSampleRange);
+    const int r = (sum + 64) >> 7;
+    j.out[(size_t)y * j.out_stride + x] = (uint16_t)min(max(r, 0), (int)j.max_sample);
+}
+
+hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s)
+{
+    if (blocks == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_superres, dim3(blocks), dim3(kSrSeg), 0, s, jobs, n);
+    return hipGetLastError();
+}
+
+}  // namespace av1pf
