@@ -218,7 +218,10 @@ vaco_av1pf_status vaco_av1pf_frame(vaco_av1pf_ctx *ctx,
 /* The same stage on device-resident planes, enqueued on `stream`
  * (hipStream_t; NULL = the context's stream). Host-side validation errors
  * return at once; sample-range errors found by the kernels are reported by
- * the next vaco_av1pf_finish() on this context. */
+ * the next vaco_av1pf_finish() on this context. The side-information arrays
+ * `desc` points to and the input planes must stay unchanged until
+ * vaco_av1pf_finish() returns: an error is replayed from them. Up to 8
+ * frames are in flight per context; the 9th call waits for the 1st. */
 vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
                                           const vaco_av1pf_frame_desc *desc,
                                           const vaco_av1pf_picture *in,

@@ -219,16 +219,31 @@ class Context:
             return outs, f.value
         return outs
 
-    def frame_device(self, frame: Frame, in_ptrs, out_ptrs, stream: int | None = None):
-        """The stage on device planes (integer device addresses, dense rows)."""
+    def device_call(self, frame: Frame, in_ptrs, out_ptrs, stream: int | None = None):
+        """A zero-argument callable that enqueues the stage on these device
+        planes (integer device addresses, dense rows). The ctypes arguments
+        are built once, so repeated calls cost one foreign call each."""
         geo_in = [(p.shape[0], p.shape[1]) for p in frame.planes]
         geo_out = frame.output_geometry()
         pin, pout = abi.Picture(), abi.Picture()
         for i in range(frame.nplanes):
             pin.plane[i] = abi.device_plane(in_ptrs[i], geo_in[i][1], geo_in[i][0])
             pout.plane[i] = abi.device_plane(out_ptrs[i], geo_out[i][1], geo_out[i][0])
-        self._check(self._lib.vaco_av1pf_frame_device(self.handle, C.byref(frame.desc), C.byref(pin),
-                                                      C.byref(pout), C.c_void_p(stream or 0)))
+        fn, h = self._lib.vaco_av1pf_frame_device, self.handle
+        d, pi, po, st = C.byref(frame.desc), C.byref(pin), C.byref(pout), C.c_void_p(stream or 0)
+        check = self._check
+
+        def call():
+            rc = fn(h, d, pi, po, st)
+            if rc:
+                check(rc)
+
+        call.keep = (frame, pin, pout)
+        return call
+
+    def frame_device(self, frame: Frame, in_ptrs, out_ptrs, stream: int | None = None):
+        """The stage on device planes (integer device addresses, dense rows)."""
+        self.device_call(frame, in_ptrs, out_ptrs, stream)()
 
     def finish(self):
         self._check(self._lib.vaco_av1pf_finish(self.handle))

@@ -278,7 +278,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
                                      cy, border, maxv);
     }
     if (__syncthreads_or(bad) && tid == 0)
-        atomicOr(f.status, kFlagSampleRange);
+        raise_status(f.status, kFlagSampleRange);
 
     if (idx >= 0) {
         // Direction search: wave w < 8 computes direction w, lane = block.

@@ -15,6 +15,13 @@ __device__ __forceinline__ int round2i(int v, int s)
     return s == 0 ? v : (v + (1 << (s - 1))) >> s;
 }
 
+// Raise `bits` in a frame status word. Status words may live in coherent
+// pinned host memory, so the update is a system-scope vector atomic.
+__device__ __forceinline__ void raise_status(uint32_t *status, uint32_t bits)
+{
+    atomicOr_system(status, bits);
+}
+
 // Work item owning workgroup `b`: the last item whose block_begin <= b.
 template <class T>
 __device__ __forceinline__ const T &find_item(const T *items, int n, uint32_t b)

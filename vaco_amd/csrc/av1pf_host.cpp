@@ -39,14 +39,22 @@ using namespace av1pf;
 
 namespace {
 
-constexpr int kSlots = 4;
+constexpr int kSlots = 8;
 
+// One in-flight frame: its staging area (pinned host + device mirror), its
+// status word (coherent pinned host memory, raised by the kernels), and
+// what the deferred error replay needs (the caller keeps desc's arrays and
+// the input planes valid until vaco_av1pf_finish()).
 struct Arena {
     uint8_t *h = nullptr;  // pinned host
     uint8_t *d = nullptr;  // device mirror
     size_t cap = 0, used = 0;
-    hipEvent_t done = nullptr;  // last upload from this slot finished
+    hipEvent_t done = nullptr;  // the frame that used this slot has finished
     bool pending = false;
+    uint32_t *status = nullptr;
+    bool check = false;  // status must be checked before reuse
+    vaco_av1pf_frame_desc desc;
+    vaco_av1pf_picture in;
 
     template <class T>
     T *take(size_t n, T **dev)
@@ -72,18 +80,11 @@ struct vaco_av1pf_ctx {
     uint16_t *img[5][3] = {};      // 0 input, 1 cdef, 2 upscaled, 3 upscaled pre-CDEF, 4 output
     Arena slot[kSlots];
     int next_slot = 0;
-    uint32_t *h_status = nullptr;  // pinned readback of frame status words
+    uint32_t *h_status = nullptr;  // coherent pinned status words, one per slot
     const char *err = "";
-    // deferred (device entry) frames awaiting status check
-    struct Pending {
-        const uint32_t *d_status;
-        vaco_av1pf_frame_desc desc;
-        vaco_av1pf_picture in;
-        std::vector<int8_t> idx;
-        std::vector<uint8_t> skip;
-        std::vector<vaco_av1pf_lr_unit> units[3];
-    };
-    std::vector<Pending> pending;
+    // first error found by a deferred check (device entry), reported by finish
+    vaco_av1pf_status deferred = VACO_AV1PF_OK;
+    const char *deferred_msg = "";
     // kernel timing (vaco_av1pf_profile)
     struct ProfRec {
         int kernel;
@@ -120,6 +121,10 @@ static vaco_av1pf_status hip_fail(vaco_av1pf_ctx *ctx, hipError_t e)
             return hip_fail(ctx, e_);        \
     } while (0)
 
+static vaco_av1pf_status resolve_slot(vaco_av1pf_ctx *ctx, Arena &a);
+
+// Next staging slot. Waits for the frame that last used it and settles that
+// frame's deferred status check first.
 static vaco_av1pf_status take_slot(vaco_av1pf_ctx *ctx, Arena **out)
 {
     Arena &a = ctx->slot[ctx->next_slot];
@@ -128,7 +133,11 @@ static vaco_av1pf_status take_slot(vaco_av1pf_ctx *ctx, Arena **out)
         HIPCHECK(hipEventSynchronize(a.done));
         a.pending = false;
     }
+    vaco_av1pf_status st = resolve_slot(ctx, a);
+    if (st != VACO_AV1PF_OK)
+        return st;
     a.used = 0;
+    *a.status = 0;
     *out = &a;
     return VACO_AV1PF_OK;
 }
@@ -138,6 +147,12 @@ static vaco_av1pf_status upload_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t 
     if (a->used == 0)
         return VACO_AV1PF_OK;
     HIPCHECK(hipMemcpyAsync(a->d, a->h, a->used, hipMemcpyHostToDevice, s));
+    return VACO_AV1PF_OK;
+}
+
+// Marks the end of the slot's frame on stream s.
+static vaco_av1pf_status close_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t s)
+{
     HIPCHECK(hipEventRecord(a->done, s));
     a->pending = true;
     return VACO_AV1PF_OK;
@@ -199,9 +214,15 @@ extern "C" vaco_av1pf_status vaco_av1pf_create(int device, uint32_t max_width, u
         if (e != hipSuccess)
             return bail(e);
     }
-    e = hipHostMalloc((void **)&ctx->h_status, 64 * sizeof(uint32_t), hipHostMallocDefault);
+    // Status words: coherent so kernel atomics are visible to the host after
+    // the frame's event completes; one 64-byte line per slot.
+    e = hipHostMalloc((void **)&ctx->h_status, kSlots * 64, hipHostMallocCoherent);
     if (e != hipSuccess)
         return bail(e);
+    for (int i = 0; i < kSlots; i++) {
+        ctx->slot[i].status = ctx->h_status + i * 16;
+        *ctx->slot[i].status = 0;
+    }
     *out = ctx;
     return VACO_AV1PF_OK;
 }
@@ -785,9 +806,6 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame(vaco_av1pf_ctx *ctx, const vaco_av
     Arena *a;
     if ((st = take_slot(ctx, &a)) != VACO_AV1PF_OK)
         return st;
-    uint32_t *dstat;
-    uint32_t *hstat = a->take<uint32_t>(1, &dstat);
-    *hstat = 0;
     DevPlane din[3], dout[3];
     for (int p = 0; p < g.nplanes; p++) {
         din[p] = DevPlane{ctx->img[0][p], g.in_w[p]};
@@ -796,16 +814,17 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame(vaco_av1pf_ctx *ctx, const vaco_av
                                   in->plane[p].stride * 2, g.in_w[p] * 2, g.in_h[p],
                                   hipMemcpyHostToDevice, ctx->stream));
     }
-    st = enqueue_frame(ctx, *desc, g, din, dout, a, dstat, ctx->stream);
+    st = enqueue_frame(ctx, *desc, g, din, dout, a, a->status, ctx->stream);
     if (st != VACO_AV1PF_OK)
         return st;
     for (int p = 0; p < g.nplanes; p++)
         HIPCHECK(hipMemcpy2DAsync(out->plane[p].data, out->plane[p].stride * 2, dout[p].data,
                                   dout[p].stride * 2, g.out_w[p] * 2, g.out_h[p],
                                   hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipMemcpyAsync(ctx->h_status, dstat, 4, hipMemcpyDeviceToHost, ctx->stream));
+    if ((st = close_slot(ctx, a, ctx->stream)) != VACO_AV1PF_OK)
+        return st;
     HIPCHECK(hipStreamSynchronize(ctx->stream));
-    if (ctx->h_status[0] & kFlagSampleRange) {
+    if (*a->status & kFlagSampleRange) {
         const char *msg = "";
         vaco_av1pf_status r = replay_checks(*desc, g, hin, fuel, &msg);
         if (r != VACO_AV1PF_OK)
@@ -853,40 +872,52 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
     Arena *a;
     if ((st = take_slot(ctx, &a)) != VACO_AV1PF_OK)
         return st;
-    uint32_t *dstat;
-    uint32_t *hstat = a->take<uint32_t>(1, &dstat);
-    *hstat = 0;
     DevPlane din[3], dout[3];
     for (int p = 0; p < g.nplanes; p++) {
         din[p] = DevPlane{in->plane[p].data, in->plane[p].stride};
         dout[p] = DevPlane{out->plane[p].data, out->plane[p].stride};
     }
-    st = enqueue_frame(ctx, *desc, g, din, dout, a, dstat, s);
+    st = enqueue_frame(ctx, *desc, g, din, dout, a, a->status, s);
     if (st != VACO_AV1PF_OK)
         return st;
-    // Keep what the deferred check needs.
-    vaco_av1pf_ctx::Pending pd;
-    pd.d_status = dstat;
-    pd.desc = *desc;
-    pd.in = *in;
-    if (g.cdef) {
-        pd.idx.assign(desc->cdef_idx, desc->cdef_idx + (size_t)g.idx_rows * desc->cdef_stride);
-        pd.skip.assign(desc->skip8, desc->skip8 + (size_t)(g.mi_rows / 2) * (g.mi_cols / 2));
-        pd.desc.cdef_idx = nullptr;
-        pd.desc.skip8 = nullptr;
+    if ((st = close_slot(ctx, a, s)) != VACO_AV1PF_OK)
+        return st;
+    // The status is settled when the slot is reused or at finish().
+    a->check = true;
+    a->desc = *desc;
+    a->in = *in;
+    return VACO_AV1PF_OK;
+}
+
+// Settle a slot's deferred check: if its frame raised the sample-range flag,
+// replay the reference's checks over a host copy of the device input.
+static vaco_av1pf_status resolve_slot(vaco_av1pf_ctx *ctx, Arena &a)
+{
+    if (!a.check)
+        return VACO_AV1PF_OK;
+    a.check = false;
+    if (!(*a.status & kFlagSampleRange) || ctx->deferred != VACO_AV1PF_OK)
+        return VACO_AV1PF_OK;
+    FrameGeo g;
+    frame_geometry(a.desc, &g);
+    std::vector<uint16_t> host[3];
+    vaco_av1pf_picture hpic = a.in;
+    for (int p = 0; p < g.nplanes; p++) {
+        host[p].resize((size_t)g.in_w[p] * g.in_h[p]);
+        HIPCHECK(hipMemcpy2D(host[p].data(), g.in_w[p] * 2, a.in.plane[p].data,
+                             a.in.plane[p].stride * 2, g.in_w[p] * 2, g.in_h[p],
+                             hipMemcpyDeviceToHost));
+        hpic.plane[p].data = host[p].data();
+        hpic.plane[p].stride = g.in_w[p];
     }
-    for (int p = 0; p < 3; p++) {
-        if (desc->lr_units[p] && desc->lr_unit_count[p])
-            pd.units[p].assign(desc->lr_units[p], desc->lr_units[p] + desc->lr_unit_count[p]);
-        pd.desc.lr_units[p] = nullptr;
+    HostPlane hin[3];
+    host_planes_of(&hpic, g.nplanes, hin);
+    const char *msg = "";
+    vaco_av1pf_status r = replay_checks(a.desc, g, hin, nullptr, &msg);
+    if (r != VACO_AV1PF_OK) {
+        ctx->deferred = r;
+        ctx->deferred_msg = msg;
     }
-    if (ctx->pending.size() >= 48) {
-        st = vaco_av1pf_finish(ctx);
-        if (st != VACO_AV1PF_OK)
-            return st;
-    }
-    ctx->pending.push_back(std::move(pd));
-    (void)s;
     return VACO_AV1PF_OK;
 }
 
@@ -895,39 +926,23 @@ extern "C" vaco_av1pf_status vaco_av1pf_finish(vaco_av1pf_ctx *ctx)
     if (!ctx)
         return VACO_AV1PF_INVALID_ARGUMENT;
     HIPCHECK(hipSetDevice(ctx->device));
-    HIPCHECK(hipDeviceSynchronize());
-    vaco_av1pf_status result = VACO_AV1PF_OK;
-    for (auto &pd : ctx->pending) {
-        uint32_t flag = 0;
-        HIPCHECK(hipMemcpy(&flag, pd.d_status, 4, hipMemcpyDeviceToHost));
-        if (!(flag & kFlagSampleRange) || result != VACO_AV1PF_OK)
-            continue;
-        vaco_av1pf_frame_desc d = pd.desc;
-        d.cdef_idx = pd.idx.empty() ? nullptr : pd.idx.data();
-        d.skip8 = pd.skip.empty() ? nullptr : pd.skip.data();
-        for (int p = 0; p < 3; p++)
-            d.lr_units[p] = pd.units[p].empty() ? nullptr : pd.units[p].data();
-        FrameGeo g;
-        frame_geometry(d, &g);
-        std::vector<uint16_t> host[3];
-        vaco_av1pf_picture hpic = pd.in;
-        for (int p = 0; p < g.nplanes; p++) {
-            host[p].resize((size_t)g.in_w[p] * g.in_h[p]);
-            HIPCHECK(hipMemcpy2D(host[p].data(), g.in_w[p] * 2, pd.in.plane[p].data,
-                                 pd.in.plane[p].stride * 2, g.in_w[p] * 2, g.in_h[p],
-                                 hipMemcpyDeviceToHost));
-            hpic.plane[p].data = host[p].data();
-            hpic.plane[p].stride = g.in_w[p];
+    for (int i = 0; i < kSlots; i++) {
+        Arena &a = ctx->slot[i];
+        if (a.pending) {
+            HIPCHECK(hipEventSynchronize(a.done));
+            a.pending = false;
         }
-        HostPlane hin[3];
-        host_planes_of(&hpic, g.nplanes, hin);
-        const char *msg = "";
-        vaco_av1pf_status r = replay_checks(d, g, hin, nullptr, &msg);
-        if (r != VACO_AV1PF_OK)
-            result = fail(ctx, r, msg);
+        vaco_av1pf_status st = resolve_slot(ctx, a);
+        if (st != VACO_AV1PF_OK)
+            return st;
     }
-    ctx->pending.clear();
-    return result;
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    const vaco_av1pf_status r = ctx->deferred;
+    if (r != VACO_AV1PF_OK)
+        fail(ctx, r, ctx->deferred_msg);
+    ctx->deferred = VACO_AV1PF_OK;
+    ctx->deferred_msg = "";
+    return r;
 }
 
 // ------------------------------------------------- kernel-level entries --

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kLrThreads) void k_lr(const LrPlane *__restrict__ p
             }
         }
         if (__syncthreads_or(bad) && tid == 0)
-            atomicOr(P.status, kFlagSampleRange);
+            raise_status(P.status, kFlagSampleRange);
         return;
     }
 
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kLrThreads) void k_lr(const LrPlane *__restrict__ p
         S[r * kS + c] = (uint16_t)v;
     }
     if (__syncthreads_or(bad) && tid == 0)
-        atomicOr(P.status, kFlagSampleRange);
+        raise_status(P.status, kFlagSampleRange);
 
     if (u.kind == kUnitWiener) {
         // wiener, restoration.rs:264-283.

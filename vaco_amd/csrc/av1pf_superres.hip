@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kSrSeg) void k_superres(const SrJob *__restrict__ j
         sum += v * kUpscale[phase][k];
     }
     if (bad)
-        atomicOr(j.status, kFlagSampleRange);
+        raise_status(j.status, kFlagSampleRange);
     const int r = (sum + 64) >> 7;
     j.out[(size_t)y * j.out_stride + x] = (uint16_t)min(max(r, 0), (int)j.max_sample);
 }
