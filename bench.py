@@ -36,6 +36,54 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measure_mixed_batch(av1pf, hip, workloads, reps: int, n: int = 64):
+    """64 seeded frames of mixed geometry (360p/720p/1080p/2160p), bit depth
+    (8/10) and superres (half the frames): the per-frame path (one call per
+    frame) against one batched call over all 64, on the same device planes.
+    The batched outputs must equal the per-frame outputs bit for bit."""
+    frames = [av1pf.generate(c, i) for c, i in workloads.mixed_batch(n)]
+    ctx = av1pf.Context(3840, 2176)
+    dev_in = [[hip.DeviceArray.from_host(p) for p in f.planes] for f in frames]
+    out_pf = [[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames]
+    out_b = [[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames]
+    ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
+    per_frame = [ctx.device_call(f, ptr(dev_in)[k], ptr(out_pf)[k]) for k, f in enumerate(frames)]
+    batch = ctx.batch_call(frames, ptr(dev_in), ptr(out_b))
+    for c in per_frame:
+        c()
+    batch()
+    ctx.finish()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for c in per_frame:
+            c()
+    ctx.finish()
+    hip.synchronize()
+    t_pf = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch()
+    ctx.finish()
+    hip.synchronize()
+    t_b = (time.perf_counter() - t0) / reps
+    exact = all(np.array_equal(a.download(), b.download())
+                for fa, fb in zip(out_pf, out_b) for a, b in zip(fa, fb))
+    mpix = sum(f.output_luma_pixels for f in frames) / 1e6
+    ctx.close()
+    return {
+        "workload": f"{n} frames: 360p 25% / 720p 25% / 1080p 35% / 2160p 15%, 8- or 10-bit 4:2:0, "
+                    "superres on half (denominator 9..16), CDEF + LR (64x64 units)",
+        "mpix_per_batch": round(mpix, 3),
+        "per_frame_mpix_s": round(mpix / t_pf, 2),
+        "batched_mpix_s": round(mpix / t_b, 2),
+        "batched_over_per_frame": round(t_pf / t_b, 3),
+        "batched_ms_per_batch": round(t_b * 1e3, 3),
+        "bit_exact_vs_per_frame": bool(exact),
+        "reps": reps,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +141,8 @@ def main():
         step(i)
     prof = ctx.profile_read()
     ctx.profile(False)
+
+    batched = measure_mixed_batch(av1pf, hip, workloads, reps=max(3, args.steps // 20))
 
     # Bit-exactness of every distinct frame against the oracle.
     from oracle import oracle as O
@@ -163,6 +213,7 @@ def main():
             "traffic": None,
             "algorithmic_bytes_per_launch": kernel_bytes[dominant],
         },
+        "batched": batched,
         "cpu_baseline": {
             "value": round(cpu_mpix_s, 4),
             "unit": "Mpix/s",

@@ -228,6 +228,19 @@ vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
                                           vaco_av1pf_picture *out,
                                           void *stream);
 
+/* Batched form of vaco_av1pf_frame_device: n (<= 256) frames of any
+ * geometry, bit depth and chroma layout, each with its own desc and
+ * device planes. All frames' work descriptors and side information go up
+ * in one copy, and each stage kernel is launched once over the tiles of
+ * every frame. Output is identical to n vaco_av1pf_frame_device calls.
+ * Same lifetime rule and deferred errors as vaco_av1pf_frame_device (the
+ * first failing frame's error is reported). No reference peer: the
+ * reference filters one frame per decode_frame call. */
+vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32_t n,
+                                          const vaco_av1pf_frame_desc *descs,
+                                          const vaco_av1pf_picture *ins,
+                                          vaco_av1pf_picture *outs, void *stream);
+
 /* Wait for the context's enqueued work and report deferred errors. */
 vaco_av1pf_status vaco_av1pf_finish(vaco_av1pf_ctx *ctx);
 

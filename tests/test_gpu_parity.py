@@ -255,3 +255,40 @@ def test_gpu_fuel_exhaustion_matches_reference_accounting(ctx):
     with pytest.raises(av1pf.Av1Error) as e:
         ctx.frame(f, fuel=need - 1)
     assert e.value.status == abi.LIMIT_EXCEEDED and e.value.message == "fuel"
+
+
+# -------------------------------------------------------------- batched ---
+
+def test_gpu_batch_of_mixed_frames_matches_per_frame_and_oracle(ctx):
+    from vaco_amd import hip
+    say("batch: mixed geometry / depth / layout / superres")
+    frames = [F.random_frame(**c) for c in RANDOM_CASES[:14]]
+    frames += [av1pf.generate(workloads.cfg0(), k) for k in range(3)]
+    dev_in = [[hip.DeviceArray.from_host(p) for p in f.planes] for f in frames]
+    dev_b = [[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames]
+    ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
+    ctx.batch_device(frames, ptr(dev_in), ptr(dev_b))
+    ctx.finish()
+    for k, f in enumerate(frames):
+        got = [t.download() for t in dev_b[k]]
+        assert_frame_equal(got, O.frame(f.desc, f.planes), f"batch/{k}")
+    # a second, different batch reuses the grown workspace
+    ctx.batch_device(frames[::-1], ptr(dev_in[::-1]), ptr(dev_b[::-1]))
+    ctx.finish()
+    for k, f in enumerate(frames):
+        assert_frame_equal([t.download() for t in dev_b[k]], O.frame(f.desc, f.planes), f"batch2/{k}")
+
+
+def test_gpu_batch_defers_the_first_failing_frames_error(ctx):
+    from vaco_amd import hip
+    say("batch: deferred error")
+    frames = [F.random_frame(50 + k, 64, 64, skip_p=0.0, idx_neg_p=0.0) for k in range(3)]
+    frames[1].planes[0][9, 9] = 999
+    dev_in = [[hip.DeviceArray.from_host(p) for p in f.planes] for f in frames]
+    dev_out = [[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames]
+    ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
+    ctx.batch_device(frames, ptr(dev_in), ptr(dev_out))
+    with pytest.raises(av1pf.Av1Error) as e:
+        ctx.finish()
+    assert e.value.message == "CDEF sample exceeds bit depth"
+    assert_frame_equal([t.download() for t in dev_out[2]], O.frame(frames[2].desc, frames[2].planes), "ok frame")

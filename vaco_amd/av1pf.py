@@ -109,9 +109,11 @@ def lib():
         L.vaco_av1pf_gen_frame.argtypes = [P(GenConfig), C.c_uint32, P(abi.FrameDesc), P(abi.Picture),
                                            P(C.c_int8), P(C.c_uint8), P(abi.LrUnit), P(abi.LrUnit),
                                            P(abi.LrUnit)]
+        L.vaco_av1pf_batch_device.argtypes = [ctxp, C.c_uint32, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture),
+                                              C.c_void_p]
         L.vaco_av1pf_profile.argtypes = [ctxp, C.c_int]
         L.vaco_av1pf_profile_read.argtypes = [ctxp, P(C.c_double), P(C.c_uint64)]
-        for name in ("vaco_av1pf_profile", "vaco_av1pf_profile_read", "vaco_av1pf_create", "vaco_av1pf_frame", "vaco_av1pf_frame_device", "vaco_av1pf_finish",
+        for name in ("vaco_av1pf_batch_device", "vaco_av1pf_profile", "vaco_av1pf_profile_read", "vaco_av1pf_create", "vaco_av1pf_frame", "vaco_av1pf_frame_device", "vaco_av1pf_finish",
                      "vaco_av1pf_cdef_blocks", "vaco_av1pf_upscale_plane", "vaco_av1pf_restore_plane",
                      "vaco_av1pf_frame_output_geometry", "vaco_av1pf_frame_input_geometry",
                      "vaco_av1pf_gen_get_layout", "vaco_av1pf_gen_frame"):
@@ -244,6 +246,34 @@ class Context:
     def frame_device(self, frame: Frame, in_ptrs, out_ptrs, stream: int | None = None):
         """The stage on device planes (integer device addresses, dense rows)."""
         self.device_call(frame, in_ptrs, out_ptrs, stream)()
+
+    def batch_call(self, frames, in_ptrs, out_ptrs, stream: int | None = None):
+        """A zero-argument callable enqueueing one batched call
+        (vaco_av1pf_batch_device) over `frames` on these device planes."""
+        n = len(frames)
+        descs = (abi.FrameDesc * n)()
+        pins, pouts = (abi.Picture * n)(), (abi.Picture * n)()
+        for k, f in enumerate(frames):
+            descs[k] = f.desc
+            geo_out = f.output_geometry()
+            for i in range(f.nplanes):
+                h, w = f.planes[i].shape
+                pins[k].plane[i] = abi.device_plane(in_ptrs[k][i], w, h)
+                pouts[k].plane[i] = abi.device_plane(out_ptrs[k][i], geo_out[i][1], geo_out[i][0])
+        fn, h = self._lib.vaco_av1pf_batch_device, self.handle
+        st = C.c_void_p(stream or 0)
+        check = self._check
+
+        def call():
+            rc = fn(h, n, descs, pins, pouts, st)
+            if rc:
+                check(rc)
+
+        call.keep = (list(frames), descs, pins, pouts)
+        return call
+
+    def batch_device(self, frames, in_ptrs, out_ptrs, stream: int | None = None):
+        self.batch_call(frames, in_ptrs, out_ptrs, stream)()
 
     def finish(self):
         self._check(self._lib.vaco_av1pf_finish(self.handle))

@@ -40,6 +40,7 @@ using namespace av1pf;
 namespace {
 
 constexpr int kSlots = 8;
+constexpr uint32_t kMaxBatch = 256;  // frames per batched call
 
 // One in-flight frame: its staging area (pinned host + device mirror), its
 // status word (coherent pinned host memory, raised by the kernels), and
@@ -49,12 +50,12 @@ struct Arena {
     uint8_t *h = nullptr;  // pinned host
     uint8_t *d = nullptr;  // device mirror
     size_t cap = 0, used = 0;
-    hipEvent_t done = nullptr;  // the frame that used this slot has finished
+    hipEvent_t done = nullptr;  // the frames that used this slot have finished
     bool pending = false;
-    uint32_t *status = nullptr;
-    bool check = false;  // status must be checked before reuse
-    vaco_av1pf_frame_desc desc;
-    vaco_av1pf_picture in;
+    uint32_t *status = nullptr;  // kMaxBatch words, one per frame
+    // frames whose status must be checked before reuse
+    std::vector<vaco_av1pf_frame_desc> descs;
+    std::vector<vaco_av1pf_picture> ins;
 
     template <class T>
     T *take(size_t n, T **dev)
@@ -85,6 +86,9 @@ struct vaco_av1pf_ctx {
     // first error found by a deferred check (device entry), reported by finish
     vaco_av1pf_status deferred = VACO_AV1PF_OK;
     const char *deferred_msg = "";
+    // device intermediates of batched calls (grown on demand)
+    uint16_t *ws = nullptr;
+    size_t ws_cap = 0;  // samples
     // kernel timing (vaco_av1pf_profile)
     struct ProfRec {
         int kernel;
@@ -137,8 +141,25 @@ static vaco_av1pf_status take_slot(vaco_av1pf_ctx *ctx, Arena **out)
     if (st != VACO_AV1PF_OK)
         return st;
     a.used = 0;
-    *a.status = 0;
+    memset(a.status, 0, kMaxBatch * sizeof(uint32_t));
     *out = &a;
+    return VACO_AV1PF_OK;
+}
+
+// Grow a slot's staging buffers (its previous frames have finished).
+static vaco_av1pf_status grow_slot(vaco_av1pf_ctx *ctx, Arena *a, size_t bytes)
+{
+    if (bytes <= a->cap)
+        return VACO_AV1PF_OK;
+    bytes = std::max(bytes, a->cap * 2);
+    HIPCHECK(hipHostFree(a->h));
+    HIPCHECK(hipFree(a->d));
+    a->h = nullptr;
+    a->d = nullptr;
+    a->cap = 0;
+    HIPCHECK(hipHostMalloc((void **)&a->h, bytes, hipHostMallocDefault));
+    HIPCHECK(hipMalloc((void **)&a->d, bytes));
+    a->cap = bytes;
     return VACO_AV1PF_OK;
 }
 
@@ -216,12 +237,13 @@ extern "C" vaco_av1pf_status vaco_av1pf_create(int device, uint32_t max_width, u
     }
     // Status words: coherent so kernel atomics are visible to the host after
     // the frame's event completes; one 64-byte line per slot.
-    e = hipHostMalloc((void **)&ctx->h_status, kSlots * 64, hipHostMallocCoherent);
+    e = hipHostMalloc((void **)&ctx->h_status, kSlots * kMaxBatch * sizeof(uint32_t),
+                      hipHostMallocCoherent);
     if (e != hipSuccess)
         return bail(e);
     for (int i = 0; i < kSlots; i++) {
-        ctx->slot[i].status = ctx->h_status + i * 16;
-        *ctx->slot[i].status = 0;
+        ctx->slot[i].status = ctx->h_status + i * kMaxBatch;
+        memset(ctx->slot[i].status, 0, kMaxBatch * sizeof(uint32_t));
     }
     *out = ctx;
     return VACO_AV1PF_OK;
@@ -248,6 +270,8 @@ extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
         hipEventDestroy(ev);
     if (ctx->pool)
         hipFree(ctx->pool);
+    if (ctx->ws)
+        hipFree(ctx->ws);
     if (ctx->stream)
         hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -432,51 +456,91 @@ static int cdef_mode(const vaco_av1pf_frame_desc &d)
     return d.subsampling_x ? 2 : 3;
 }
 
-// Build descriptors for one frame into `a` and launch the stage on stream
-// `s`. `in`/`out` are device planes. `status` is the device status word.
-static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_desc &d,
-                                       const FrameGeo &g, const DevPlane in[3],
-                                       const DevPlane out[3], Arena *a, uint32_t *d_status,
-                                       hipStream_t s)
+// Device intermediates (CDEF image, upscaled images) of the frames of one
+// launch set, carved from one allocation.
+struct Scratch {
+    uint16_t *base = nullptr;
+    size_t cap = 0, used = 0;  // in samples
+    uint16_t *take(size_t n)
+    {
+        const size_t a = (used + 127) & ~size_t(127);
+        if (a + n > cap)
+            return nullptr;
+        used = a + n;
+        return base + a;
+    }
+};
+
+// Samples of intermediates a frame needs (see plan_frame).
+static size_t scratch_need(const FrameGeo &g)
+{
+    size_t n = 0;
+    for (int p = 0; p < g.nplanes; p++) {
+        if (g.cdef && (g.superres || g.lr))
+            n += (size_t)g.in_w[p] * g.in_h[p] + 128;
+        if (g.superres && g.lr)
+            n += 2 * ((size_t)g.sr_out_w[p] * g.sr_h[p] + 128);
+    }
+    return n;
+}
+
+// Work descriptors of one launch set: every frame's CDEF superblocks, superres
+// rows and restoration tiles, each kernel's items with running block offsets.
+struct Plan {
+    std::vector<CdefFrame> cdef[4];  // by chroma layout (cdef_mode)
+    uint32_t cdef_blocks[4] = {};
+    std::vector<SrJob> sr;
+    uint32_t sr_blocks = 0;
+    std::vector<LrPlane> lr;
+    uint32_t lr_blocks = 0;
+    struct Copy {
+        DevPlane src, dst;
+        uint32_t w, h;
+    };
+    std::vector<Copy> copies;  // frames in which no stage runs
+};
+
+#define TAKE(ptr, dptr, type, n)                                                    \
+    type *dptr;                                                                     \
+    type *ptr = a->take<type>((n), &dptr);                                          \
+    if (!ptr)                                                                       \
+        return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
+
+// Append one frame's work (decode.rs:554-559) to `plan`: side information
+// goes to the staging arena `a`, intermediates come from `ws`.
+static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_desc &d,
+                                    const FrameGeo &g, const DevPlane in[3], const DevPlane out[3],
+                                    Arena *a, uint32_t *status, Scratch &ws, Plan &plan)
 {
     const int np = g.nplanes;
     DevPlane cdef_out[3], cur[3], pre[3];
-    for (int p = 0; p < np; p++) {
-        cdef_out[p] = in[p];
-        cur[p] = in[p];
-        pre[p] = in[p];
+    for (int p = 0; p < np; p++)
+        cdef_out[p] = cur[p] = pre[p] = in[p];
+    if (!g.cdef && !g.superres && !g.lr) {
+        // Nothing runs: the stage's output is its input (decode.rs:741-748).
+        for (int p = 0; p < np; p++)
+            plan.copies.push_back({in[p], out[p], g.in_w[p], g.in_h[p]});
+        return VACO_AV1PF_OK;
     }
-    const bool final_cdef = !g.superres && !g.lr;
-
-    CdefFrame *hc = nullptr, *dc = nullptr;
-    SrJob jobs[6], *dj = nullptr;
-    int njobs = 0;
-    uint32_t sr_blocks = 0;
-    LrPlane lrp[3], *dl = nullptr;
-    int nlr = 0;
-    uint32_t lr_blocks = 0;
-    uint32_t cdef_blocks = 0;
 
     // ---- CDEF (decode.rs:739-861)
     if (g.cdef) {
-        hc = a->take<CdefFrame>(1, &dc);
-        int8_t *didx;
-        uint8_t *dskip;
         const size_t nidx = (size_t)g.idx_rows * d.cdef_stride;
         const size_t nskip = (size_t)(g.mi_rows / 2) * (g.mi_cols / 2);
-        int8_t *hidx = a->take<int8_t>(nidx, &didx);
-        uint8_t *hskip = a->take<uint8_t>(nskip, &dskip);
-        if (!hc || !hidx || !hskip)
-            return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
+        TAKE(hidx, didx, int8_t, nidx);
+        TAKE(hskip, dskip, uint8_t, nskip);
         memcpy(hidx, d.cdef_idx, nidx);
         memcpy(hskip, d.skip8, nskip);
-        CdefFrame &f = *hc;
+        CdefFrame f;
         memset(&f, 0, sizeof f);
         for (int p = 0; p < np; p++) {
-            if (final_cdef)
+            if (!g.superres && !g.lr) {
                 cdef_out[p] = out[p];
-            else
-                cdef_out[p] = DevPlane{ctx->img[1][p], g.in_w[p]};
+            } else {
+                cdef_out[p] = DevPlane{ws.take((size_t)g.in_w[p] * g.in_h[p]), g.in_w[p]};
+                if (!cdef_out[p].data)
+                    return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "workspace exhausted");
+            }
             f.in[p] = in[p].data;
             f.out[p] = cdef_out[p].data;
             f.width[p] = g.in_w[p];
@@ -490,8 +554,7 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
         f.skip_stride = g.mi_cols / 2;
         f.sb_cols = (g.in_w[0] + 63) / 64;
         f.sb_rows = (g.in_h[0] + 63) / 64;
-        f.status = d_status;
-        f.block_begin = 0;
+        f.status = status;
         f.bit_depth = d.bit_depth;
         f.ss_x = d.subsampling_x;
         f.ss_y = d.subsampling_y;
@@ -503,7 +566,10 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
             f.uv_pri[i] = d.cdef.uv[i].primary;
             f.uv_sec[i] = d.cdef.uv[i].secondary;
         }
-        cdef_blocks = f.sb_cols * f.sb_rows;
+        const int mode = cdef_mode(d);
+        f.block_begin = plan.cdef_blocks[mode];
+        plan.cdef_blocks[mode] += f.sb_cols * f.sb_rows;
+        plan.cdef[mode].push_back(f);
         for (int p = 0; p < np; p++)
             cur[p] = cdef_out[p];
     }
@@ -523,7 +589,7 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
         for (int img = 0; img < (g.lr ? 2 : 1); img++) {
             for (int p = 0; p < np; p++) {
                 // img 0: the CDEF image; img 1: the pre-CDEF image, only at
-                // rows restoration reads across stripe borders.
+                // the rows restoration reads across stripe borders.
                 const uint32_t *drows = nullptr;
                 uint32_t nrows = g.sr_h[p];
                 if (img == 1) {
@@ -533,25 +599,23 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
                     lr_border_rows(lr_rows[p], g.lr_h[p], rows);
                     if (rows.empty())
                         continue;
-                    uint32_t *dr;
-                    uint32_t *hr = a->take<uint32_t>(rows.size(), &dr);
-                    if (!hr)
-                        return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
+                    TAKE(hr, dr, uint32_t, rows.size());
                     memcpy(hr, rows.data(), rows.size() * 4);
                     drows = dr;
                     nrows = (uint32_t)rows.size();
                 }
                 DevPlane src = img == 0 ? cur[p] : in[p];
-                DevPlane dst;
-                if (img == 0)
-                    dst = g.lr ? DevPlane{ctx->img[2][p], g.sr_out_w[p]} : out[p];
-                else
-                    dst = DevPlane{ctx->img[3][p], g.sr_out_w[p]};
-                SrJob &j = jobs[njobs++];
+                DevPlane dst = out[p];
+                if (g.lr) {
+                    dst = DevPlane{ws.take((size_t)g.sr_out_w[p] * g.sr_h[p]), g.sr_out_w[p]};
+                    if (!dst.data)
+                        return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "workspace exhausted");
+                }
+                SrJob j;
                 memset(&j, 0, sizeof j);
                 j.in = src.data;
                 j.out = dst.data;
-                j.status = d_status;
+                j.status = status;
                 j.rows = drows;
                 j.in_width = g.in_w[p];
                 j.in_stride = src.stride;
@@ -561,27 +625,22 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
                 j.segs = (g.sr_out_w[p] + kSrSeg - 1) / kSrSeg;
                 j.step = g.sr_step[p];
                 j.start = g.sr_start[p];
-                j.block_begin = sr_blocks;
+                j.block_begin = plan.sr_blocks;
                 j.max_sample = (uint16_t)((1u << d.bit_depth) - 1);
-                sr_blocks += j.n_rows * j.segs;
+                plan.sr_blocks += j.n_rows * j.segs;
+                plan.sr.push_back(j);
                 if (img == 0)
                     cur[p] = dst;
                 else
                     pre[p] = dst;
             }
         }
-        if (njobs) {
-            SrJob *hj = a->take<SrJob>(njobs, &dj);
-            if (!hj)
-                return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
-            memcpy(hj, jobs, sizeof(SrJob) * njobs);
-        }
     }
 
     // ---- loop restoration (decode.rs:683-737, restoration.rs:178-207)
     if (g.lr) {
         for (int p = 0; p < np; p++) {
-            LrPlane &L = lrp[nlr];
+            LrPlane L;
             memset(&L, 0, sizeof L);
             const bool none = d.lr_type[p] == VACO_AV1PF_RESTORE_NONE;
             std::vector<LrRowSeg> rows_none;
@@ -590,32 +649,20 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
             const std::vector<LrColSeg> *cols = &lr_cols[p];
             if (none) {
                 // copy_visible_plane (decode.rs:673-681): 64x64 copy tiles
-                for (uint32_t y = 0; y < g.lr_h[p]; y += 64) {
-                    LrRowSeg r{(int)y, (int)std::min<uint32_t>(64, g.lr_h[p] - y), 0, 0, 0};
-                    rows_none.push_back(r);
-                }
-                for (uint32_t x = 0; x < g.lr_w[p]; x += 64) {
-                    LrColSeg c{(int)x, (int)std::min<uint32_t>(64, g.lr_w[p] - x), 0};
-                    cols_none.push_back(c);
-                }
+                for (uint32_t y = 0; y < g.lr_h[p]; y += 64)
+                    rows_none.push_back({(int)y, (int)std::min<uint32_t>(64, g.lr_h[p] - y), 0, 0, 0});
+                for (uint32_t x = 0; x < g.lr_w[p]; x += 64)
+                    cols_none.push_back({(int)x, (int)std::min<uint32_t>(64, g.lr_w[p] - x), 0});
                 rows = &rows_none;
                 cols = &cols_none;
             }
-            LrRowSeg *drs;
-            LrColSeg *dcs;
-            LrRowSeg *hrs = a->take<LrRowSeg>(rows->size(), &drs);
-            LrColSeg *hcs = a->take<LrColSeg>(cols->size(), &dcs);
-            if (!hrs || !hcs)
-                return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
+            TAKE(hrs, drs, LrRowSeg, rows->size());
+            TAKE(hcs, dcs, LrColSeg, cols->size());
             memcpy(hrs, rows->data(), rows->size() * sizeof(LrRowSeg));
             memcpy(hcs, cols->data(), cols->size() * sizeof(LrColSeg));
-            L.units = nullptr;
             if (!none) {
-                LrUnit *du;
-                LrUnit *hu = a->take<LrUnit>(d.lr_unit_count[p], &du);
-                if (!hu)
-                    return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
                 static_assert(sizeof(LrUnit) == sizeof(vaco_av1pf_lr_unit), "unit layout");
+                TAKE(hu, du, LrUnit, d.lr_unit_count[p]);
                 memcpy(hu, d.lr_units[p], sizeof(LrUnit) * d.lr_unit_count[p]);
                 L.units = du;
                 uint32_t ucols, urows;
@@ -627,7 +674,7 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
             L.out = out[p].data;
             L.rows = drs;
             L.cols = dcs;
-            L.status = d_status;
+            L.status = status;
             L.before_stride = pre[p].stride;
             L.after_stride = cur[p].stride;
             L.out_stride = out[p].stride;
@@ -635,38 +682,78 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
             L.height = g.lr_h[p];
             L.n_rows = (uint32_t)rows->size();
             L.n_cols = (uint32_t)cols->size();
-            L.block_begin = lr_blocks;
+            L.block_begin = plan.lr_blocks;
             L.bit_depth = d.bit_depth;
             L.ss_y = (uint8_t)(p && d.subsampling_y);
-            lr_blocks += L.n_rows * L.n_cols;
-            nlr++;
+            plan.lr_blocks += L.n_rows * L.n_cols;
+            plan.lr.push_back(L);
         }
-        LrPlane *hl = a->take<LrPlane>(nlr, &dl);
-        if (!hl)
-            return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
-        memcpy(hl, lrp, sizeof(LrPlane) * nlr);
     }
+    return VACO_AV1PF_OK;
+}
 
+// Upload the plan's descriptors with the arena (one copy) and launch each
+// kernel once over all of its items.
+static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Arena *a, hipStream_t s)
+{
+    const CdefFrame *dc[4] = {};
+    for (int m = 0; m < 4; m++) {
+        if (plan.cdef[m].empty())
+            continue;
+        TAKE(h, d, CdefFrame, plan.cdef[m].size());
+        memcpy(h, plan.cdef[m].data(), plan.cdef[m].size() * sizeof(CdefFrame));
+        dc[m] = d;
+    }
+    const SrJob *dj = nullptr;
+    if (!plan.sr.empty()) {
+        TAKE(h, d, SrJob, plan.sr.size());
+        memcpy(h, plan.sr.data(), plan.sr.size() * sizeof(SrJob));
+        dj = d;
+    }
+    const LrPlane *dl = nullptr;
+    if (!plan.lr.empty()) {
+        TAKE(h, d, LrPlane, plan.lr.size());
+        memcpy(h, plan.lr.data(), plan.lr.size() * sizeof(LrPlane));
+        dl = d;
+    }
     vaco_av1pf_status st = upload_slot(ctx, a, s);
     if (st != VACO_AV1PF_OK)
         return st;
-    if (!g.cdef && !g.superres && !g.lr) {
-        // Nothing runs: the stage's output is its input (decode.rs:741-748).
-        for (int p = 0; p < np; p++)
-            HIPCHECK(hipMemcpy2DAsync(out[p].data, out[p].stride * 2, in[p].data, in[p].stride * 2,
-                                      g.in_w[p] * 2, g.in_h[p], hipMemcpyDeviceToDevice, s));
-        return VACO_AV1PF_OK;
-    }
-    if (g.cdef)
-        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, s,
-                              [&] { return launch_cdef(dc, 1, cdef_blocks, cdef_mode(d), s); }));
-    if (njobs)
-        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES, s,
-                              [&] { return launch_superres(dj, njobs, sr_blocks, s); }));
-    if (nlr)
-        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION, s,
-                              [&] { return launch_lr(dl, nlr, lr_blocks, s); }));
+    for (const auto &c : plan.copies)
+        HIPCHECK(hipMemcpy2DAsync(c.dst.data, c.dst.stride * 2, c.src.data, c.src.stride * 2, c.w * 2,
+                                  c.h, hipMemcpyDeviceToDevice, s));
+    for (int m = 0; m < 4; m++)
+        if (dc[m])
+            HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, s, [&] {
+                return launch_cdef(dc[m], (int)plan.cdef[m].size(), plan.cdef_blocks[m], m, s);
+            }));
+    if (dj)
+        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES, s, [&] {
+            return launch_superres(dj, (int)plan.sr.size(), plan.sr_blocks, s);
+        }));
+    if (dl)
+        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION, s, [&] {
+            return launch_lr(dl, (int)plan.lr.size(), plan.lr_blocks, s);
+        }));
     return VACO_AV1PF_OK;
+}
+
+#undef TAKE
+
+// One frame on the per-frame path: intermediates from the context's images.
+static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_desc &d,
+                                       const FrameGeo &g, const DevPlane in[3],
+                                       const DevPlane out[3], Arena *a, uint32_t *status,
+                                       hipStream_t s)
+{
+    Scratch ws;
+    ws.base = ctx->img[1][0];
+    ws.cap = ctx->plane_cap * 9;
+    Plan plan;
+    vaco_av1pf_status st = plan_frame(ctx, d, g, in, out, a, status, ws, plan);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    return launch_plan(ctx, plan, a, s);
 }
 
 // ---------------------------------------------------------- profiling --
@@ -883,9 +970,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
     if ((st = close_slot(ctx, a, s)) != VACO_AV1PF_OK)
         return st;
     // The status is settled when the slot is reused or at finish().
-    a->check = true;
-    a->desc = *desc;
-    a->in = *in;
+    a->descs.assign(1, *desc);
+    a->ins.assign(1, *in);
     return VACO_AV1PF_OK;
 }
 
@@ -893,31 +979,134 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
 // replay the reference's checks over a host copy of the device input.
 static vaco_av1pf_status resolve_slot(vaco_av1pf_ctx *ctx, Arena &a)
 {
-    if (!a.check)
-        return VACO_AV1PF_OK;
-    a.check = false;
-    if (!(*a.status & kFlagSampleRange) || ctx->deferred != VACO_AV1PF_OK)
-        return VACO_AV1PF_OK;
-    FrameGeo g;
-    frame_geometry(a.desc, &g);
-    std::vector<uint16_t> host[3];
-    vaco_av1pf_picture hpic = a.in;
+    for (size_t i = 0; i < a.descs.size(); i++) {
+        if (!(a.status[i] & kFlagSampleRange) || ctx->deferred != VACO_AV1PF_OK)
+            continue;
+        const vaco_av1pf_frame_desc &d = a.descs[i];
+        const vaco_av1pf_picture &in = a.ins[i];
+        FrameGeo g;
+        frame_geometry(d, &g);
+        std::vector<uint16_t> host[3];
+        vaco_av1pf_picture hpic = in;
+        for (int p = 0; p < g.nplanes; p++) {
+            host[p].resize((size_t)g.in_w[p] * g.in_h[p]);
+            HIPCHECK(hipMemcpy2D(host[p].data(), g.in_w[p] * 2, in.plane[p].data,
+                                 in.plane[p].stride * 2, g.in_w[p] * 2, g.in_h[p],
+                                 hipMemcpyDeviceToHost));
+            hpic.plane[p].data = host[p].data();
+            hpic.plane[p].stride = g.in_w[p];
+        }
+        HostPlane hin[3];
+        host_planes_of(&hpic, g.nplanes, hin);
+        const char *msg = "";
+        vaco_av1pf_status r = replay_checks(d, g, hin, nullptr, &msg);
+        if (r != VACO_AV1PF_OK) {
+            ctx->deferred = r;
+            ctx->deferred_msg = msg;
+        }
+    }
+    a.descs.clear();
+    a.ins.clear();
+    return VACO_AV1PF_OK;
+}
+
+// Staging bytes one frame can need (side information, segment lists,
+// descriptors), an upper bound.
+static size_t arena_need(const vaco_av1pf_frame_desc &d, const FrameGeo &g)
+{
+    size_t n = 4096;
+    if (g.cdef)
+        n += (size_t)g.idx_rows * d.cdef_stride + (size_t)(g.mi_rows / 2) * (g.mi_cols / 2) + 1024;
     for (int p = 0; p < g.nplanes; p++) {
-        host[p].resize((size_t)g.in_w[p] * g.in_h[p]);
-        HIPCHECK(hipMemcpy2D(host[p].data(), g.in_w[p] * 2, a.in.plane[p].data,
-                             a.in.plane[p].stride * 2, g.in_w[p] * 2, g.in_h[p],
-                             hipMemcpyDeviceToHost));
-        hpic.plane[p].data = host[p].data();
-        hpic.plane[p].stride = g.in_w[p];
+        n += (size_t)d.lr_unit_count[p] * sizeof(LrUnit) + 512;
+        n += ((size_t)g.lr_h[p] / 16 + 4) * sizeof(LrRowSeg) + ((size_t)g.lr_w[p] / 16 + 4) * sizeof(LrColSeg);
+        n += ((size_t)g.lr_h[p] / 4 + 16) * sizeof(uint32_t);
     }
-    HostPlane hin[3];
-    host_planes_of(&hpic, g.nplanes, hin);
-    const char *msg = "";
-    vaco_av1pf_status r = replay_checks(a.desc, g, hin, nullptr, &msg);
-    if (r != VACO_AV1PF_OK) {
-        ctx->deferred = r;
-        ctx->deferred_msg = msg;
+    return n;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32_t n,
+                                                     const vaco_av1pf_frame_desc *descs,
+                                                     const vaco_av1pf_picture *ins,
+                                                     vaco_av1pf_picture *outs, void *stream)
+{
+    if (!ctx || (n && (!descs || !ins || !outs)))
+        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "null argument");
+    if (n > kMaxBatch)
+        return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "batch larger than 256 frames");
+    ctx->err = "";
+    if (n == 0)
+        return VACO_AV1PF_OK;
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    std::vector<FrameGeo> geo(n);
+    size_t ws_need = 0, arena = 1 << 16;
+    for (uint32_t i = 0; i < n; i++) {
+        frame_geometry(descs[i], &geo[i]);
+        vaco_av1pf_status st = check_planes(ctx, geo[i], descs[i].cdef_stride, &ins[i], &outs[i]);
+        if (st != VACO_AV1PF_OK)
+            return st;
+        if (!params_clean(descs[i], geo[i])) {
+            // Parameter errors need the samples to order them exactly.
+            std::vector<uint16_t> host[3];
+            vaco_av1pf_picture hpic = ins[i];
+            HIPCHECK(hipStreamSynchronize(s));
+            for (int p = 0; p < geo[i].nplanes; p++) {
+                host[p].resize((size_t)geo[i].in_w[p] * geo[i].in_h[p]);
+                HIPCHECK(hipMemcpy2D(host[p].data(), geo[i].in_w[p] * 2, ins[i].plane[p].data,
+                                     ins[i].plane[p].stride * 2, geo[i].in_w[p] * 2, geo[i].in_h[p],
+                                     hipMemcpyDeviceToHost));
+                hpic.plane[p].data = host[p].data();
+                hpic.plane[p].stride = geo[i].in_w[p];
+            }
+            HostPlane hin[3];
+            host_planes_of(&hpic, geo[i].nplanes, hin);
+            const char *msg = "";
+            vaco_av1pf_status r = replay_checks(descs[i], geo[i], hin, nullptr, &msg);
+            if (r != VACO_AV1PF_OK)
+                return fail(ctx, r, msg);
+        }
+        ws_need += scratch_need(geo[i]);
+        arena += arena_need(descs[i], geo[i]);
     }
+    // Intermediates of every frame of the batch; grown only when a batch
+    // needs more than any batch before it.
+    if (ws_need > ctx->ws_cap) {
+        HIPCHECK(hipStreamSynchronize(s));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        if (ctx->ws)
+            HIPCHECK(hipFree(ctx->ws));
+        ctx->ws = nullptr;
+        ctx->ws_cap = 0;
+        HIPCHECK(hipMalloc((void **)&ctx->ws, ws_need * sizeof(uint16_t)));
+        ctx->ws_cap = ws_need;
+    }
+    Arena *a;
+    vaco_av1pf_status st = take_slot(ctx, &a);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    if ((st = grow_slot(ctx, a, arena)) != VACO_AV1PF_OK)
+        return st;
+    Scratch ws;
+    ws.base = ctx->ws;
+    ws.cap = ctx->ws_cap;
+    Plan plan;
+    for (uint32_t i = 0; i < n; i++) {
+        DevPlane din[3], dout[3];
+        for (int p = 0; p < geo[i].nplanes; p++) {
+            din[p] = DevPlane{ins[i].plane[p].data, ins[i].plane[p].stride};
+            dout[p] = DevPlane{outs[i].plane[p].data, outs[i].plane[p].stride};
+        }
+        st = plan_frame(ctx, descs[i], geo[i], din, dout, a, a->status + i, ws, plan);
+        if (st != VACO_AV1PF_OK)
+            return st;
+    }
+    if ((st = launch_plan(ctx, plan, a, s)) != VACO_AV1PF_OK)
+        return st;
+    if ((st = close_slot(ctx, a, s)) != VACO_AV1PF_OK)
+        return st;
+    a->descs.assign(descs, descs + n);
+    a->ins.assign(ins, ins + n);
     return VACO_AV1PF_OK;
 }
 

@@ -46,3 +46,26 @@ def cfg2() -> GenConfig:
 
 
 CONFIGS = {0: cfg0, 1: cfg1, 2: cfg2}
+
+
+MIXED_SHAPES = [((640, 360), 0.25), ((1280, 720), 0.25), ((1920, 1080), 0.35), ((3840, 2160), 0.15)]
+
+
+def mixed_batch(n=64, seed=BASE_SEED + 3):
+    """Per-frame configurations of the mixed-geometry batch: shape drawn from
+    360p 25% / 720p 25% / 1080p 35% / 2160p 15%, bit depth U{8, 10}, superres
+    on half the frames (denominator U[9, 16]), CDEF and restoration parameters
+    as cfg0/cfg2 (64x64 units). Returns [(GenConfig, frame_index)]."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    cum = np.cumsum([p for _, p in MIXED_SHAPES])
+    for i in range(n):
+        (w, h) = MIXED_SHAPES[int(np.searchsorted(cum, rng.random()))][0]
+        bd = 8 if rng.random() < 0.5 else 10
+        denom = 0 if rng.random() < 0.5 else 8
+        c = make(w, h, bd, sigma=6 if bd == 8 else 24, edges=0, unit=64, superres_denom=denom,
+                 seed=seed + 1000 * i)
+        out.append((c, i))
+    return out
