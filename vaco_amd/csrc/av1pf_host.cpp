@@ -28,7 +28,8 @@
 namespace av1pf {
 hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, hipStream_t s);
 hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s);
-hipError_t launch_lr(const LrPlane *planes, int n, uint32_t blocks, hipStream_t s);
+hipError_t launch_lr(const LrPlane *planes, const uint32_t *const lists[3], const uint32_t counts[3],
+                     hipStream_t s);
 hipError_t launch_cdef_cases(const uint16_t *inputs, const void *params, uint8_t *dirs,
                              uint32_t *vars, uint16_t *outs, uint32_t n, hipStream_t s);
 }  // namespace av1pf
@@ -493,6 +494,9 @@ struct Plan {
     uint32_t sr_blocks = 0;
     std::vector<LrPlane> lr;
     uint32_t lr_blocks = 0;
+    // restoration tiles by kernel: 0 copy, 1 Wiener, 2 self-guided;
+    // entry = LrPlane index << 22 | tile index within the plane
+    std::vector<uint32_t> lr_tiles[3];
     struct Copy {
         DevPlane src, dst;
         uint32_t w, h;
@@ -686,6 +690,17 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             L.bit_depth = d.bit_depth;
             L.ss_y = (uint8_t)(p && d.subsampling_y);
             plan.lr_blocks += L.n_rows * L.n_cols;
+            const uint32_t item = (uint32_t)plan.lr.size();
+            for (uint32_t ri = 0; ri < L.n_rows; ri++)
+                for (uint32_t ci = 0; ci < L.n_cols; ci++) {
+                    int kind = 0;
+                    if (!none) {
+                        const uint8_t k =
+                            d.lr_units[p][(*rows)[ri].unit_row * L.unit_cols + (*cols)[ci].unit_col].kind;
+                        kind = k == VACO_AV1PF_UNIT_WIENER ? 1 : k == VACO_AV1PF_UNIT_SELF_GUIDED ? 2 : 0;
+                    }
+                    plan.lr_tiles[kind].push_back(item << 22 | (ri * L.n_cols + ci));
+                }
             plan.lr.push_back(L);
         }
     }
@@ -711,10 +726,20 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
         dj = d;
     }
     const LrPlane *dl = nullptr;
+    const uint32_t *dlist[3] = {};
+    uint32_t nlist[3] = {};
     if (!plan.lr.empty()) {
         TAKE(h, d, LrPlane, plan.lr.size());
         memcpy(h, plan.lr.data(), plan.lr.size() * sizeof(LrPlane));
         dl = d;
+        for (int k = 0; k < 3; k++) {
+            nlist[k] = (uint32_t)plan.lr_tiles[k].size();
+            if (!nlist[k])
+                continue;
+            TAKE(hl, dlk, uint32_t, nlist[k]);
+            memcpy(hl, plan.lr_tiles[k].data(), nlist[k] * sizeof(uint32_t));
+            dlist[k] = dlk;
+        }
     }
     vaco_av1pf_status st = upload_slot(ctx, a, s);
     if (st != VACO_AV1PF_OK)
@@ -732,9 +757,8 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
             return launch_superres(dj, (int)plan.sr.size(), plan.sr_blocks, s);
         }));
     if (dl)
-        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION, s, [&] {
-            return launch_lr(dl, (int)plan.lr.size(), plan.lr_blocks, s);
-        }));
+        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION, s,
+                              [&] { return launch_lr(dl, dlist, nlist, s); }));
     return VACO_AV1PF_OK;
 }
 
@@ -1270,8 +1294,19 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     lr_col_segments(cfg->width, cfg->unit_size, cols);
     const size_t nb = (size_t)before->height * before->width, na = (size_t)after->height * after->width;
     const size_t no = (size_t)cfg->width * cfg->height;
+    uint32_t ucols, urows;
+    lr_unit_counts(cfg->width, cfg->height, cfg->unit_size, &ucols, &urows);
+    // tile lists by kernel kind (0 copy, 1 Wiener, 2 self-guided)
+    std::vector<uint32_t> tiles[3];
+    for (uint32_t ri = 0; ri < rows.size(); ri++)
+        for (uint32_t ci = 0; ci < cols.size(); ci++) {
+            const uint8_t k = units[rows[ri].unit_row * ucols + cols[ci].unit_col].kind;
+            const int kind = k == VACO_AV1PF_UNIT_WIENER ? 1 : k == VACO_AV1PF_UNIT_SELF_GUIDED ? 2 : 0;
+            tiles[kind].push_back(ri * (uint32_t)cols.size() + ci);
+        }
+    const size_t ntiles = rows.size() * cols.size();
     const size_t bytes_tail = rows.size() * sizeof(LrRowSeg) + cols.size() * sizeof(LrColSeg) +
-                              n_units * sizeof(LrUnit) + sizeof(LrPlane) + 8 * 256;
+                              n_units * sizeof(LrUnit) + sizeof(LrPlane) + ntiles * 4 + 10 * 256;
     uint8_t *buf = nullptr;
     HIPCHECK(hipMalloc(&buf, (nb + na + no) * 2 + bytes_tail));
     uint16_t *db = (uint16_t *)buf, *da = db + nb, *dout = da + na;
@@ -1284,6 +1319,13 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     p8 += (cols.size() * sizeof(LrColSeg) + 255) & ~size_t(255);
     LrUnit *du = (LrUnit *)p8;
     p8 += (n_units * sizeof(LrUnit) + 255) & ~size_t(255);
+    const uint32_t *dlist[3];
+    uint32_t nlist[3];
+    for (int k = 0; k < 3; k++) {
+        dlist[k] = (const uint32_t *)p8;
+        nlist[k] = (uint32_t)tiles[k].size();
+        p8 += (nlist[k] * 4 + 255) & ~size_t(255);
+    }
     uint32_t *dst = (uint32_t *)p8;
     LrPlane L;
     memset(&L, 0, sizeof L);
@@ -1301,8 +1343,6 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     L.height = cfg->height;
     L.n_rows = (uint32_t)rows.size();
     L.n_cols = (uint32_t)cols.size();
-    uint32_t ucols, urows;
-    lr_unit_counts(cfg->width, cfg->height, cfg->unit_size, &ucols, &urows);
     L.unit_cols = ucols;
     L.block_begin = 0;
     L.bit_depth = cfg->bit_depth;
@@ -1324,10 +1364,14 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
                            ctx->stream);
     if (e == hipSuccess && n_units)
         e = hipMemcpyAsync(du, units, n_units * sizeof(LrUnit), hipMemcpyHostToDevice, ctx->stream);
+    for (int k = 0; k < 3 && e == hipSuccess; k++)
+        if (nlist[k])
+            e = hipMemcpyAsync((void *)dlist[k], tiles[k].data(), nlist[k] * 4, hipMemcpyHostToDevice,
+                               ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-        e = launch_lr(dl, 1, L.n_rows * L.n_cols, ctx->stream);
+        e = launch_lr(dl, dlist, nlist, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpy2DAsync(out->data, out->stride * 2, dout, cfg->width * 2, cfg->width * 2,
                              cfg->height, hipMemcpyDeviceToHost, ctx->stream);
