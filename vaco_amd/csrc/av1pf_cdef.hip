@@ -166,6 +166,53 @@ __device__ __forceinline__ int filter_px(const uint16_t *t, int ci, const Blk &B
     return min(max(y, mn), mx);
 }
 
+// Packed form of filter_px for two horizontally adjacent samples of one
+// block (interior superblocks: every tap available). Every quantity fits
+// int16: samples <= 4095, differences within +-4095, and the tap sum is at
+// most (4+2)*2*240 + (2+1)*4*64 = 3648, so each operation works on both
+// samples at once (v_pk_*_i16).
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s16x2 ld2(const uint16_t *t, int i)
+{
+    return s16x2{(short)t[i], (short)t[i + 1]};
+}
+
+__device__ __forceinline__ void tap2_pk(const uint16_t *t, int ci, int off, s16x2 x, s16x2 S,
+                                        s16x2 sh, s16x2 w, s16x2 &sum, s16x2 &mn, s16x2 &mx)
+{
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const s16x2 p = ld2(t, s ? ci + off : ci - off);
+        const s16x2 d = p - x;
+        const s16x2 ad = __builtin_elementwise_max(d, -d);
+        const s16x2 q = (s16x2)((u16x2)ad >> (u16x2)sh);
+        const s16x2 m = __builtin_elementwise_min(__builtin_elementwise_max(S - q, (s16x2)0), ad);
+        const s16x2 sg = d >> (s16x2)15;
+        sum += w * ((m ^ sg) - sg);
+        mn = __builtin_elementwise_min(mn, p);
+        mx = __builtin_elementwise_max(mx, p);
+    }
+}
+
+__device__ __forceinline__ s16x2 filter_pair(const uint16_t *t, int ci, const Blk &B)
+{
+    const s16x2 x = ld2(t, ci);
+    s16x2 sum = (s16x2)0, mn = x, mx = x;
+    const s16x2 P = (s16x2)(short)B.pri, Q = (s16x2)(short)B.sec;
+    const s16x2 ps = (s16x2)(short)(B.shifts & 15), qs = (s16x2)(short)(B.shifts >> 4);
+    const short t0 = (B.flags & 2) ? 3 : 4;
+    tap2_pk(t, ci, B.off[0], x, P, ps, (s16x2)t0, sum, mn, mx);
+    tap2_pk(t, ci, B.off[1], x, P, ps, (s16x2)(short)(6 - t0), sum, mn, mx);
+    tap2_pk(t, ci, B.off[2], x, Q, qs, (s16x2)2, sum, mn, mx);
+    tap2_pk(t, ci, B.off[3], x, Q, qs, (s16x2)1, sum, mn, mx);
+    tap2_pk(t, ci, B.off[4], x, Q, qs, (s16x2)2, sum, mn, mx);
+    tap2_pk(t, ci, B.off[5], x, Q, qs, (s16x2)1, sum, mn, mx);
+    const s16x2 y = x + ((sum + (s16x2)8 + (sum >> (s16x2)15)) >> (s16x2)4);
+    return __builtin_elementwise_min(__builtin_elementwise_max(y, mn), mx);
+}
+
 // Stage rows ys .. ys+ROWS-1, columns xs .. xs+COLS-1 of a plane into an LDS
 // tile at column offset 6; positions outside the plane hold kUnavailable.
 template <int ROWS, int COLS>
@@ -237,6 +284,59 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
                 if (store)
                     f.out[p][(size_t)y * f.out_stride[p] + x] = (uint16_t)res;
             }
+        }
+    }
+}
+
+// Interior superblocks: filter_sb with two samples per lane (filter_pair).
+// Luma: 16 lanes per 8x8 block, pairs in rows i0 and i0+4 at columns
+// 2*j2, 2*j2+1. Chroma: 16 lanes per block, pairs of U then V.
+template <int SX, int SY>
+__device__ __forceinline__ void filter_sb_packed(const CdefFrame &f, const uint16_t (*lt)[68 * kRow],
+                                                 const Blk (*blk)[64], const uint8_t *bact,
+                                                 bool any, int x0, int y0)
+{
+    constexpr int BW = 8 >> SX, BH = 8 >> SY, PPP = BW * BH / 2, CPX = BW * BH / 16;
+    const int tid = threadIdx.x;
+    const int b = tid >> 4, q = tid & 15;
+    const int bi = b >> 3, bj = b & 7;
+    const bool act = any && bact[b];
+    {
+        const int j = (q & 3) * 2, i0 = q >> 2;
+        Blk B = {};
+        if (act)
+            B = blk[0][b];
+        const uint16_t *t = lt[0];
+        uint16_t *dst = f.out[0];
+        const int ost = f.out_stride[0];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int i = i0 + 4 * k;
+            const int x = x0 + bj * 8 + j, y = y0 + bi * 8 + i;
+            const int ci = (2 + bi * 8 + i) * kRow + 8 + bj * 8 + j;
+            const s16x2 res = act ? filter_pair(t, ci, B) : ld2(t, ci);
+            uint16_t *o = dst + (size_t)y * ost + x;
+            o[0] = (uint16_t)res.x;
+            o[1] = (uint16_t)res.y;
+        }
+    }
+    if (f.nplanes > 1) {
+        Blk B = {};
+        if (act)
+            B = blk[1][b];
+        const int cx0 = x0 >> SX, cy0 = y0 >> SY;
+#pragma unroll
+        for (int k = 0; k < CPX; k++) {
+            const int pi = q + 16 * k;
+            const int p = pi < PPP ? 1 : 2, r = pi % PPP;
+            const int i = r / (BW / 2), j = (r % (BW / 2)) * 2;
+            const int x = cx0 + bj * BW + j, y = cy0 + bi * BH + i;
+            const int ci = (2 + bi * BH + i) * kRow + 8 + bj * BW + j;
+            const uint16_t *t = lt[p];
+            const s16x2 res = act ? filter_pair(t, ci, B) : ld2(t, ci);
+            uint16_t *o = f.out[p] + (size_t)y * f.out_stride[p] + x;
+            o[0] = (uint16_t)res.x;
+            o[1] = (uint16_t)res.y;
         }
     }
 }
@@ -337,7 +437,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     if (border)
         filter_sb<true, SX, SY>(f, lt, blk, bact, any, x0, y0);
     else
-        filter_sb<false, SX, SY>(f, lt, blk, bact, any, x0, y0);
+        filter_sb_packed<SX, SY>(f, lt, blk, bact, any, x0, y0);
 }
 
 // Kernel-level cases (cdef::find_direction + cdef::filter_block on one

@@ -49,7 +49,7 @@ struct CdefFrame {
 
 // One plane of one image: superres.rs:163-216. Workgroups cover
 // rows x segments of kSrSeg output samples.
-constexpr int kSrSeg = 256;
+constexpr int kSrSeg = 1024;
 
 struct SrJob {
     const uint16_t *in;

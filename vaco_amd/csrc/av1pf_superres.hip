@@ -44,42 +44,63 @@ __constant__ int16_t kUpscale[64][8] = {
     {0, 1, -2, 4, 127, -3, 1, 0},     {0, 0, -1, 2, 128, -1, 0, 0},
 };
 
-// One row segment of kSrSeg output samples (superres.rs:173-214).
-__global__ __launch_bounds__(kSrSeg) void k_superres(const SrJob *__restrict__ jobs, int n_jobs)
+constexpr int kSrThreads = 256;
+constexpr int kSrPer = kSrSeg / kSrThreads;  // outputs per thread
+
+// One row segment of kSrSeg output samples (superres.rs:173-214). The
+// source window the segment's taps read (edge-clamped to the padded width)
+// is staged in LDS once; it is exactly the union of the taps, because the
+// step is below one source sample, so the sample-range flag is raised
+// exactly when the reference would reject a tap.
+__global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict__ jobs, int n_jobs)
 {
+    __shared__ __align__(16) int16_t tab[64 * 8];
+    __shared__ uint16_t win[kSrSeg + 16];
     const SrJob &j = find_item(jobs, n_jobs, blockIdx.x);
     const uint32_t local = blockIdx.x - j.block_begin;
     const uint32_t ri = local / j.segs;
     const uint32_t seg = local - ri * j.segs;
     const uint32_t y = j.rows ? j.rows[ri] : ri;
-    const int x = (int)(seg * kSrSeg + threadIdx.x);
-    if (x >= (int)j.out_width)
-        return;
-    const int src_x = -(1 << 14) + j.start + x * j.step;
-    const int px = src_x >> 14;
-    const int phase = (src_x & 0x3fff) >> 8;
+    const int tid = threadIdx.x;
+    const int x0 = (int)seg * kSrSeg;
+    const int xn = min(x0 + kSrSeg, (int)j.out_width) - 1;
+    const int lo = ((-(1 << 14) + j.start + x0 * j.step) >> 14) - 3;
+    const int hi = ((-(1 << 14) + j.start + xn * j.step) >> 14) + 4;
     const uint16_t *row = j.in + (size_t)y * j.in_stride;
     const int last = (int)j.in_width - 1;
-    int sum = 0;
     bool bad = false;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int sx = min(max(px + k - 3, 0), last);
-        const int v = row[sx];
+    for (int i = tid; i <= hi - lo; i += kSrThreads) {
+        const int v = row[min(max(lo + i, 0), last)];
         bad |= v > j.max_sample;
-        sum += v * kUpscale[phase][k];
+        win[i] = (uint16_t)v;
     }
-    if (bad)
+    for (int i = tid; i < 64 * 8; i += kSrThreads)
+        tab[i] = kUpscale[i >> 3][i & 7];
+    if (__syncthreads_or(bad) && tid == 0)
         raise_status(j.status, kFlagSampleRange);
-    const int r = (sum + 64) >> 7;
-    j.out[(size_t)y * j.out_stride + x] = (uint16_t)min(max(r, 0), (int)j.max_sample);
+    uint16_t *out = j.out + (size_t)y * j.out_stride;
+#pragma unroll
+    for (int k = 0; k < kSrPer; k++) {
+        const int x = x0 + tid + k * kSrThreads;
+        if (x > xn)
+            break;
+        const int src_x = -(1 << 14) + j.start + x * j.step;
+        const int base = (src_x >> 14) - 3 - lo;
+        const int16_t *f = &tab[((src_x & 0x3fff) >> 8) * 8];
+        int sum = 0;
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+            sum += win[base + t] * f[t];
+        const int r = (sum + 64) >> 7;
+        out[x] = (uint16_t)min(max(r, 0), (int)j.max_sample);
+    }
 }
 
 hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s)
 {
     if (blocks == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_superres, dim3(blocks), dim3(kSrSeg), 0, s, jobs, n);
+    hipLaunchKernelGGL(k_superres, dim3(blocks), dim3(kSrThreads), 0, s, jobs, n);
     return hipGetLastError();
 }
 
