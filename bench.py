@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--frames", type=int, default=4, help="distinct seeded frames cycled through")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--mode", choices=["full", "batched", "copy"], default="full",
+                    help="full: the JSON line; batched: only the mixed batch (for profiling); "
+                         "copy: 256 MiB device copies (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: every box has one MI355X and the "
@@ -99,6 +102,23 @@ def main():
         sys.exit(2)
 
     from vaco_amd import av1pf, hip, workloads
+
+    if args.mode == "copy":
+        # Known-size traffic for scaling FETCH_SIZE / WRITE_SIZE: 16 copies of
+        # 256 MiB through the runtime's copy kernel.
+        src = hip.DeviceArray((128 << 20,), np.uint16)
+        dst = hip.DeviceArray((128 << 20,), np.uint16)
+        for _ in range(16):
+            hip.check(hip.rt().hipMemcpy(hip.C.c_void_p(dst.ptr), hip.C.c_void_p(src.ptr), src.nbytes, hip.D2D),
+                      "hipMemcpy D2D")
+        hip.synchronize()
+        print(json.dumps({"mode": "copy", "bytes_per_copy": src.nbytes, "copies": 16}), flush=True)
+        return
+    if args.mode == "batched":
+        print(json.dumps({"mode": "batched",
+                          **measure_mixed_batch(av1pf, hip, workloads, reps=max(3, args.steps // 20))}),
+              flush=True)
+        return
 
     cfg = workloads.cfg1()
     frames = [av1pf.generate(cfg, k) for k in range(args.frames)]
