@@ -84,6 +84,106 @@ def measure_mixed_batch(av1pf, hip, workloads, reps: int, n: int = 64):
     }
 
 
+def measure_served(av1pf, hip, workloads, seconds: float, fps: float = 3000.0, streams: int = 32,
+                   distinct: int = 2, ring: int = 8, budget: int = 32 << 30, window_us: int = 2000):
+    """The serving loop under 32 streams of Poisson arrivals (aggregate `fps`).
+
+    Each stream has a fixed resolution (720p/1080p/2160p) and depth (8/10);
+    its frames cycle over `distinct` seeded frames whose planes are resident
+    on the device, and its outputs over a ring of `ring` device buffers.
+    Latency is per frame, from submit to the server seeing its batch done.
+    1% of requests (every 100th) write to a buffer of their own, compared
+    afterwards with the plain per-frame path's output of the same frame.
+    """
+    import numpy as np
+
+    cfgs = workloads.serve_streams(streams)
+    frames = [[av1pf.generate(c, k) for k in range(distinct)] for c in cfgs]
+    dev_in = [[[hip.DeviceArray.from_host(p) for p in f.planes] for f in fs] for fs in frames]
+    outs = [[[hip.DeviceArray(g) for g in fs[0].output_geometry()] for _ in range(ring)] for fs in frames]
+    ptrs = lambda arrs: [t.ptr for t in arrs]
+
+    rng = np.random.default_rng(workloads.BASE_SEED + 4)
+    n = int(fps * seconds)
+    arrivals = np.cumsum(rng.exponential(1.0 / fps, size=n))
+    which = rng.integers(0, streams, size=n)
+    checked = {i: [hip.DeviceArray(g) for g in frames[which[i]][0].output_geometry()] for i in range(0, n, 100)}
+
+    ctx = av1pf.Context(3840, 2176)
+    # per-frame reference outputs of every distinct frame
+    ref = {}
+    for s in range(streams):
+        for k in range(distinct):
+            o = [hip.DeviceArray(g) for g in frames[s][k].output_geometry()]
+            ctx.frame_device(frames[s][k], ptrs(dev_in[s][k]), ptrs(o))
+            ref[(s, k)] = o
+    ctx.finish()
+    hip.synchronize()
+
+    srv = av1pf.Server(ctx, hbm_budget_bytes=budget, window_us=window_us, max_batch=64, max_inflight=2)
+    per_stream_seq = np.zeros(streams, np.int64)
+    ring_owner = [[-1] * ring for _ in range(streams)]
+    done_t = {}
+    latency, batch_sizes, lag = [], [], []
+    mpix = 0.0
+    frame_of = np.zeros(n, np.int64)
+
+    def drain_completions():
+        nonlocal mpix
+        for tag, st, bs, t_sub, t_launch, t_done in srv.poll():
+            if st != 0:
+                raise RuntimeError(f"served frame {tag} failed with status {st}")
+            done_t[tag] = t_done
+            latency.append((t_done - t_sub) / 1e6)
+            batch_sizes.append(bs)
+            mpix += frames[which[tag]][0].output_luma_pixels / 1e6
+
+    t0 = av1pf.now_ns()
+    for i in range(n):
+        s = int(which[i])
+        seq = int(per_stream_seq[s])
+        per_stream_seq[s] += 1
+        slot = seq % ring
+        prev = ring_owner[s][slot]
+        while prev >= 0 and prev not in done_t:  # ring buffer still in use
+            drain_completions()
+        ring_owner[s][slot] = i
+        due = t0 + int(arrivals[i] * 1e9)
+        while av1pf.now_ns() < due:
+            drain_completions()
+        lag.append((av1pf.now_ns() - due) / 1e6)
+        k = frame_of[i] = seq % distinct
+        out = checked[i] if i in checked else outs[s][slot]
+        srv.submit(frames[s][k], ptrs(dev_in[s][k]), ptrs(out), tag=i)
+    srv.drain()
+    drain_completions()
+    t1 = max(done_t.values())
+    srv.close()
+    wall = (t1 - t0) / 1e9
+
+    exact = True
+    for i, o in checked.items():
+        s, k = int(which[i]), int(frame_of[i])
+        exact &= all(np.array_equal(a.download(), b.download()) for a, b in zip(o, ref[(s, k)]))
+    ctx.close()
+    lat = np.array(latency)
+    return {
+        "workload": f"{streams} streams (720p/1080p/2160p x 8/10-bit 4:2:0, fixed per stream), Poisson "
+                    f"arrivals at {fps:.0f} frames/s aggregate for {seconds:.1f} s; {budget >> 30} GiB HBM "
+                    f"budget, {window_us / 1000:.1f} ms batching window, <= 2 batches in flight",
+        "frames": len(lat),
+        "served_mpix_s": round(mpix / wall, 2),
+        "offered_mpix_s": round(sum(frames[w][0].output_luma_pixels for w in which) / 1e6 / arrivals[-1], 2),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "max_ms": round(float(lat.max()), 3),
+        "mean_batch": round(float(np.mean(batch_sizes)), 2),
+        "submit_lag_p99_ms": round(float(np.percentile(lag, 99)), 3),
+        "spot_checks": len(checked),
+        "bit_exact_spot_checks": bool(exact),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,7 +192,8 @@ def main():
     ap.add_argument("--frames", type=int, default=4, help="distinct seeded frames cycled through")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--mode", choices=["full", "batched", "copy"], default="full",
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--mode", choices=["full", "batched", "served", "copy"], default="full",
                     help="full: the JSON line; batched: only the mixed batch (for profiling); "
                          "copy: 256 MiB device copies (HBM counter calibration)")
     args = ap.parse_args()
@@ -113,6 +214,10 @@ def main():
                       "hipMemcpy D2D")
         hip.synchronize()
         print(json.dumps({"mode": "copy", "bytes_per_copy": src.nbytes, "copies": 16}), flush=True)
+        return
+    if args.mode == "served":
+        print(json.dumps({"mode": "served", **measure_served(av1pf, hip, workloads, args.serve_seconds)}),
+              flush=True)
         return
     if args.mode == "batched":
         print(json.dumps({"mode": "batched",
@@ -163,6 +268,9 @@ def main():
     ctx.profile(False)
 
     batched = measure_mixed_batch(av1pf, hip, workloads, reps=max(3, args.steps // 20))
+    log("batched leg:", json.dumps(batched))
+    served = measure_served(av1pf, hip, workloads, args.serve_seconds)
+    log("served leg:", json.dumps(served))
 
     # Bit-exactness of every distinct frame against the oracle.
     from oracle import oracle as O
@@ -234,6 +342,7 @@ def main():
             "algorithmic_bytes_per_launch": kernel_bytes[dominant],
         },
         "batched": batched,
+        "served": served,
         "cpu_baseline": {
             "value": round(cpu_mpix_s, 4),
             "unit": "Mpix/s",

@@ -241,6 +241,57 @@ vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32_t n,
                                           const vaco_av1pf_picture *ins,
                                           vaco_av1pf_picture *outs, void *stream);
 
+/* ----------------------------------------------------------- serving --- */
+
+/* No reference peer: a request loop over many streams on one device. The
+ * server owns `ctx` while it exists (do not call other entries on it).
+ * Requests (device-resident planes) are admitted against an HBM budget,
+ * batched dynamically and run with vaco_av1pf_batch_device's kernels:
+ *  - a batch launches when fewer than max_inflight batches are on the GPU
+ *    and either the GPU is idle, max_batch requests wait, or the oldest
+ *    waiting request has waited window_us (continuous batching: requests
+ *    arriving while batches run join the next one);
+ *  - admission is FIFO; a request is admitted when the device bytes of the
+ *    frames in flight plus its own stay within hbm_budget_bytes.
+ * Results are identical to the same frame run alone. */
+typedef struct vaco_av1pf_server vaco_av1pf_server;
+
+typedef struct vaco_av1pf_server_config {
+    uint64_t hbm_budget_bytes;  /* inputs + outputs + intermediates in flight */
+    uint32_t window_us;         /* batching window while the GPU is busy */
+    uint32_t max_batch;         /* frames per launch, 1..256 */
+    uint32_t max_inflight;      /* batches on the GPU at once, 1..4 */
+} vaco_av1pf_server_config;
+
+typedef struct vaco_av1pf_completion {
+    uint64_t tag;
+    int32_t status;       /* vaco_av1pf_status of this frame */
+    uint32_t batch_size;  /* frames in the launch that ran it */
+    uint64_t t_submit_ns; /* vaco_av1pf_now_ns() clock */
+    uint64_t t_launch_ns;
+    uint64_t t_done_ns;   /* when the server saw the batch complete */
+} vaco_av1pf_completion;
+
+vaco_av1pf_status vaco_av1pf_server_create(vaco_av1pf_ctx *ctx,
+                                           const vaco_av1pf_server_config *cfg,
+                                           vaco_av1pf_server **out);
+/* Queue one frame (device planes; `desc`'s arrays and the planes must stay
+ * valid until its completion is polled). Thread-safe. */
+vaco_av1pf_status vaco_av1pf_server_submit(vaco_av1pf_server *srv,
+                                           const vaco_av1pf_frame_desc *desc,
+                                           const vaco_av1pf_picture *in,
+                                           vaco_av1pf_picture *out, uint64_t tag);
+/* Copy up to `max` completions out; returns how many. Non-blocking. */
+uint32_t vaco_av1pf_server_poll(vaco_av1pf_server *srv, vaco_av1pf_completion *out,
+                                uint32_t max);
+/* Device bytes one frame holds while in flight (what admission counts). */
+uint64_t vaco_av1pf_server_frame_bytes(const vaco_av1pf_frame_desc *desc);
+/* Block until every submitted request has completed. */
+vaco_av1pf_status vaco_av1pf_server_drain(vaco_av1pf_server *srv);
+void vaco_av1pf_server_destroy(vaco_av1pf_server *srv);
+/* Monotonic clock of the completion timestamps, in nanoseconds. */
+uint64_t vaco_av1pf_now_ns(void);
+
 /* Wait for the context's enqueued work and report deferred errors. */
 vaco_av1pf_status vaco_av1pf_finish(vaco_av1pf_ctx *ctx);
 

@@ -44,6 +44,24 @@ def test_struct_sizes_match_the_header_layout():
     assert C.sizeof(abi.Plane) == 24
 
 
+def test_server_config_and_frame_bytes_host_checks():
+    L = av1pf.lib()
+    assert C.sizeof(abi.ServerConfig) == 24 and C.sizeof(abi.Completion) == 40
+    h = C.c_void_p()
+    bad = [abi.ServerConfig(1 << 30, 2000, 0, 2), abi.ServerConfig(1 << 30, 2000, 257, 2),
+           abi.ServerConfig(1 << 30, 2000, 64, 0), abi.ServerConfig(0, 2000, 64, 2)]
+    for cfg in bad:  # rejected before any device work
+        assert L.vaco_av1pf_server_create(C.c_void_p(1), C.byref(cfg), C.byref(h)) == abi.INVALID_ARGUMENT
+    good = abi.ServerConfig(1 << 30, 2000, 64, 2)
+    assert L.vaco_av1pf_server_create(None, C.byref(good), C.byref(h)) == abi.INVALID_ARGUMENT
+    # admission bytes: input + output planes plus intermediates, at least the planes themselves
+    f = av1pf.generate(workloads.cfg1(), 0)
+    planes = sum(p.size for p in f.planes) + sum(h * w for h, w in f.output_geometry())
+    assert 2 * planes <= av1pf.frame_bytes(f) <= 2 * 4 * planes
+    g = av1pf.generate(workloads.cfg0(), 0)
+    assert av1pf.frame_bytes(g) < av1pf.frame_bytes(f)
+
+
 def test_adjust_strength_closed_form_buckets():
     # AV1 7.15.1: priStr * (4 + min(FloorLog2(var >> 6), 12)) + 8 >> 4, 0 when var == 0.
     cases = {(15, 0): 0, (15, 1): (15 * 4 + 8) >> 4, (15, 127): (15 * 4 + 8) >> 4, (15, 128): (15 * 5 + 8) >> 4,

@@ -125,6 +125,26 @@ class FrameDesc(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_uint64),
+        ("window_us", C.c_uint32),
+        ("max_batch", C.c_uint32),
+        ("max_inflight", C.c_uint32),
+    ]
+
+
+class Completion(C.Structure):
+    _fields_ = [
+        ("tag", C.c_uint64),
+        ("status", C.c_int32),
+        ("batch_size", C.c_uint32),
+        ("t_submit_ns", C.c_uint64),
+        ("t_launch_ns", C.c_uint64),
+        ("t_done_ns", C.c_uint64),
+    ]
+
+
 def u16_ptr(a: np.ndarray):
     assert a.dtype == np.uint16 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(C.POINTER(C.c_uint16))

@@ -113,7 +113,19 @@ def lib():
                                               C.c_void_p]
         L.vaco_av1pf_profile.argtypes = [ctxp, C.c_int]
         L.vaco_av1pf_profile_read.argtypes = [ctxp, P(C.c_double), P(C.c_uint64)]
-        for name in ("vaco_av1pf_batch_device", "vaco_av1pf_profile", "vaco_av1pf_profile_read", "vaco_av1pf_create", "vaco_av1pf_frame", "vaco_av1pf_frame_device", "vaco_av1pf_finish",
+        L.vaco_av1pf_server_create.argtypes = [ctxp, P(abi.ServerConfig), P(C.c_void_p)]
+        L.vaco_av1pf_server_submit.argtypes = [C.c_void_p, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture),
+                                               C.c_uint64]
+        L.vaco_av1pf_server_poll.argtypes = [C.c_void_p, P(abi.Completion), C.c_uint32]
+        L.vaco_av1pf_server_poll.restype = C.c_uint32
+        L.vaco_av1pf_server_drain.argtypes = [C.c_void_p]
+        L.vaco_av1pf_server_destroy.argtypes = [C.c_void_p]
+        L.vaco_av1pf_server_destroy.restype = None
+        L.vaco_av1pf_now_ns.restype = C.c_uint64
+        L.vaco_av1pf_server_frame_bytes.argtypes = [P(abi.FrameDesc)]
+        L.vaco_av1pf_server_frame_bytes.restype = C.c_uint64
+        for name in ("vaco_av1pf_server_create", "vaco_av1pf_server_submit", "vaco_av1pf_server_drain",
+                     "vaco_av1pf_batch_device", "vaco_av1pf_profile", "vaco_av1pf_profile_read", "vaco_av1pf_create", "vaco_av1pf_frame", "vaco_av1pf_frame_device", "vaco_av1pf_finish",
                      "vaco_av1pf_cdef_blocks", "vaco_av1pf_upscale_plane", "vaco_av1pf_restore_plane",
                      "vaco_av1pf_frame_output_geometry", "vaco_av1pf_frame_input_geometry",
                      "vaco_av1pf_gen_get_layout", "vaco_av1pf_gen_frame"):
@@ -328,6 +340,89 @@ class Context:
                                                        C.byref(abi.plane_of(a)), C.byref(cfg), arr,
                                                        len(units), C.byref(po)))
         return out[: cfg.height, : cfg.width]
+
+
+# --------------------------------------------------------------- server --
+
+class Server:
+    """The serving loop (``vaco_av1pf_server``) over a Context it owns while open.
+
+    submit() queues one frame on device planes; poll() returns completions
+    as ``(tag, status, batch_size, t_submit_ns, t_launch_ns, t_done_ns)``.
+    """
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int = 8 << 30, window_us: int = 2000,
+                 max_batch: int = 64, max_inflight: int = 2):
+        self._lib = lib()
+        self.ctx = ctx
+        cfg = abi.ServerConfig(hbm_budget_bytes, window_us, max_batch, max_inflight)
+        h = C.c_void_p()
+        st = self._lib.vaco_av1pf_server_create(ctx.handle, C.byref(cfg), C.byref(h))
+        if st != abi.OK:
+            raise Av1Error(st, "vaco_av1pf_server_create failed")
+        self.handle = h
+        self._keep = {}
+        self._buf = (abi.Completion * 256)()
+
+    def submit_call(self, frame: Frame, in_ptrs, out_ptrs, tag: int):
+        """A zero-argument callable submitting `frame` on these device planes."""
+        geo_out = frame.output_geometry()
+        pin, pout = abi.Picture(), abi.Picture()
+        for i in range(frame.nplanes):
+            h, w = frame.planes[i].shape
+            pin.plane[i] = abi.device_plane(in_ptrs[i], w, h)
+            pout.plane[i] = abi.device_plane(out_ptrs[i], geo_out[i][1], geo_out[i][0])
+        fn, h = self._lib.vaco_av1pf_server_submit, self.handle
+        d, pi, po, t = C.byref(frame.desc), C.byref(pin), C.byref(pout), C.c_uint64(tag)
+
+        def call():
+            rc = fn(h, d, pi, po, t)
+            if rc:
+                raise Av1Error(rc, "vaco_av1pf_server_submit failed")
+
+        call.keep = (frame, pin, pout)
+        return call
+
+    def submit(self, frame: Frame, in_ptrs, out_ptrs, tag: int):
+        c = self.submit_call(frame, in_ptrs, out_ptrs, tag)
+        self._keep[tag] = c
+        c()
+
+    def poll(self):
+        out = []
+        while True:
+            n = self._lib.vaco_av1pf_server_poll(self.handle, self._buf, 256)
+            for i in range(n):
+                c = self._buf[i]
+                self._keep.pop(c.tag, None)
+                out.append((c.tag, c.status, c.batch_size, c.t_submit_ns, c.t_launch_ns, c.t_done_ns))
+            if n < 256:
+                return out
+
+    def drain(self):
+        st = self._lib.vaco_av1pf_server_drain(self.handle)
+        if st != abi.OK:
+            raise Av1Error(st, "vaco_av1pf_server_drain failed")
+
+    def close(self):
+        if self.handle:
+            self._lib.vaco_av1pf_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def frame_bytes(frame: Frame) -> int:
+    """Device bytes `frame` holds in flight, as the server's admission counts them."""
+    return lib().vaco_av1pf_server_frame_bytes(C.byref(frame.desc))
+
+
+def now_ns() -> int:
+    return lib().vaco_av1pf_now_ns()
 
 
 # --------------------------------------------------------- host helpers --

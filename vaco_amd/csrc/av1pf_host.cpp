@@ -24,6 +24,7 @@
 #include "../../include/vaco_av1pf.h"
 #include "av1pf_device.h"
 #include "av1pf_host.h"
+#include "av1pf_internal.h"
 
 namespace av1pf {
 hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, hipStream_t s);
@@ -1049,10 +1050,60 @@ static size_t arena_need(const vaco_av1pf_frame_desc &d, const FrameGeo &g)
     return n;
 }
 
+size_t av1pf_frame_device_bytes(const vaco_av1pf_frame_desc &d)
+{
+    FrameGeo g;
+    frame_geometry(d, &g);
+    size_t n = scratch_need(g);
+    for (int p = 0; p < g.nplanes; p++)
+        n += (size_t)g.in_w[p] * g.in_h[p] + (size_t)g.out_w[p] * g.out_h[p];
+    return n * sizeof(uint16_t);
+}
+
+int av1pf_ctx_device(const vaco_av1pf_ctx *ctx) { return ctx->device; }
+
+// Settle one frame's sample-range flag: replay the reference's checks over a
+// host copy of its device input. Returns the reference's outcome.
+vaco_av1pf_status av1pf_replay_device_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_desc &d,
+                                            const vaco_av1pf_picture &in, const char **msg)
+{
+    FrameGeo g;
+    frame_geometry(d, &g);
+    std::vector<uint16_t> host[3];
+    vaco_av1pf_picture hpic = in;
+    for (int p = 0; p < g.nplanes; p++) {
+        host[p].resize((size_t)g.in_w[p] * g.in_h[p]);
+        HIPCHECK(hipMemcpy2D(host[p].data(), g.in_w[p] * 2, in.plane[p].data, in.plane[p].stride * 2,
+                             g.in_w[p] * 2, g.in_h[p], hipMemcpyDeviceToHost));
+        hpic.plane[p].data = host[p].data();
+        hpic.plane[p].stride = g.in_w[p];
+    }
+    HostPlane hin[3];
+    host_planes_of(&hpic, g.nplanes, hin);
+    *msg = "";
+    return replay_checks(d, g, hin, nullptr, msg);
+}
+
 extern "C" vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32_t n,
                                                      const vaco_av1pf_frame_desc *descs,
                                                      const vaco_av1pf_picture *ins,
                                                      vaco_av1pf_picture *outs, void *stream)
+{
+    if (!ctx)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    return av1pf_enqueue_batch(ctx, n, descs, ins, outs,
+                               stream ? (hipStream_t)stream : ctx->stream, nullptr, nullptr);
+}
+
+// The batched call. With `statuses` set, the caller takes over the frames'
+// status checks: *statuses receives the slot's per-frame status words
+// (valid until the slot is reused, kSlots batched calls later) and
+// *done the event that completes the batch; otherwise the checks are
+// deferred to vaco_av1pf_finish().
+vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
+                                      const vaco_av1pf_frame_desc *descs,
+                                      const vaco_av1pf_picture *ins, vaco_av1pf_picture *outs,
+                                      hipStream_t s, const uint32_t **statuses, hipEvent_t *done)
 {
     if (!ctx || (n && (!descs || !ins || !outs)))
         return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "null argument");
@@ -1062,7 +1113,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32
     if (n == 0)
         return VACO_AV1PF_OK;
     HIPCHECK(hipSetDevice(ctx->device));
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    if (!s)
+        s = ctx->stream;
     std::vector<FrameGeo> geo(n);
     size_t ws_need = 0, arena = 1 << 16;
     for (uint32_t i = 0; i < n; i++) {
@@ -1129,6 +1181,12 @@ extern "C" vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32
         return st;
     if ((st = close_slot(ctx, a, s)) != VACO_AV1PF_OK)
         return st;
+    if (statuses) {
+        *statuses = a->status;
+        if (done)
+            *done = a->done;
+        return VACO_AV1PF_OK;
+    }
     a->descs.assign(descs, descs + n);
     a->ins.assign(ins, ins + n);
     return VACO_AV1PF_OK;

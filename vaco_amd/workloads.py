@@ -9,6 +9,9 @@ Each preset is a :class:`GenConfig` for the integer generator in
   at random angles, 128x128 restoration units (plain per-frame GPU case)
 * ``cfg2``: 10-bit 4:2:0 3840x2160 frames, N(0, 24), superres with the
   denominator drawn from 9..16 per frame, 64x64 units as cfg0
+* ``mixed_batch``: 64 frames of mixed geometry, depth and superres
+* ``serve_streams``: 32 streams of fixed resolution and depth for the
+  serving loop
 """
 from __future__ import annotations
 
@@ -68,4 +71,28 @@ def mixed_batch(n=64, seed=BASE_SEED + 3):
         c = make(w, h, bd, sigma=6 if bd == 8 else 24, edges=0, unit=64, superres_denom=denom,
                  seed=seed + 1000 * i)
         out.append((c, i))
+    return out
+
+
+SERVE_SHAPES = [(1280, 720), (1920, 1080), (3840, 2160)]
+
+
+def serve_streams(n=32, seed=BASE_SEED + 4):
+    """Stream configurations of the serving workload: each stream has a fixed
+    resolution drawn from 720p / 1080p / 2160p and a fixed bit depth from
+    {8, 10}; content and filter parameters as cfg1 (8-bit: hard edges,
+    128x128 units) and cfg2 (10-bit: N(0, 24), superres with the
+    denominator drawn per frame, 64x64 units). Returns
+    [GenConfig] (frame k of stream s is generate(cfg[s], k))."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n):
+        w, h = SERVE_SHAPES[int(rng.integers(0, len(SERVE_SHAPES)))]
+        if rng.random() < 0.5:
+            c = make(w, h, 8, sigma=6, edges=8, unit=128, seed=seed + 1000 * s)
+        else:
+            c = make(w, h, 10, sigma=24, edges=0, unit=64, superres_denom=0, seed=seed + 1000 * s)
+        out.append(c)
     return out
