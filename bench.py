@@ -13,6 +13,17 @@ CPU oracle (bit-exact), and the oracle itself - the reference-faithful
 single-threaded C restatement - is timed on a bounded sample of the same
 frames for the CPU baseline.
 
+Beside the headline figure the same line carries:
+
+* ``kernels`` / ``roofline``: per-kernel launch time (HIP events on the
+  library's stream) and achieved algorithmic GB/s against 8 TB/s;
+* ``uhd_10bit``: 16 frames of 3840x2160 10-bit with superres, per frame and
+  batched, with the per-kernel table;
+* ``batched``: 64 mixed frames, one batched call against 64 per-frame calls;
+* ``served``: the serving loop under 32 Poisson streams at 3,000 frames/s,
+  p50/p99 latency;
+* ``cpu_baseline``: the oracle on one core.
+
 Prints one JSON line (rank 0).
 """
 from __future__ import annotations
@@ -184,6 +195,140 @@ def measure_served(av1pf, hip, workloads, seconds: float, fps: float = 3000.0, s
     }
 
 
+def kernel_table(prof, steps):
+    """Per-kernel launch time and achieved algorithmic GB/s from profile_read()."""
+    out = {}
+    for k, (ms, n, b) in prof.items():
+        if n == 0:
+            continue
+        s = ms / 1e3 / n
+        out[k] = {
+            "launches": n,
+            "ms_per_step": round(ms / steps, 5),
+            "us_per_launch": round(s * 1e6, 3),
+            "bytes_per_launch": b // n,
+            "achieved_gbs": round(b / n / s / 1e9, 2),
+        }
+    return out
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE / WRITE_SIZE, scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_hbm_traffic_1080p_per_frame.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic-json:"):
+                return json.loads(line.split(":", 1)[1]).get(kernel)
+    except OSError:
+        pass
+    return None
+
+
+def measure_per_frame(av1pf, hip, cfg, n_frames, steps, warmup):
+    """The plain per-frame path: `steps` calls of vaco_av1pf_frame_device, one
+    seeded frame each (cycling over `n_frames`), inputs and outputs resident."""
+    frames = [av1pf.generate(cfg, k) for k in range(n_frames)]
+    geo_out = frames[0].output_geometry()
+    ctx = av1pf.Context(int(cfg.width), (int(cfg.height) + 63) // 64 * 64)
+    dev_in = [[hip.DeviceArray.from_host(p) for p in f.planes] for f in frames]
+    dev_out = [[hip.DeviceArray(g) for g in geo_out] for _ in frames]
+    hip.synchronize()
+    calls = [ctx.device_call(f, [t.ptr for t in dev_in[k]], [t.ptr for t in dev_out[k]])
+             for k, f in enumerate(frames)]
+    for i in range(warmup):
+        calls[i % len(calls)]()
+    ctx.finish()
+    hip.synchronize()
+
+    # Timed region: K steps, device synchronised on both sides.
+    t0 = time.perf_counter()
+    for i in range(steps):
+        calls[i % len(calls)]()
+    t_enq = time.perf_counter()
+    ctx.finish()
+    hip.synchronize()
+    t1 = time.perf_counter()
+
+    # Kernel durations from HIP events on the library's stream (a separate
+    # pass over the same steps, so event records do not perturb the timing).
+    ctx.profile(True)
+    for i in range(steps):
+        calls[i % len(calls)]()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    outputs = [[t.download() for t in ts] for ts in dev_out]
+    ctx.close()
+    mpix = frames[0].output_luma_pixels / 1e6
+    return {
+        "mpix_s": round(mpix * steps / (t1 - t0), 2),
+        "ms_per_step": round((t1 - t0) * 1e3 / steps, 4),
+        "host_enqueue_ms_per_step": round((t_enq - t0) * 1e3 / steps, 4),
+        "kernels": kernel_table(prof, steps),
+        "_frames": frames,
+        "_outputs": outputs,
+    }
+
+
+def measure_4k10(av1pf, hip, workloads, reps: int, n: int = 16):
+    """16 seeded 3840x2160 10-bit 4:2:0 frames with superres (denominator
+    9..16) then LR: per-frame calls and one batched call, per-kernel
+    roofline. Frame 0 is checked against the oracle, the batch against the
+    per-frame outputs."""
+    cfg = workloads.cfg2()
+    frames = [av1pf.generate(cfg, k) for k in range(n)]
+    ctx = av1pf.Context(3840, 2176)
+    dev_in = [[hip.DeviceArray.from_host(p) for p in f.planes] for f in frames]
+    out_pf = [[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames]
+    out_b = [[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames]
+    ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
+    calls = [ctx.device_call(f, ptr(dev_in)[k], ptr(out_pf)[k]) for k, f in enumerate(frames)]
+    batch = ctx.batch_call(frames, ptr(dev_in), ptr(out_b))
+    for c in calls:
+        c()
+    batch()
+    ctx.finish()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for c in calls:
+            c()
+    ctx.finish()
+    hip.synchronize()
+    t_pf = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch()
+    ctx.finish()
+    hip.synchronize()
+    t_b = (time.perf_counter() - t0) / reps
+    ctx.profile(True)
+    for c in calls:
+        c()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    same = all(np.array_equal(a.download(), b.download()) for fa, fb in zip(out_pf, out_b) for a, b in zip(fa, fb))
+    from oracle import oracle as O
+
+    want = O.frame(frames[0].desc, frames[0].planes)
+    exact0 = all(np.array_equal(t.download(), w) for t, w in zip(out_pf[0], want))
+    ctx.close()
+    mpix = sum(f.output_luma_pixels for f in frames) / 1e6
+    kern = kernel_table(prof, n)
+    total_bytes = sum(int(p.size) for f in frames for p in f.planes) * 2 + \
+        sum(h * w for f in frames for h, w in f.output_geometry()) * 2
+    return {
+        "workload": f"{n} frames 3840x2160 10-bit 4:2:0, superres denominator 9..16, CDEF + LR (64x64 units)",
+        "per_frame_mpix_s": round(mpix / t_pf, 2),
+        "per_frame_ms_per_frame": round(t_pf * 1e3 / n, 4),
+        "batched_mpix_s": round(mpix / t_b, 2),
+        "stage_hbm_frac_per_frame": round(total_bytes / t_pf / 1e9 / HBM_PEAK_GBS, 5),
+        "kernels_per_frame": kern,
+        "bit_exact_frame0_vs_oracle": bool(exact0),
+        "batched_equals_per_frame": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,8 +338,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
-    ap.add_argument("--mode", choices=["full", "batched", "served", "copy"], default="full",
-                    help="full: the JSON line; batched: only the mixed batch (for profiling); "
+    ap.add_argument("--mode", choices=["full", "perframe", "batched", "served", "copy"], default="full",
+                    help="full: the JSON line; perframe / batched / served: one leg only (for profiling); "
                          "copy: 256 MiB device copies (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -225,60 +370,27 @@ def main():
               flush=True)
         return
 
-    cfg = workloads.cfg1()
-    frames = [av1pf.generate(cfg, k) for k in range(args.frames)]
-    geo_out = frames[0].output_geometry()
-    ctx = av1pf.Context(1920, 1088)
-    dev_in, dev_out = [], []
-    for f in frames:
-        dev_in.append([hip.DeviceArray.from_host(p) for p in f.planes])
-        dev_out.append([hip.DeviceArray(g) for g in geo_out])
-    in_ptrs = [[t.ptr for t in ts] for ts in dev_in]
-    out_ptrs = [[t.ptr for t in ts] for ts in dev_out]
-    hip.synchronize()
-
-    calls = [ctx.device_call(f, in_ptrs[k], out_ptrs[k]) for k, f in enumerate(frames)]
-
-    def step(i):
-        calls[i % len(calls)]()
-
-    for i in range(args.warmup):
-        step(i)
-    ctx.finish()
-    hip.synchronize()
-
-    # Timed region: K steps, device synchronised on both sides.
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(i)
-    t_enq = time.perf_counter()
-    ctx.finish()
-    hip.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    mpix = frames[0].output_luma_pixels / 1e6
-    value = mpix * args.steps / (t1 - t0)
-
-    # Kernel durations from HIP events on the library's stream (separate
-    # pass over the same steps so event records do not perturb the timing).
-    ctx.profile(True)
-    for i in range(args.steps):
-        step(i)
-    prof = ctx.profile_read()
-    ctx.profile(False)
-
+    per_frame = measure_per_frame(av1pf, hip, workloads.cfg1(), args.frames, args.steps, args.warmup)
+    if args.mode == "perframe":
+        print(json.dumps({"mode": "perframe", **{k: v for k, v in per_frame.items() if k[0] != "_"}}),
+              flush=True)
+        return
+    log("per-frame leg:", json.dumps({k: v for k, v in per_frame.items() if k[0] != "_"}))
+    uhd = measure_4k10(av1pf, hip, workloads, reps=3)
+    log("4K 10-bit leg:", json.dumps(uhd))
     batched = measure_mixed_batch(av1pf, hip, workloads, reps=max(3, args.steps // 20))
     log("batched leg:", json.dumps(batched))
     served = measure_served(av1pf, hip, workloads, args.serve_seconds)
     log("served leg:", json.dumps(served))
 
-    # Bit-exactness of every distinct frame against the oracle.
+    # Bit-exactness of every distinct frame against the oracle, and the
+    # oracle's own single-threaded rate on the same frames.
     from oracle import oracle as O
 
+    frames, outs = per_frame["_frames"], per_frame["_outputs"]
     exact = True
     cpu_times = []
-    for k, f in enumerate(frames):
-        got = [t.download() for t in dev_out[k]]
+    for f, got in zip(frames, outs):
         t = time.perf_counter()
         want = O.frame(f.desc, f.planes)
         cpu_times.append(time.perf_counter() - t)
@@ -292,31 +404,21 @@ def main():
             O.frame(f.desc, f.planes)
             cpu_times.append(time.perf_counter() - t)
             k += 1
+    mpix = frames[0].output_luma_pixels / 1e6
     cpu_mpix_s = mpix * len(cpu_times) / sum(cpu_times)
 
-    # Algorithmic bytes (input read once + output written once, u16 samples).
-    in_samples = sum(int(p.size) for p in frames[0].planes)
-    out_samples = sum(h * w for h, w in geo_out)
-    vis_in = 1920 * 1080 + 2 * 960 * 540
-    kernel_bytes = {
-        "cdef": 2 * (in_samples + in_samples),           # padded input -> CDEF image
-        "restoration": 2 * (vis_in + out_samples),       # CDEF image -> output
-        "superres": 0,
-        "fused": 2 * (vis_in + out_samples),
-    }
-    dominant = max((k for k in prof if prof[k][1] > 0), key=lambda k: prof[k][0])
-    total_ms, launches = prof[dominant]
-    avg_s = total_ms / 1e3 / launches
-    achieved = kernel_bytes[dominant] / avg_s / 1e9
-
+    kern = per_frame["kernels"]
+    dominant = max(kern, key=lambda k: kern[k]["ms_per_step"])
+    d = kern[dominant]
+    traffic = read_traffic(dominant)
     rec = {
         "metric": METRIC,
-        "value": round(value, 2),
+        "value": per_frame["mpix_s"],
         "unit": "Mpix/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": per_frame["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -329,18 +431,21 @@ def main():
             "frames_cycled": len(frames),
             "parallelism": "single GPU",
         },
-        "host_enqueue_ms_per_step": round((t_enq - t0) * 1e3 / args.steps, 4),
-        "kernels_ms_per_launch": {k: round(v[0] / v[1], 5) for k, v in prof.items() if v[1]},
+        "host_enqueue_ms_per_step": per_frame["host_enqueue_ms_per_step"],
+        "kernels": kern,
         "roofline": {
             "bound": "hbm",
             "kernel": dominant,
-            "achieved": round(achieved, 2),
+            "achieved": d["achieved_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
-            "algorithmic_bytes_per_launch": kernel_bytes[dominant],
+            "frac": round(d["achieved_gbs"] / HBM_PEAK_GBS, 5),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": d["bytes_per_launch"],
+            "avg_launch_us": d["us_per_launch"],
+            "note": "integer stage, VALU-bound: see DESIGN.md section 7",
         },
+        "uhd_10bit": uhd,
         "batched": batched,
         "served": served,
         "cpu_baseline": {
@@ -353,7 +458,6 @@ def main():
         },
     }
     print(json.dumps(rec), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

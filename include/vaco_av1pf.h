@@ -333,19 +333,26 @@ enum {
     VACO_AV1PF_KERNEL_CDEF = 0,
     VACO_AV1PF_KERNEL_SUPERRES = 1,
     VACO_AV1PF_KERNEL_RESTORATION = 2,
-    VACO_AV1PF_KERNEL_FUSED = 3,
+    VACO_AV1PF_KERNEL_UPLOAD = 3,
     VACO_AV1PF_KERNEL_COUNT = 4
 };
 
 /* No reference counterpart (measurement hook). While enabled, every kernel
  * launch of the frame entries is bracketed by HIP events on the stream it
  * runs on; vaco_av1pf_profile_read() synchronises the context and returns,
- * per kernel id, the summed launch durations in milliseconds and the launch
- * counts since the last read. Enabling allocates the event pool. */
+ * per kernel id, the summed launch durations in milliseconds, the launch
+ * counts and the algorithmic bytes of those launches since the last read.
+ * Algorithmic bytes count every sample a kernel must read once and every
+ * sample it writes once, 2 bytes each (CDEF: its planes in and out;
+ * superres: visible source rows in, output rows out; restoration: its
+ * tiles in and out; upload: the staged side information, read from host
+ * memory and written to the device). Any argument may be NULL. Enabling
+ * allocates the event pool. */
 vaco_av1pf_status vaco_av1pf_profile(vaco_av1pf_ctx *ctx, int enable);
 vaco_av1pf_status vaco_av1pf_profile_read(vaco_av1pf_ctx *ctx,
                                           double ms[VACO_AV1PF_KERNEL_COUNT],
-                                          uint64_t launches[VACO_AV1PF_KERNEL_COUNT]);
+                                          uint64_t launches[VACO_AV1PF_KERNEL_COUNT],
+                                          uint64_t bytes[VACO_AV1PF_KERNEL_COUNT]);
 
 /* ------------------------------------------------ host-only helpers --- */
 

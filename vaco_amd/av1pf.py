@@ -112,7 +112,7 @@ def lib():
         L.vaco_av1pf_batch_device.argtypes = [ctxp, C.c_uint32, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture),
                                               C.c_void_p]
         L.vaco_av1pf_profile.argtypes = [ctxp, C.c_int]
-        L.vaco_av1pf_profile_read.argtypes = [ctxp, P(C.c_double), P(C.c_uint64)]
+        L.vaco_av1pf_profile_read.argtypes = [ctxp, P(C.c_double), P(C.c_uint64), P(C.c_uint64)]
         L.vaco_av1pf_server_create.argtypes = [ctxp, P(abi.ServerConfig), P(C.c_void_p)]
         L.vaco_av1pf_server_submit.argtypes = [C.c_void_p, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture),
                                                C.c_uint64]
@@ -292,17 +292,19 @@ class Context:
 
     # ---- kernel timing
 
-    KERNELS = ("cdef", "superres", "restoration", "fused")
+    KERNELS = ("cdef", "superres", "restoration", "upload")
 
     def profile(self, enable: bool = True):
         self._check(self._lib.vaco_av1pf_profile(self.handle, 1 if enable else 0))
 
     def profile_read(self):
-        """{kernel: (total_ms, launches)} since the last read (synchronises)."""
-        ms = (C.c_double * 4)()
-        n = (C.c_uint64 * 4)()
-        self._check(self._lib.vaco_av1pf_profile_read(self.handle, ms, n))
-        return {k: (ms[i], n[i]) for i, k in enumerate(self.KERNELS)}
+        """{kernel: (total_ms, launches, algorithmic_bytes)} since the last read (synchronises)."""
+        k = len(self.KERNELS)
+        ms = (C.c_double * k)()
+        n = (C.c_uint64 * k)()
+        b = (C.c_uint64 * k)()
+        self._check(self._lib.vaco_av1pf_profile_read(self.handle, ms, n, b))
+        return {name: (ms[i], n[i], b[i]) for i, name in enumerate(self.KERNELS)}
 
     # ---- kernel-level entries
 

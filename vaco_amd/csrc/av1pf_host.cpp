@@ -29,8 +29,9 @@
 namespace av1pf {
 hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, hipStream_t s);
 hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s);
-hipError_t launch_lr(const LrPlane *planes, const uint32_t *const lists[3], const uint32_t counts[3],
+hipError_t launch_lr(const LrPlane *planes, const uint32_t *list, const uint32_t counts[3],
                      hipStream_t s);
+hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_cdef_cases(const uint16_t *inputs, const void *params, uint8_t *dirs,
                              uint32_t *vars, uint16_t *outs, uint32_t n, hipStream_t s);
 }  // namespace av1pf
@@ -49,8 +50,9 @@ constexpr uint32_t kMaxBatch = 256;  // frames per batched call
 // what the deferred error replay needs (the caller keeps desc's arrays and
 // the input planes valid until vaco_av1pf_finish()).
 struct Arena {
-    uint8_t *h = nullptr;  // pinned host
-    uint8_t *d = nullptr;  // device mirror
+    uint8_t *h = nullptr;   // pinned, coherent host memory
+    uint8_t *hd = nullptr;  // the same memory as the device addresses it
+    uint8_t *d = nullptr;   // device mirror
     size_t cap = 0, used = 0;
     hipEvent_t done = nullptr;  // the frames that used this slot have finished
     bool pending = false;
@@ -95,6 +97,7 @@ struct vaco_av1pf_ctx {
     struct ProfRec {
         int kernel;
         uint32_t a, b;
+        uint64_t bytes;
     };
     bool prof = false;
     std::vector<hipEvent_t> prof_pool;
@@ -102,11 +105,13 @@ struct vaco_av1pf_ctx {
     std::vector<ProfRec> prof_recs;
     double prof_ms[VACO_AV1PF_KERNEL_COUNT] = {};
     uint64_t prof_n[VACO_AV1PF_KERNEL_COUNT] = {};
+    uint64_t prof_bytes[VACO_AV1PF_KERNEL_COUNT] = {};
 };
 
 static hipError_t profile_collect(vaco_av1pf_ctx *ctx);
 template <class F>
-static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, hipStream_t s, F launch);
+static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, uint64_t bytes, hipStream_t s,
+                               F launch);
 
 static vaco_av1pf_status fail(vaco_av1pf_ctx *ctx, vaco_av1pf_status st, const char *msg)
 {
@@ -129,6 +134,26 @@ static vaco_av1pf_status hip_fail(vaco_av1pf_ctx *ctx, hipError_t e)
 
 static vaco_av1pf_status resolve_slot(vaco_av1pf_ctx *ctx, Arena &a);
 
+// (Re)allocate a slot's staging buffers at `bytes` (a multiple of 256). The
+// host side is coherent pinned memory that k_upload reads directly.
+static hipError_t alloc_slot(Arena &a, size_t bytes)
+{
+    if (a.h)
+        (void)hipHostFree(a.h);
+    if (a.d)
+        (void)hipFree(a.d);
+    a.h = a.hd = a.d = nullptr;
+    a.cap = 0;
+    hipError_t e = hipHostMalloc((void **)&a.h, bytes, hipHostMallocCoherent);
+    if (e == hipSuccess)
+        e = hipHostGetDevicePointer((void **)&a.hd, a.h, 0);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&a.d, bytes);
+    if (e == hipSuccess)
+        a.cap = bytes;
+    return e;
+}
+
 // Next staging slot. Waits for the frame that last used it and settles that
 // frame's deferred status check first.
 static vaco_av1pf_status take_slot(vaco_av1pf_ctx *ctx, Arena **out)
@@ -148,28 +173,40 @@ static vaco_av1pf_status take_slot(vaco_av1pf_ctx *ctx, Arena **out)
     return VACO_AV1PF_OK;
 }
 
-// Grow a slot's staging buffers (its previous frames have finished).
+// Grow a slot's staging buffers (its previous frames have finished). Every
+// other slot whose work has finished grows with it, so a workload settles
+// after one growth instead of one per slot.
 static vaco_av1pf_status grow_slot(vaco_av1pf_ctx *ctx, Arena *a, size_t bytes)
 {
     if (bytes <= a->cap)
         return VACO_AV1PF_OK;
-    bytes = std::max(bytes, a->cap * 2);
-    HIPCHECK(hipHostFree(a->h));
-    HIPCHECK(hipFree(a->d));
-    a->h = nullptr;
-    a->d = nullptr;
-    a->cap = 0;
-    HIPCHECK(hipHostMalloc((void **)&a->h, bytes, hipHostMallocDefault));
-    HIPCHECK(hipMalloc((void **)&a->d, bytes));
-    a->cap = bytes;
+    bytes = (std::max(bytes, a->cap * 2) + 255) & ~size_t(255);
+    HIPCHECK(alloc_slot(*a, bytes));
+    for (int i = 0; i < kSlots; i++) {
+        Arena &b = ctx->slot[i];
+        if (&b == a || b.cap >= bytes)
+            continue;
+        if (b.pending) {
+            const hipError_t q = hipEventQuery(b.done);
+            if (q == hipErrorNotReady)
+                continue;
+            HIPCHECK(q);
+        }
+        HIPCHECK(alloc_slot(b, bytes));
+    }
     return VACO_AV1PF_OK;
 }
+
+template <class F>
+static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, uint64_t bytes, hipStream_t s,
+                               F launch);
 
 static vaco_av1pf_status upload_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t s)
 {
     if (a->used == 0)
         return VACO_AV1PF_OK;
-    HIPCHECK(hipMemcpyAsync(a->d, a->h, a->used, hipMemcpyHostToDevice, s));
+    HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_UPLOAD, (uint64_t)a->used * 2, s,
+                          [&] { return launch_upload(a->d, a->hd, a->used, s); }));
     return VACO_AV1PF_OK;
 }
 
@@ -223,14 +260,11 @@ extern "C" vaco_av1pf_status vaco_av1pf_create(int device, uint32_t max_width, u
             ctx->img[i][p] = ctx->pool + ctx->plane_cap * (size_t)(i * 3 + p);
     const size_t blocks8 = (size_t)(ctx->max_w / 8 + 2) * (ctx->max_h / 8 + 2);
     const size_t units = (size_t)(ctx->max_w / 32 + 2) * (ctx->max_h / 32 + 2) * 3;
-    const size_t cap = (1u << 20) + blocks8 * 2 + units * sizeof(LrUnit) + ctx->max_h * 64;
+    const size_t cap =
+        ((1u << 20) + blocks8 * 2 + units * sizeof(LrUnit) + ctx->max_h * 64 + 255) & ~size_t(255);
     for (int i = 0; i < kSlots; i++) {
         Arena &a = ctx->slot[i];
-        a.cap = cap;
-        e = hipHostMalloc((void **)&a.h, cap, hipHostMallocDefault);
-        if (e != hipSuccess)
-            return bail(e);
-        e = hipMalloc((void **)&a.d, cap);
+        e = alloc_slot(a, cap);
         if (e != hipSuccess)
             return bail(e);
         e = hipEventCreateWithFlags(&a.done, hipEventDisableTiming);
@@ -503,6 +537,9 @@ struct Plan {
         uint32_t w, h;
     };
     std::vector<Copy> copies;  // frames in which no stage runs
+    // algorithmic bytes per kernel id (samples read once + written once, 2 B)
+    uint64_t bytes[VACO_AV1PF_KERNEL_COUNT] = {};
+    uint64_t cdef_bytes[4] = {};  // CDEF by chroma layout
 };
 
 #define TAKE(ptr, dptr, type, n)                                                    \
@@ -572,6 +609,8 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             f.uv_sec[i] = d.cdef.uv[i].secondary;
         }
         const int mode = cdef_mode(d);
+        for (int p = 0; p < np; p++)
+            plan.cdef_bytes[mode] += (uint64_t)g.in_w[p] * g.in_h[p] * 4;
         f.block_begin = plan.cdef_blocks[mode];
         plan.cdef_blocks[mode] += f.sb_cols * f.sb_rows;
         plan.cdef[mode].push_back(f);
@@ -633,6 +672,8 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
                 j.block_begin = plan.sr_blocks;
                 j.max_sample = (uint16_t)((1u << d.bit_depth) - 1);
                 plan.sr_blocks += j.n_rows * j.segs;
+                plan.bytes[VACO_AV1PF_KERNEL_SUPERRES] +=
+                    (uint64_t)j.n_rows * (g.sr_vis[p] + g.sr_out_w[p]) * 2;
                 plan.sr.push_back(j);
                 if (img == 0)
                     cur[p] = dst;
@@ -701,6 +742,8 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
                         kind = k == VACO_AV1PF_UNIT_WIENER ? 1 : k == VACO_AV1PF_UNIT_SELF_GUIDED ? 2 : 0;
                     }
                     plan.lr_tiles[kind].push_back(item << 22 | (ri * L.n_cols + ci));
+                    plan.bytes[VACO_AV1PF_KERNEL_RESTORATION] +=
+                        (uint64_t)(*rows)[ri].h * (*cols)[ci].w * 4;
                 }
             plan.lr.push_back(L);
         }
@@ -727,20 +770,22 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
         dj = d;
     }
     const LrPlane *dl = nullptr;
-    const uint32_t *dlist[3] = {};
+    const uint32_t *dlist = nullptr;
     uint32_t nlist[3] = {};
     if (!plan.lr.empty()) {
         TAKE(h, d, LrPlane, plan.lr.size());
         memcpy(h, plan.lr.data(), plan.lr.size() * sizeof(LrPlane));
         dl = d;
-        for (int k = 0; k < 3; k++) {
-            nlist[k] = (uint32_t)plan.lr_tiles[k].size();
-            if (!nlist[k])
-                continue;
-            TAKE(hl, dlk, uint32_t, nlist[k]);
+        size_t n = 0;
+        for (int k = 0; k < 3; k++)
+            n += nlist[k] = (uint32_t)plan.lr_tiles[k].size();
+        // self-guided tiles first, then Wiener, then copies (k_lr's order)
+        TAKE(hl, dlk, uint32_t, n);
+        for (int k = 2; k >= 0; k--) {
             memcpy(hl, plan.lr_tiles[k].data(), nlist[k] * sizeof(uint32_t));
-            dlist[k] = dlk;
+            hl += nlist[k];
         }
+        dlist = dlk;
     }
     vaco_av1pf_status st = upload_slot(ctx, a, s);
     if (st != VACO_AV1PF_OK)
@@ -750,15 +795,16 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
                                   c.h, hipMemcpyDeviceToDevice, s));
     for (int m = 0; m < 4; m++)
         if (dc[m])
-            HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, s, [&] {
+            HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, plan.cdef_bytes[m], s, [&] {
                 return launch_cdef(dc[m], (int)plan.cdef[m].size(), plan.cdef_blocks[m], m, s);
             }));
     if (dj)
-        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES, s, [&] {
+        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES, plan.bytes[VACO_AV1PF_KERNEL_SUPERRES], s, [&] {
             return launch_superres(dj, (int)plan.sr.size(), plan.sr_blocks, s);
         }));
     if (dl)
-        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION, s,
+        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION,
+                              plan.bytes[VACO_AV1PF_KERNEL_RESTORATION], s,
                               [&] { return launch_lr(dl, dlist, nlist, s); }));
     return VACO_AV1PF_OK;
 }
@@ -797,6 +843,7 @@ static hipError_t profile_collect(vaco_av1pf_ctx *ctx)
             return e;
         ctx->prof_ms[r.kernel] += ms;
         ctx->prof_n[r.kernel] += 1;
+        ctx->prof_bytes[r.kernel] += r.bytes;
     }
     ctx->prof_recs.clear();
     ctx->prof_next = 0;
@@ -804,7 +851,8 @@ static hipError_t profile_collect(vaco_av1pf_ctx *ctx)
 }
 
 template <class F>
-static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, hipStream_t s, F launch)
+static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, uint64_t bytes, hipStream_t s,
+                               F launch)
 {
     if (!ctx->prof)
         return launch();
@@ -820,7 +868,7 @@ static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, hipStream_t s, F
     if (e == hipSuccess)
         e = hipEventRecord(ctx->prof_pool[b], s);
     if (e == hipSuccess)
-        ctx->prof_recs.push_back({kernel, a, b});
+        ctx->prof_recs.push_back({kernel, a, b, bytes});
     return e;
 }
 
@@ -841,7 +889,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_profile(vaco_av1pf_ctx *ctx, int enable)
 }
 
 extern "C" vaco_av1pf_status vaco_av1pf_profile_read(vaco_av1pf_ctx *ctx, double ms[VACO_AV1PF_KERNEL_COUNT],
-                                                     uint64_t launches[VACO_AV1PF_KERNEL_COUNT])
+                                                     uint64_t launches[VACO_AV1PF_KERNEL_COUNT],
+                                                     uint64_t bytes[VACO_AV1PF_KERNEL_COUNT])
 {
     if (!ctx)
         return VACO_AV1PF_INVALID_ARGUMENT;
@@ -852,8 +901,11 @@ extern "C" vaco_av1pf_status vaco_av1pf_profile_read(vaco_av1pf_ctx *ctx, double
             ms[k] = ctx->prof_ms[k];
         if (launches)
             launches[k] = ctx->prof_n[k];
+        if (bytes)
+            bytes[k] = ctx->prof_bytes[k];
         ctx->prof_ms[k] = 0;
         ctx->prof_n[k] = 0;
+        ctx->prof_bytes[k] = 0;
     }
     return VACO_AV1PF_OK;
 }
@@ -1377,13 +1429,15 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     p8 += (cols.size() * sizeof(LrColSeg) + 255) & ~size_t(255);
     LrUnit *du = (LrUnit *)p8;
     p8 += (n_units * sizeof(LrUnit) + 255) & ~size_t(255);
-    const uint32_t *dlist[3];
+    // one list for k_lr: self-guided tiles, then Wiener, then copies
+    uint32_t *dlist = (uint32_t *)p8;
     uint32_t nlist[3];
-    for (int k = 0; k < 3; k++) {
-        dlist[k] = (const uint32_t *)p8;
+    std::vector<uint32_t> list;
+    for (int k = 0; k < 3; k++)
         nlist[k] = (uint32_t)tiles[k].size();
-        p8 += (nlist[k] * 4 + 255) & ~size_t(255);
-    }
+    for (int k = 2; k >= 0; k--)
+        list.insert(list.end(), tiles[k].begin(), tiles[k].end());
+    p8 += (ntiles * 4 + 255) & ~size_t(255);
     uint32_t *dst = (uint32_t *)p8;
     LrPlane L;
     memset(&L, 0, sizeof L);
@@ -1422,10 +1476,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
                            ctx->stream);
     if (e == hipSuccess && n_units)
         e = hipMemcpyAsync(du, units, n_units * sizeof(LrUnit), hipMemcpyHostToDevice, ctx->stream);
-    for (int k = 0; k < 3 && e == hipSuccess; k++)
-        if (nlist[k])
-            e = hipMemcpyAsync((void *)dlist[k], tiles[k].data(), nlist[k] * 4, hipMemcpyHostToDevice,
-                               ctx->stream);
+    if (e == hipSuccess && !list.empty())
+        e = hipMemcpyAsync(dlist, list.data(), list.size() * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)

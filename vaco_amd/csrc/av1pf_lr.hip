@@ -7,12 +7,15 @@
 //
 // A tile is at most 64x64 output samples inside one restoration stripe and
 // one restoration unit, so its filter type is uniform. The host sorts tiles
-// by type into three lists and each type has its own kernel, sized for it:
-//   k_lr_copy   RestorationUnit::None and planes without restoration
-//   k_lr_wiener 256 threads, ~19 KB LDS: staged source, 16-bit row pass
-//   k_lr_sgr    512 threads, ~39 KB LDS: staged source, column sums of x
-//               and x^2, box sums per centre, A/B grids (built in registers
-//               and written over the column sums), 3x3 weighted combination
+// by type and one launch of k_lr (512 threads) covers all of them: the
+// workgroups first take the self-guided tiles, then the Wiener tiles, then
+// the copies, so the longest tiles start first and the three kinds share
+// the machine. Each kind is a branch-free path of its own:
+//   copy        RestorationUnit::None and planes without restoration
+//   Wiener      ~19 KB LDS: staged source, 16-bit row pass, column pass
+//   self-guided ~39 KB LDS: staged source, column sums of x and x^2, box
+//               sums per centre, A/B grids (built in registers and written
+//               over the column sums), 3x3 weighted combination
 // Each staged window follows Source::get: CDEF output inside the stripe,
 // pre-CDEF rows across its borders, edge clamping.
 #include "av1pf_common.h"
@@ -86,16 +89,16 @@ __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int max
 
 // ---------------------------------------------------------------- copy --
 
-__global__ __launch_bounds__(256) void k_lr_copy(const LrPlane *__restrict__ planes,
-                                                 const uint32_t *__restrict__ list)
+constexpr int kLrThreads = 512;
+
+__device__ __forceinline__ void lr_copy(const Tile &t)
 {
-    const Tile t = fetch_tile(planes, list);
     const LrPlane &P = *t.P;
     const int maxv = (1 << P.bit_depth) - 1;
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
     bool bad = false;
 #pragma unroll 4
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    for (int e = threadIdx.x; e < 64 * 64; e += kLrThreads) {
         const int r = e >> 6, c = e & 63;
         if (r < th && c < tw) {
             const int v = P.after[(size_t)(y0 + r) * P.after_stride + x0 + c];
@@ -109,17 +112,17 @@ __global__ __launch_bounds__(256) void k_lr_copy(const LrPlane *__restrict__ pla
 
 // -------------------------------------------------------------- Wiener --
 
-__global__ __launch_bounds__(256) void k_lr_wiener(const LrPlane *__restrict__ planes,
-                                                   const uint32_t *__restrict__ list)
+constexpr int kWienerLds = 70 * kS * 2 + 70 * 64 * 2;
+
+__device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
 {
-    __shared__ __align__(16) uint16_t S[70 * kS];
-    __shared__ __align__(16) int16_t M[70 * 64];
-    const Tile t = fetch_tile(planes, list);
+    uint16_t *S = reinterpret_cast<uint16_t *>(lds);
+    int16_t *M = reinterpret_cast<int16_t *>(lds + 70 * kS * 2);
     const LrPlane &P = *t.P;
     const int bd = P.bit_depth, maxv = (1 << bd) - 1;
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
     const int tid = threadIdx.x;
-    if (__syncthreads_or(stage_source<256>(S, t, maxv)) && tid == 0)
+    if (__syncthreads_or(stage_source<kLrThreads>(S, t, maxv)) && tid == 0)
         raise_status(P.status, kFlagSampleRange);
 
     // wiener, restoration.rs:264-283: rows, then columns. The intermediate
@@ -135,7 +138,7 @@ __global__ __launch_bounds__(256) void k_lr_wiener(const LrPlane *__restrict__ p
     const int v3 = 128 - 2 * (v0 + v1 + v2);
     const int srows = th + 6;
 #pragma unroll 4
-    for (int e = tid; e < srows * 64; e += 256) {
+    for (int e = tid; e < srows * 64; e += kLrThreads) {
         const int r = e >> 6, c = e & 63;
         const uint16_t *s = &S[r * kS + c];
         int h = h0 * (s[0] + s[6]) + h1 * (s[1] + s[5]) + h2 * (s[2] + s[4]) + h3 * s[3];
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(256) void k_lr_wiener(const LrPlane *__restrict__ p
     }
     __syncthreads();
 #pragma unroll 4
-    for (int e = tid; e < 64 * 64; e += 256) {
+    for (int e = tid; e < 64 * 64; e += kLrThreads) {
         const int r = e >> 6, c = e & 63;
         if (r >= th || c >= tw)
             continue;
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(256) void k_lr_wiener(const LrPlane *__restrict__ p
 
 // --------------------------------------------------------- self-guided --
 
-constexpr int kSgrThreads = 512;
+constexpr int kSgrThreads = kLrThreads;
 constexpr int kCentres = (66 * kS + kSgrThreads - 1) / kSgrThreads;  // per thread
 constexpr int kSgrPx = 64 * 64 / kSgrThreads;                          // per thread
 
@@ -241,14 +244,14 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
 __device__ __forceinline__ int ab_a(uint32_t v) { return (int)(v & 511u); }
 __device__ __forceinline__ int ab_b(uint32_t v) { return (int)(v >> 9); }
 
-__global__ __launch_bounds__(kSgrThreads) void k_lr_sgr(const LrPlane *__restrict__ planes,
-                                                        const uint32_t *__restrict__ list)
-{
-    __shared__ __align__(16) uint16_t S[70 * kS];
-    __shared__ __align__(16) uint32_t X[66 * kS * 3 / 2];  // VS (u16) + VQ, then the A/B grid
-    __shared__ uint16_t xbx1[256];
+// LDS: S (staged source), X (VS u16 + VQ, then the A/B grid), xbx1.
+constexpr int kSgrLds = 70 * kS * 2 + 66 * kS * 3 / 2 * 4 + 256 * 2;
 
-    const Tile t = fetch_tile(planes, list);
+__device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
+{
+    uint16_t *S = reinterpret_cast<uint16_t *>(lds);
+    uint32_t *X = reinterpret_cast<uint32_t *>(lds + 70 * kS * 2);
+    uint16_t *xbx1 = reinterpret_cast<uint16_t *>(lds + 70 * kS * 2 + 66 * kS * 3 / 2 * 4);
     const LrPlane &P = *t.P;
     const int bd = P.bit_depth, maxv = (1 << bd) - 1;
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
@@ -317,16 +320,30 @@ __global__ __launch_bounds__(kSgrThreads) void k_lr_sgr(const LrPlane *__restric
     }
 }
 
-// lists[k]: tile entries of kind k (0 copy, 1 Wiener, 2 self-guided).
-hipError_t launch_lr(const LrPlane *planes, const uint32_t *const lists[3], const uint32_t counts[3],
+// One launch over every tile of the plan: list holds the self-guided tiles,
+// then the Wiener tiles, then the copies (n_sgr + n_wiener + n_copy
+// entries, one per workgroup).
+__global__ __launch_bounds__(kLrThreads) void k_lr(const LrPlane *__restrict__ planes,
+                                                   const uint32_t *__restrict__ list, uint32_t n_sgr,
+                                                   uint32_t n_wiener)
+{
+    __shared__ __align__(16) uint8_t lds[kSgrLds > kWienerLds ? kSgrLds : kWienerLds];
+    const Tile t = fetch_tile(planes, list);
+    if (blockIdx.x < n_sgr)
+        lr_sgr(t, lds);
+    else if (blockIdx.x < n_sgr + n_wiener)
+        lr_wiener(t, lds);
+    else
+        lr_copy(t);
+}
+
+hipError_t launch_lr(const LrPlane *planes, const uint32_t *list, const uint32_t counts[3],
                      hipStream_t s)
 {
-    if (counts[0])
-        hipLaunchKernelGGL(k_lr_copy, dim3(counts[0]), dim3(256), 0, s, planes, lists[0]);
-    if (counts[1])
-        hipLaunchKernelGGL(k_lr_wiener, dim3(counts[1]), dim3(256), 0, s, planes, lists[1]);
-    if (counts[2])
-        hipLaunchKernelGGL(k_lr_sgr, dim3(counts[2]), dim3(kSgrThreads), 0, s, planes, lists[2]);
+    const uint32_t n = counts[0] + counts[1] + counts[2];
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_lr, dim3(n), dim3(kLrThreads), 0, s, planes, list, counts[2], counts[1]);
     return hipGetLastError();
 }
 

@@ -1,0 +1,37 @@
+// av1pf_stage.hip - the side-information upload of one call.
+//
+// Each call's side information (skip map, CDEF indices, restoration units,
+// work lists, kernel descriptors) is written by the host into a pinned,
+// coherent staging buffer. Instead of a DMA transfer, whose hand-over to the
+// compute queue costs more than the copy itself for these tens of kilobytes,
+// this kernel reads the buffer across the host link and writes its device
+// mirror, in stream order with the kernels that read it.
+#include <algorithm>
+
+#include "av1pf_common.h"
+
+namespace av1pf {
+
+constexpr int kUpThreads = 256;
+constexpr uint32_t kUpMaxBlocks = 1024;
+
+// n16: 16-byte words to copy (both buffers hold at least n16 * 16 bytes).
+__global__ __launch_bounds__(kUpThreads) void k_upload(uint4 *__restrict__ dst,
+                                                       const uint4 *__restrict__ src, uint32_t n16)
+{
+    for (uint32_t i = blockIdx.x * kUpThreads + threadIdx.x; i < n16; i += gridDim.x * kUpThreads)
+        dst[i] = src[i];
+}
+
+hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    const uint32_t n16 = (uint32_t)((bytes + 15) / 16);
+    if (n16 == 0)
+        return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>(kUpMaxBlocks, (n16 + kUpThreads - 1) / kUpThreads);
+    hipLaunchKernelGGL(k_upload, dim3(blocks), dim3(kUpThreads), 0, s, (uint4 *)dst,
+                       (const uint4 *)src, n16);
+    return hipGetLastError();
+}
+
+}  // namespace av1pf
