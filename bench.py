@@ -250,6 +250,16 @@ def measure_per_frame(av1pf, hip, cfg, n_frames, steps, warmup):
     hip.synchronize()
     t1 = time.perf_counter()
 
+    # Host cost of one call with the staging ring empty (no waits on the GPU).
+    host_us = []
+    for rep in range(8):
+        for i in range(6):
+            t = time.perf_counter()
+            calls[i % len(calls)]()
+            host_us.append((time.perf_counter() - t) * 1e6)
+        ctx.finish()
+    host_us.sort()
+
     # Kernel durations from HIP events on the library's stream (a separate
     # pass over the same steps, so event records do not perturb the timing).
     ctx.profile(True)
@@ -264,6 +274,7 @@ def measure_per_frame(av1pf, hip, cfg, n_frames, steps, warmup):
         "mpix_s": round(mpix * steps / (t1 - t0), 2),
         "ms_per_step": round((t1 - t0) * 1e3 / steps, 4),
         "host_enqueue_ms_per_step": round((t_enq - t0) * 1e3 / steps, 4),
+        "host_call_us_median": round(host_us[len(host_us) // 2], 2),
         "kernels": kernel_table(prof, steps),
         "_frames": frames,
         "_outputs": outputs,

@@ -46,6 +46,8 @@ constexpr int bin_count(int d, int b)
 }
 
 constexpr int kRow = 80;         // LDS tile row: columns px0-8 .. px0+71
+constexpr int kTile = 68 * kRow; // one plane's tile
+constexpr int kShift = 3 * kTile; // tile p + kShift: plane p shifted by one sample
 constexpr int kThreads = 1024;
 
 // Cost of direction D (cdef.rs:104-113) of the 8x8 block whose first row
@@ -97,11 +99,17 @@ __device__ __forceinline__ uint32_t dir_cost_any(int d, const uint16_t *p, int s
 }
 
 // Filter parameters of one 8x8 block (one plane class), cdef.rs:35-54.
-struct __align__(16) Blk {
-    int16_t off[6];      // LDS tap offsets: dir d1, d2; dir+2 d1, d2; dir+6 d1, d2
+// off[k] is the LDS offset of tap k (dir d1, d2; dir+2 d1, d2; dir+6 d1,
+// d2). The packed path reads sample pairs starting at ci + off and ci - off
+// as one aligned 32-bit word: for an odd offset that pair starts at an odd
+// index, so it is read from the shifted copy of the tile (index i stored
+// at i + 1), at offsets offp / offm.
+struct __align__(8) Blk {
+    int16_t off[6];
     uint8_t pri, sec;    // strengths after scaling / variance adjustment
     uint8_t shifts;      // damping - ilog2(strength), primary | secondary << 4
     uint8_t flags;       // 1: filter this block, 2: primary taps are (3, 3)
+    int16_t offp[6], offm[6];
 };
 
 __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int damping,
@@ -120,6 +128,12 @@ __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int
         B.off[k] = (int16_t)(kDirDy[dir][k] * row + kDirDx[dir][k]);
         B.off[2 + k] = (int16_t)(kDirDy[d2][k] * row + kDirDx[d2][k]);
         B.off[4 + k] = (int16_t)(kDirDy[d6][k] * row + kDirDx[d6][k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int o = B.off[k];
+        B.offp[k] = (int16_t)((o & 1) ? kShift + 1 + o : o);
+        B.offm[k] = (int16_t)((o & 1) ? kShift + 1 - o : -o);
     }
     return B;
 }
@@ -174,17 +188,19 @@ __device__ __forceinline__ int filter_px(const uint16_t *t, int ci, const Blk &B
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
+// Samples i and i + 1 of a tile as one aligned 32-bit LDS read (i even).
 __device__ __forceinline__ s16x2 ld2(const uint16_t *t, int i)
 {
-    return s16x2{(short)t[i], (short)t[i + 1]};
+    return *reinterpret_cast<const s16x2 *>(t + i);
 }
 
-__device__ __forceinline__ void tap2_pk(const uint16_t *t, int ci, int off, s16x2 x, s16x2 S,
-                                        s16x2 sh, s16x2 w, s16x2 &sum, s16x2 &mn, s16x2 &mx)
+__device__ __forceinline__ void tap2_pk(const uint16_t *t, int ci, int offp, int offm, s16x2 x,
+                                        s16x2 S, s16x2 sh, s16x2 w, s16x2 &sum, s16x2 &mn,
+                                        s16x2 &mx)
 {
 #pragma unroll
     for (int s = 0; s < 2; s++) {
-        const s16x2 p = ld2(t, s ? ci + off : ci - off);
+        const s16x2 p = ld2(t, ci + (s ? offp : offm));
         const s16x2 d = p - x;
         const s16x2 ad = __builtin_elementwise_max(d, -d);
         const s16x2 q = (s16x2)((u16x2)ad >> (u16x2)sh);
@@ -203,18 +219,19 @@ __device__ __forceinline__ s16x2 filter_pair(const uint16_t *t, int ci, const Bl
     const s16x2 P = (s16x2)(short)B.pri, Q = (s16x2)(short)B.sec;
     const s16x2 ps = (s16x2)(short)(B.shifts & 15), qs = (s16x2)(short)(B.shifts >> 4);
     const short t0 = (B.flags & 2) ? 3 : 4;
-    tap2_pk(t, ci, B.off[0], x, P, ps, (s16x2)t0, sum, mn, mx);
-    tap2_pk(t, ci, B.off[1], x, P, ps, (s16x2)(short)(6 - t0), sum, mn, mx);
-    tap2_pk(t, ci, B.off[2], x, Q, qs, (s16x2)2, sum, mn, mx);
-    tap2_pk(t, ci, B.off[3], x, Q, qs, (s16x2)1, sum, mn, mx);
-    tap2_pk(t, ci, B.off[4], x, Q, qs, (s16x2)2, sum, mn, mx);
-    tap2_pk(t, ci, B.off[5], x, Q, qs, (s16x2)1, sum, mn, mx);
+    tap2_pk(t, ci, B.offp[0], B.offm[0], x, P, ps, (s16x2)t0, sum, mn, mx);
+    tap2_pk(t, ci, B.offp[1], B.offm[1], x, P, ps, (s16x2)(short)(6 - t0), sum, mn, mx);
+    tap2_pk(t, ci, B.offp[2], B.offm[2], x, Q, qs, (s16x2)2, sum, mn, mx);
+    tap2_pk(t, ci, B.offp[3], B.offm[3], x, Q, qs, (s16x2)1, sum, mn, mx);
+    tap2_pk(t, ci, B.offp[4], B.offm[4], x, Q, qs, (s16x2)2, sum, mn, mx);
+    tap2_pk(t, ci, B.offp[5], B.offm[5], x, Q, qs, (s16x2)1, sum, mn, mx);
     const s16x2 y = x + ((sum + (s16x2)8 + (sum >> (s16x2)15)) >> (s16x2)4);
     return __builtin_elementwise_min(__builtin_elementwise_max(y, mn), mx);
 }
 
 // Stage rows ys .. ys+ROWS-1, columns xs .. xs+COLS-1 of a plane into an LDS
 // tile at column offset 6; positions outside the plane hold kUnavailable.
+// Interior superblocks also fill the shifted copy at dst + kShift.
 template <int ROWS, int COLS>
 __device__ __forceinline__ bool stage(uint16_t *dst, const uint16_t *src, int W, int H, int st,
                                       int xs, int ys, bool border, int maxv)
@@ -230,6 +247,8 @@ __device__ __forceinline__ bool stage(uint16_t *dst, const uint16_t *src, int W,
             bad |= v > maxv;
         }
         dst[r * kRow + c + 6] = v;
+        if (!border)
+            dst[kShift + r * kRow + c + 7] = v;
     }
     return bad;
 }
@@ -238,7 +257,7 @@ __device__ __forceinline__ bool stage(uint16_t *dst, const uint16_t *src, int W,
 // Luma: 16 lanes per 8x8 block, rows i0, i0+2, i0+4, i0+6 of column j.
 // Chroma: 16 lanes per chroma block, BW*BH/16 samples each, both planes.
 template <bool BORDER, int SX, int SY>
-__device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*lt)[68 * kRow],
+__device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*lt)[kTile],
                                           const Blk (*blk)[64], const uint8_t *bact, bool any,
                                           int x0, int y0)
 {
@@ -292,7 +311,7 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
 // Luma: 16 lanes per 8x8 block, pairs in rows i0 and i0+4 at columns
 // 2*j2, 2*j2+1. Chroma: 16 lanes per block, pairs of U then V.
 template <int SX, int SY>
-__device__ __forceinline__ void filter_sb_packed(const CdefFrame &f, const uint16_t (*lt)[68 * kRow],
+__device__ __forceinline__ void filter_sb_packed(const CdefFrame &f, const uint16_t (*lt)[kTile],
                                                  const Blk (*blk)[64], const uint8_t *bact,
                                                  bool any, int x0, int y0)
 {
@@ -348,13 +367,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
 {
     constexpr int CW = 64 >> SX, CH = 64 >> SY, BW = 8 >> SX, BH = 8 >> SY;
     constexpr int CPX = BW * BH / 16;  // chroma samples per thread per plane
-    __shared__ __align__(16) uint16_t lt[3][68 * kRow];
+    __shared__ __align__(16) uint16_t lt[6][kTile];  // planes, then their shifted copies
     __shared__ Blk blk[2][64];
     __shared__ uint32_t cost[8][64];
     __shared__ uint8_t bact[64];
 
-    const CdefFrame &f = find_item(frames, n_frames, blockIdx.x);
-    const uint32_t sb = blockIdx.x - f.block_begin;
+    const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+    const CdefFrame &f = find_item(frames, n_frames, wg);
+    const uint32_t sb = wg - f.block_begin;
     const int sby = (int)(sb / f.sb_cols), sbx = (int)(sb - (uint32_t)sby * f.sb_cols);
     const int tid = threadIdx.x;
     const int bd = f.bit_depth, shift = bd - 8, maxv = (1 << bd) - 1;

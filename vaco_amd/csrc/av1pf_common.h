@@ -22,6 +22,17 @@ __device__ __forceinline__ void raise_status(uint32_t *status, uint32_t bits)
     atomicOr_system(status, bits);
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
+// XCD b % 8) and each XCD has its own L2. The remap gives each XCD one
+// contiguous range of the n work items, so neighbouring items, which share
+// halo lines, are fetched into the same L2. It is a bijection of [0, n).
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n)
+{
+    constexpr uint32_t kXcds = 8;
+    const uint32_t per = n / kXcds, rem = n % kXcds, x = b % kXcds, l = b / kXcds;
+    return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
+}
+
 // Work item owning workgroup `b`: the last item whose block_begin <= b.
 template <class T>
 __device__ __forceinline__ const T &find_item(const T *items, int n, uint32_t b)
