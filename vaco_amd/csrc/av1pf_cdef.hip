@@ -6,12 +6,15 @@
 // decode.rs:739-861 with cdef_block_skipped 892-896.
 //
 // One workgroup of 1024 threads (16 wave64) per 64x64 luma superblock and
-// its co-located chroma. The superblock plus a 2-sample border is staged in
-// LDS once; the direction search keeps each block's 15 line sums in
-// registers (one wave per direction, one lane per 8x8 block, 16-byte LDS
-// row loads); then 16 lanes filter each 8x8 luma block (4 samples each)
-// with the block's parameters held in registers. Only superblocks that
-// touch a plane edge pay for tap-availability checks.
+// its co-located chroma, in XCD-contiguous order. The superblock plus a
+// 2-sample border is staged in LDS once (8-byte loads inside the plane),
+// with a copy shifted by one sample so that every pair of samples is one
+// aligned 32-bit LDS read. The direction search keeps each block's 15 line
+// sums in registers (one wave per direction, one lane per 8x8 block,
+// 16-byte LDS row loads). Then 16 lanes filter each 8x8 luma block, two
+// samples at a time in packed 16-bit arithmetic, with the block's
+// parameters held in registers. Superblocks that touch a plane edge add one
+// bit select per tap for unavailable taps.
 #include "av1pf_common.h"
 
 namespace av1pf {
@@ -138,8 +141,9 @@ __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int
     return B;
 }
 
-// Both taps of one direction/distance, cdef.rs:171-198. constrain() is
-// min(|d|, max(0, S - (|d| >> shift))) with the sign of d.
+// Both taps of one direction/distance, cdef.rs:171-198. constrain()
+// (cdef.rs:126-133) is sign(d) * min(|d|, T) with T = max(0, S - (|d| >>
+// shift)); since T >= 0 that is d clamped to [-T, T].
 template <bool BORDER>
 __device__ __forceinline__ void tap2(const uint16_t *t, int ci, int off, int x, int S, int sh,
                                      int w, int &sum, int &mn, int &mx)
@@ -148,9 +152,8 @@ __device__ __forceinline__ void tap2(const uint16_t *t, int ci, int off, int x, 
     for (int s = 0; s < 2; s++) {
         const int p = t[s ? ci + off : ci - off];
         const int d = p - x;
-        const int ad = abs(d);
-        const int m = min(max(S - (ad >> sh), 0), ad);
-        const int c = d < 0 ? -m : m;
+        const int T = max(S - (abs(d) >> sh), 0);
+        const int c = min(max(d, -T), T);
         if (BORDER) {  // unavailable taps are skipped (cdef.rs:189-191)
             const bool valid = p != kUnavailable;
             sum += valid ? w * c : 0;
@@ -194,24 +197,35 @@ __device__ __forceinline__ s16x2 ld2(const uint16_t *t, int i)
     return *reinterpret_cast<const s16x2 *>(t + i);
 }
 
+// An unavailable tap (kUnavailable, -1 as int16) is skipped by the
+// reference (cdef.rs:189-191): no term, no part in the clamp range. Taking
+// the centre sample in its place has exactly that effect (constrain(0) = 0,
+// and x is already in [mn, mx]), so BORDER superblocks swap it in with one
+// bit select per tap.
+template <bool BORDER>
 __device__ __forceinline__ void tap2_pk(const uint16_t *t, int ci, int offp, int offm, s16x2 x,
                                         s16x2 S, s16x2 sh, s16x2 w, s16x2 &sum, s16x2 &mn,
                                         s16x2 &mx)
 {
 #pragma unroll
     for (int s = 0; s < 2; s++) {
-        const s16x2 p = ld2(t, ci + (s ? offp : offm));
+        s16x2 p = ld2(t, ci + (s ? offp : offm));
+        if (BORDER) {
+            const uint32_t m = __builtin_bit_cast(uint32_t, p >> (s16x2)15);
+            const uint32_t v = (m & __builtin_bit_cast(uint32_t, x)) | (~m & __builtin_bit_cast(uint32_t, p));
+            p = __builtin_bit_cast(s16x2, v);
+        }
         const s16x2 d = p - x;
         const s16x2 ad = __builtin_elementwise_max(d, -d);
         const s16x2 q = (s16x2)((u16x2)ad >> (u16x2)sh);
-        const s16x2 m = __builtin_elementwise_min(__builtin_elementwise_max(S - q, (s16x2)0), ad);
-        const s16x2 sg = d >> (s16x2)15;
-        sum += w * ((m ^ sg) - sg);
+        const s16x2 T = __builtin_elementwise_max(S - q, (s16x2)0);
+        sum += w * __builtin_elementwise_min(__builtin_elementwise_max(d, -T), T);
         mn = __builtin_elementwise_min(mn, p);
         mx = __builtin_elementwise_max(mx, p);
     }
 }
 
+template <bool BORDER>
 __device__ __forceinline__ s16x2 filter_pair(const uint16_t *t, int ci, const Blk &B)
 {
     const s16x2 x = ld2(t, ci);
@@ -219,19 +233,19 @@ __device__ __forceinline__ s16x2 filter_pair(const uint16_t *t, int ci, const Bl
     const s16x2 P = (s16x2)(short)B.pri, Q = (s16x2)(short)B.sec;
     const s16x2 ps = (s16x2)(short)(B.shifts & 15), qs = (s16x2)(short)(B.shifts >> 4);
     const short t0 = (B.flags & 2) ? 3 : 4;
-    tap2_pk(t, ci, B.offp[0], B.offm[0], x, P, ps, (s16x2)t0, sum, mn, mx);
-    tap2_pk(t, ci, B.offp[1], B.offm[1], x, P, ps, (s16x2)(short)(6 - t0), sum, mn, mx);
-    tap2_pk(t, ci, B.offp[2], B.offm[2], x, Q, qs, (s16x2)2, sum, mn, mx);
-    tap2_pk(t, ci, B.offp[3], B.offm[3], x, Q, qs, (s16x2)1, sum, mn, mx);
-    tap2_pk(t, ci, B.offp[4], B.offm[4], x, Q, qs, (s16x2)2, sum, mn, mx);
-    tap2_pk(t, ci, B.offp[5], B.offm[5], x, Q, qs, (s16x2)1, sum, mn, mx);
+    tap2_pk<BORDER>(t, ci, B.offp[0], B.offm[0], x, P, ps, (s16x2)t0, sum, mn, mx);
+    tap2_pk<BORDER>(t, ci, B.offp[1], B.offm[1], x, P, ps, (s16x2)(short)(6 - t0), sum, mn, mx);
+    tap2_pk<BORDER>(t, ci, B.offp[2], B.offm[2], x, Q, qs, (s16x2)2, sum, mn, mx);
+    tap2_pk<BORDER>(t, ci, B.offp[3], B.offm[3], x, Q, qs, (s16x2)1, sum, mn, mx);
+    tap2_pk<BORDER>(t, ci, B.offp[4], B.offm[4], x, Q, qs, (s16x2)2, sum, mn, mx);
+    tap2_pk<BORDER>(t, ci, B.offp[5], B.offm[5], x, Q, qs, (s16x2)1, sum, mn, mx);
     const s16x2 y = x + ((sum + (s16x2)8 + (sum >> (s16x2)15)) >> (s16x2)4);
     return __builtin_elementwise_min(__builtin_elementwise_max(y, mn), mx);
 }
 
 // Stage rows ys .. ys+ROWS-1, columns xs .. xs+COLS-1 of a plane into an LDS
-// tile at column offset 6; positions outside the plane hold kUnavailable.
-// Interior superblocks also fill the shifted copy at dst + kShift.
+// tile at column offset 6, and into the shifted copy at dst + kShift;
+// positions outside the plane hold kUnavailable.
 template <int ROWS, int COLS>
 __device__ __forceinline__ bool stage(uint16_t *dst, const uint16_t *src, int W, int H, int st,
                                       int xs, int ys, bool border, int maxv)
@@ -247,73 +261,49 @@ __device__ __forceinline__ bool stage(uint16_t *dst, const uint16_t *src, int W,
             bad |= v > maxv;
         }
         dst[r * kRow + c + 6] = v;
-        if (!border)
-            dst[kShift + r * kRow + c + 7] = v;
+        dst[kShift + r * kRow + c + 7] = v;
     }
     return bad;
 }
 
-// Filter (or pass through) every sample of a staged superblock and store it.
-// Luma: 16 lanes per 8x8 block, rows i0, i0+2, i0+4, i0+6 of column j.
-// Chroma: 16 lanes per chroma block, BW*BH/16 samples each, both planes.
+// stage() for interior superblocks of planes with 8-byte aligned rows: the
+// window widened to start at column xs - 2 and loaded four samples at a
+// time (the two extra columns on each side are never read by a tap). Also
+// fills the shifted copy.
+template <int ROWS, int COLS>
+__device__ __forceinline__ bool stage_vec(uint16_t *dst, const uint16_t *src, int st, int xs, int ys,
+                                          int maxv)
+{
+    constexpr int V = (COLS + 4) / 4;  // 4-sample groups per row
+    bool bad = false;
+#pragma unroll 2
+    for (int e = threadIdx.x; e < ROWS * V; e += kThreads) {
+        const int r = e / V, v = e - r * V;
+        const uint2 q =
+            *reinterpret_cast<const uint2 *>(src + (size_t)(ys + r) * st + xs - 2 + 4 * v);
+        const uint32_t m = max(max(q.x & 0xffffu, q.x >> 16), max(q.y & 0xffffu, q.y >> 16));
+        bad |= (int)m > maxv;
+        uint16_t *t = dst + r * kRow + 4 + 4 * v;
+        *reinterpret_cast<uint2 *>(t) = q;
+        uint16_t *u = t + kShift + 1;  // sample i of the tile at i + 1
+        u[0] = (uint16_t)q.x;
+        *reinterpret_cast<uint32_t *>(u + 1) = (q.x >> 16) | (q.y << 16);
+        u[3] = (uint16_t)(q.y >> 16);
+    }
+    return bad;
+}
+
+// Filter (or pass through) every sample of a staged superblock and store
+// it, two horizontally adjacent samples per lane (filter_pair). Luma: 16
+// lanes per 8x8 block, pairs in rows i0 and i0+4 at columns 2*j2, 2*j2+1.
+// Chroma: 16 lanes per block, pairs of U then V. BORDER: the superblock
+// touches a plane edge (unavailable taps; partial superblocks store only
+// samples inside the plane; plane widths are even, so a pair is all in or
+// all out).
 template <bool BORDER, int SX, int SY>
 __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*lt)[kTile],
                                           const Blk (*blk)[64], const uint8_t *bact, bool any,
                                           int x0, int y0)
-{
-    constexpr int BW = 8 >> SX, BH = 8 >> SY, CPX = BW * BH / 16;
-    const int tid = threadIdx.x;
-    const int b = tid >> 4, q = tid & 15;
-    const int bi = b >> 3, bj = b & 7;
-    const bool act = any && bact[b];
-    {
-        const int j = q & 7, i0 = q >> 3;
-        Blk B = {};
-        if (act)
-            B = blk[0][b];
-        const uint16_t *t = lt[0];
-        uint16_t *dst = f.out[0];
-        const int ost = f.out_stride[0], W0 = f.width[0], H0 = f.height[0];
-#pragma unroll 2
-        for (int k = 0; k < 4; k++) {
-            const int i = i0 + 2 * k;
-            const int x = x0 + bj * 8 + j, y = y0 + bi * 8 + i;
-            const int ci = (2 + bi * 8 + i) * kRow + 8 + bj * 8 + j;
-            const int res = act ? filter_px<BORDER>(t, ci, B) : t[ci];
-            if (!BORDER || (x < W0 && y < H0))
-                dst[(size_t)y * ost + x] = (uint16_t)res;
-        }
-    }
-    if (f.nplanes > 1) {
-        Blk B = {};
-        if (act)
-            B = blk[1][b];
-        const int W1 = f.width[1], H1 = f.height[1];
-        const int cx0 = x0 >> SX, cy0 = y0 >> SY;
-#pragma unroll
-        for (int k = 0; k < CPX; k++) {
-            const int pi = q + 16 * k, i = pi / BW, j = pi % BW;
-            const int x = cx0 + bj * BW + j, y = cy0 + bi * BH + i;
-            const int ci = (2 + bi * BH + i) * kRow + 8 + bj * BW + j;
-            const bool store = !BORDER || (x < W1 && y < H1);
-#pragma unroll
-            for (int p = 1; p < 3; p++) {
-                const uint16_t *t = lt[p];
-                const int res = act ? filter_px<BORDER>(t, ci, B) : t[ci];
-                if (store)
-                    f.out[p][(size_t)y * f.out_stride[p] + x] = (uint16_t)res;
-            }
-        }
-    }
-}
-
-// Interior superblocks: filter_sb with two samples per lane (filter_pair).
-// Luma: 16 lanes per 8x8 block, pairs in rows i0 and i0+4 at columns
-// 2*j2, 2*j2+1. Chroma: 16 lanes per block, pairs of U then V.
-template <int SX, int SY>
-__device__ __forceinline__ void filter_sb_packed(const CdefFrame &f, const uint16_t (*lt)[kTile],
-                                                 const Blk (*blk)[64], const uint8_t *bact,
-                                                 bool any, int x0, int y0)
 {
     constexpr int BW = 8 >> SX, BH = 8 >> SY, PPP = BW * BH / 2, CPX = BW * BH / 16;
     const int tid = threadIdx.x;
@@ -327,23 +317,25 @@ __device__ __forceinline__ void filter_sb_packed(const CdefFrame &f, const uint1
             B = blk[0][b];
         const uint16_t *t = lt[0];
         uint16_t *dst = f.out[0];
-        const int ost = f.out_stride[0];
+        const int ost = f.out_stride[0], W0 = f.width[0], H0 = f.height[0];
 #pragma unroll
         for (int k = 0; k < 2; k++) {
             const int i = i0 + 4 * k;
             const int x = x0 + bj * 8 + j, y = y0 + bi * 8 + i;
             const int ci = (2 + bi * 8 + i) * kRow + 8 + bj * 8 + j;
-            const s16x2 res = act ? filter_pair(t, ci, B) : ld2(t, ci);
-            uint16_t *o = dst + (size_t)y * ost + x;
-            o[0] = (uint16_t)res.x;
-            o[1] = (uint16_t)res.y;
+            const s16x2 res = act ? filter_pair<BORDER>(t, ci, B) : ld2(t, ci);
+            if (!BORDER || (x < W0 && y < H0)) {
+                uint16_t *o = dst + (size_t)y * ost + x;
+                o[0] = (uint16_t)res.x;
+                o[1] = (uint16_t)res.y;
+            }
         }
     }
     if (f.nplanes > 1) {
         Blk B = {};
         if (act)
             B = blk[1][b];
-        const int cx0 = x0 >> SX, cy0 = y0 >> SY;
+        const int cx0 = x0 >> SX, cy0 = y0 >> SY, W1 = f.width[1], H1 = f.height[1];
 #pragma unroll
         for (int k = 0; k < CPX; k++) {
             const int pi = q + 16 * k;
@@ -352,10 +344,12 @@ __device__ __forceinline__ void filter_sb_packed(const CdefFrame &f, const uint1
             const int x = cx0 + bj * BW + j, y = cy0 + bi * BH + i;
             const int ci = (2 + bi * BH + i) * kRow + 8 + bj * BW + j;
             const uint16_t *t = lt[p];
-            const s16x2 res = act ? filter_pair(t, ci, B) : ld2(t, ci);
-            uint16_t *o = f.out[p] + (size_t)y * f.out_stride[p] + x;
-            o[0] = (uint16_t)res.x;
-            o[1] = (uint16_t)res.y;
+            const s16x2 res = act ? filter_pair<BORDER>(t, ci, B) : ld2(t, ci);
+            if (!BORDER || (x < W1 && y < H1)) {
+                uint16_t *o = f.out[p] + (size_t)y * f.out_stride[p] + x;
+                o[0] = (uint16_t)res.x;
+                o[1] = (uint16_t)res.y;
+            }
         }
     }
 }
@@ -389,13 +383,25 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
         bact[tid] = idx >= 0 && bx < W0 && by < H0 &&
                     f.skip8[(size_t)(by >> 3) * f.skip_stride + (bx >> 3)] == 0;
     }
-    bool bad = stage<68, 68>(lt[0], f.in[0], W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border, maxv);
-    if (nplanes > 1) {
-        const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
-        bad |= stage<CH + 4, CW + 4>(lt[1], f.in[1], f.width[1], f.height[1], f.in_stride[1], cx,
-                                     cy, border, maxv);
-        bad |= stage<CH + 4, CW + 4>(lt[2], f.in[2], f.width[2], f.height[2], f.in_stride[2], cx,
-                                     cy, border, maxv);
+    bool bad = false;
+    const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
+    if (border || !f.vec_ok) {
+        bad = stage<68, 68>(lt[0], f.in[0], W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border, maxv);
+        if (nplanes > 1) {
+            bad |= stage<CH + 4, CW + 4>(lt[1], f.in[1], f.width[1], f.height[1], f.in_stride[1],
+                                         cx, cy, border, maxv);
+            bad |= stage<CH + 4, CW + 4>(lt[2], f.in[2], f.width[2], f.height[2], f.in_stride[2],
+                                         cx, cy, border, maxv);
+        }
+    } else {
+        // Interior: rows y0-2 .. y0+65 and columns x0-4 .. x0+67 lie inside the
+        // plane, because padded sizes are multiples of 8 and the superblock
+        // does not touch the last 8 rows or columns (likewise for chroma).
+        bad = stage_vec<68, 68>(lt[0], f.in[0], f.in_stride[0], x0 - 2, y0 - 2, maxv);
+        if (nplanes > 1) {
+            bad |= stage_vec<CH + 4, CW + 4>(lt[1], f.in[1], f.in_stride[1], cx, cy, maxv);
+            bad |= stage_vec<CH + 4, CW + 4>(lt[2], f.in[2], f.in_stride[2], cx, cy, maxv);
+        }
     }
     if (__syncthreads_or(bad) && tid == 0)
         raise_status(f.status, kFlagSampleRange);
@@ -457,7 +463,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     if (border)
         filter_sb<true, SX, SY>(f, lt, blk, bact, any, x0, y0);
     else
-        filter_sb_packed<SX, SY>(f, lt, blk, bact, any, x0, y0);
+        filter_sb<false, SX, SY>(f, lt, blk, bact, any, x0, y0);
 }
 
 // Kernel-level cases (cdef::find_direction + cdef::filter_block on one

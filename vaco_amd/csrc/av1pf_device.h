@@ -42,6 +42,7 @@ struct CdefFrame {
     uint32_t block_begin;
     uint8_t bit_depth, ss_x, ss_y, nplanes;
     uint8_t damping;                   // CdefDamping (includes +3)
+    uint8_t vec_ok;                    // inputs 8-byte aligned, strides multiples of 4
     uint8_t y_pri[8], y_sec[8], uv_pri[8], uv_sec[8];
 };
 

@@ -602,6 +602,10 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
         f.ss_y = d.subsampling_y;
         f.nplanes = (uint8_t)np;
         f.damping = d.cdef.damping;
+        f.vec_ok = 1;
+        for (int p = 0; p < np; p++)
+            if (((uintptr_t)in[p].data & 7) || (in[p].stride & 3) || (g.in_w[p] & 3))
+                f.vec_ok = 0;
         for (int i = 0; i < 8; i++) {
             f.y_pri[i] = d.cdef.y[i].primary;
             f.y_sec[i] = d.cdef.y[i].secondary;
