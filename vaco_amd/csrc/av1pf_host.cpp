@@ -55,6 +55,7 @@ struct Arena {
     uint8_t *d = nullptr;   // device mirror
     size_t cap = 0, used = 0;
     hipEvent_t done = nullptr;  // the frames that used this slot have finished
+    hipEvent_t up_done = nullptr;  // its upload (on the context's upload stream) has finished
     bool pending = false;
     uint32_t *status = nullptr;  // kMaxBatch words, one per frame
     // frames whose status must be checked before reuse
@@ -79,6 +80,7 @@ struct Arena {
 struct vaco_av1pf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t up_stream = nullptr;  // side-information uploads, ahead of the work
     uint32_t max_w = 0, max_h = 0;
     size_t plane_cap = 0;          // samples per plane buffer
     uint16_t *pool = nullptr;      // 5 images x 3 planes
@@ -201,12 +203,19 @@ template <class F>
 static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, uint64_t bytes, hipStream_t s,
                                F launch);
 
+// Copy the slot's staged bytes to its device mirror on the upload stream,
+// which is otherwise idle, so the copy runs while earlier frames still
+// compute; stream s waits for it before the call's kernels. The slot's
+// previous frame has finished (take_slot), so its mirror is free.
 static vaco_av1pf_status upload_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t s)
 {
     if (a->used == 0)
         return VACO_AV1PF_OK;
-    HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_UPLOAD, (uint64_t)a->used * 2, s,
-                          [&] { return launch_upload(a->d, a->hd, a->used, s); }));
+    hipStream_t u = ctx->up_stream;
+    HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_UPLOAD, (uint64_t)a->used * 2, u,
+                          [&] { return launch_upload(a->d, a->hd, a->used, u); }));
+    HIPCHECK(hipEventRecord(a->up_done, u));
+    HIPCHECK(hipStreamWaitEvent(s, a->up_done, 0));
     return VACO_AV1PF_OK;
 }
 
@@ -246,6 +255,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_create(int device, uint32_t max_width, u
     if (e != hipSuccess)
         return bail(e);
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&ctx->up_stream, hipStreamNonBlocking);
     if (e != hipSuccess)
         return bail(e);
     // Mi-padded (multiple of 8) maximum geometry.
@@ -268,6 +279,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_create(int device, uint32_t max_width, u
         if (e != hipSuccess)
             return bail(e);
         e = hipEventCreateWithFlags(&a.done, hipEventDisableTiming);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&a.up_done, hipEventDisableTiming);
         if (e != hipSuccess)
             return bail(e);
     }
@@ -292,6 +305,8 @@ extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
     hipSetDevice(ctx->device);
     if (ctx->stream)
         hipStreamSynchronize(ctx->stream);
+    if (ctx->up_stream)
+        hipStreamSynchronize(ctx->up_stream);
     for (int i = 0; i < kSlots; i++) {
         if (ctx->slot[i].h)
             hipHostFree(ctx->slot[i].h);
@@ -299,6 +314,8 @@ extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
             hipFree(ctx->slot[i].d);
         if (ctx->slot[i].done)
             hipEventDestroy(ctx->slot[i].done);
+        if (ctx->slot[i].up_done)
+            hipEventDestroy(ctx->slot[i].up_done);
     }
     if (ctx->h_status)
         hipHostFree(ctx->h_status);
@@ -310,6 +327,8 @@ extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
         hipFree(ctx->ws);
     if (ctx->stream)
         hipStreamDestroy(ctx->stream);
+    if (ctx->up_stream)
+        hipStreamDestroy(ctx->up_stream);
     delete ctx;
 }
 
