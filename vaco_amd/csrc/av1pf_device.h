@@ -107,7 +107,8 @@ struct LrPlane {
     uint32_t n_rows, n_cols, unit_cols;
     uint32_t block_begin;
     uint8_t bit_depth, ss_y;
-    uint16_t pad;
+    uint8_t vec_ok;                    // before/after 8-byte aligned, strides multiples of 4
+    uint8_t pad;
 };
 
 }  // namespace av1pf

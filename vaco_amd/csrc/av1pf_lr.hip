@@ -67,6 +67,35 @@ __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int max
     const int x0 = t.cs.x0, y0 = t.rs.y0, ss = t.rs.stripe_start, se = t.rs.stripe_end;
     const int srows = t.rs.h + 6, scols = t.cs.w + 6;
     bool bad = false;
+    if (P.vec_ok && t.cs.w == 64 && x0 >= 4 && x0 + 68 <= W) {
+        // Columns x0-4 .. x0+67 all inside the plane: 8-byte loads of four
+        // samples (the column left of the window is loaded and dropped).
+        constexpr int V = 18;
+#pragma unroll 2
+        for (int e = threadIdx.x; e < 70 * V; e += THREADS) {
+            const int r = e / V, v = e - r * V;
+            if (r >= srows)
+                continue;
+            const int yc = min(max(y0 - 3 + r, 0), H - 1);
+            const uint16_t *row;
+            if (yc < ss)
+                row = P.before + (size_t)max(yc, ss - 2) * P.before_stride;
+            else if (yc > se)
+                row = P.before + (size_t)min(yc, se + 2) * P.before_stride;
+            else
+                row = P.after + (size_t)yc * P.after_stride;
+            const uint2 q = *reinterpret_cast<const uint2 *>(row + x0 - 4 + 4 * v);
+            const uint32_t m = max(max(q.x & 0xffffu, q.x >> 16), max(q.y & 0xffffu, q.y >> 16));
+            bad |= (int)m > maxv;
+            uint16_t *d = S + r * kS + 4 * v - 1;  // column c of S is x0 - 3 + c
+            if (v)
+                d[0] = (uint16_t)q.x;
+            d[1] = (uint16_t)(q.x >> 16);
+            d[2] = (uint16_t)q.y;
+            d[3] = (uint16_t)(q.y >> 16);
+        }
+        return bad;
+    }
 #pragma unroll 4
     for (int e = threadIdx.x; e < 70 * kS; e += THREADS) {
         const int r = e / kS, c = e - r * kS;
@@ -162,7 +191,6 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
 // --------------------------------------------------------- self-guided --
 
 constexpr int kSgrThreads = kLrThreads;
-constexpr int kCentres = (66 * kS + kSgrThreads - 1) / kSgrThreads;  // per thread
 constexpr int kSgrPx = 64 * 64 / kSgrThreads;                          // per thread
 
 // A/B of one box from its sums (restoration.rs:314-339), packed as
@@ -186,22 +214,27 @@ __device__ __forceinline__ uint32_t ab_of(int sum, int sumsq, int bd, uint32_t s
 
 // Column sums over 2R+1 staged rows into VS/VQ, then per centre (y0-1+cr,
 // x0-1+cc), cr < th+2, cc < tw+2, the packed A/B, written over VS/VQ as
-// grid AB (row stride kG). Pass 0 (R = 2) uses odd rows only.
+// grid AB (row stride kG). Pass 0 (R = 2) needs odd rows y only
+// (restoration.rs:347-356), so it walks only those centre rows.
 template <int R>
 __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th, int tw, int y0,
                                          int bd, int eps, const uint16_t *xbx1)
 {
     constexpr int N = (2 * R + 1) * (2 * R + 1);
+    constexpr int STEP = R == 2 ? 2 : 1;  // centre rows walked
+    constexpr int KC = ((R == 2 ? 34 : 66) * kS + kSgrThreads - 1) / kSgrThreads;
     const uint32_t n2e = (uint32_t)(N * N * eps);
     const uint32_t scale = ((1u << 20) + n2e / 2) / n2e;
     const int crows = th + 2, ccols = tw + 2, scols = tw + 6;
+    const int cr0 = R == 2 ? (y0 & 1) : 0;  // first centre row with y0-1+cr odd
+    const int nr = (crows - cr0 + STEP - 1) / STEP;
     const int tid = threadIdx.x;
     uint16_t *VS = reinterpret_cast<uint16_t *>(X);
     int *VQ = reinterpret_cast<int *>(X) + 66 * kS / 2;
 #pragma unroll 2
-    for (int e = tid; e < crows * kS; e += kSgrThreads) {
-        const int cr = e / kS, c = e - cr * kS;
-        if (c >= scols || (R == 2 && !((y0 - 1 + cr) & 1)))
+    for (int e = tid; e < nr * kS; e += kSgrThreads) {
+        const int k = e / kS, c = e - k * kS, cr = cr0 + STEP * k;
+        if (c >= scols)
             continue;
         int s = 0, q = 0;
 #pragma unroll
@@ -214,13 +247,13 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
         VQ[cr * kS + c] = q;
     }
     __syncthreads();
-    uint32_t ab[kCentres];
+    uint32_t ab[KC];
 #pragma unroll
-    for (int k = 0; k < kCentres; k++) {
-        const int e = tid + k * kSgrThreads;
-        const int cr = e / kS, cc = e - cr * kS;
-        ab[k] = 0;
-        if (cr >= crows || cc >= ccols || (R == 2 && !((y0 - 1 + cr) & 1)))
+    for (int i = 0; i < KC; i++) {
+        const int e = tid + i * kSgrThreads;
+        const int k = e / kS, cc = e - k * kS, cr = cr0 + STEP * k;
+        ab[i] = 0;
+        if (k >= nr || cc >= ccols)
             continue;
         int s = 0, q = 0;
 #pragma unroll
@@ -228,15 +261,15 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
             s += VS[cr * kS + cc + 2 + dx];
             q += VQ[cr * kS + cc + 2 + dx];
         }
-        ab[k] = ab_of<N>(s, q, bd, scale, xbx1);
+        ab[i] = ab_of<N>(s, q, bd, scale, xbx1);
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kCentres; k++) {
-        const int e = tid + k * kSgrThreads;
-        const int cr = e / kS, cc = e - cr * kS;
-        if (cr < crows && cc < ccols)
-            X[cr * kG + cc] = ab[k];
+    for (int i = 0; i < KC; i++) {
+        const int e = tid + i * kSgrThreads;
+        const int k = e / kS, cc = e - k * kS, cr = cr0 + STEP * k;
+        if (k < nr && cc < ccols)
+            X[cr * kG + cc] = ab[i];
     }
     __syncthreads();
 }
