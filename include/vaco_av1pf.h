@@ -338,8 +338,9 @@ enum {
 };
 
 /* No reference counterpart (measurement hook). While enabled, every kernel
- * launch of the frame entries is bracketed by HIP events on the stream it
- * runs on; vaco_av1pf_profile_read() synchronises the context and returns,
+ * launch of the frame entries records HIP events at the start and end of
+ * its dispatch (the interval a kernel trace reports);
+ * vaco_av1pf_profile_read() synchronises the context and returns,
  * per kernel id, the summed launch durations in milliseconds, the launch
  * counts and the algorithmic bytes of those launches since the last read.
  * Algorithmic bytes count every sample a kernel must read once and every

@@ -524,16 +524,19 @@ __global__ __launch_bounds__(64) void k_cdef_cases(const uint16_t *__restrict__ 
     outs[(size_t)cs * 64 + lane] = res;
 }
 
-// mode: 0 = 4:2:0, 1 = 4:4:4 or monochrome, 2 = 4:2:2, 3 = 4:4:0
-hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, hipStream_t s)
+// mode: 0 = 4:2:0, 1 = 4:4:4 or monochrome, 2 = 4:2:2, 3 = 4:4:0. ev0/ev1
+// (may be null) receive the dispatch's start and end.
+hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, hipStream_t s,
+                       hipEvent_t ev0, hipEvent_t ev1)
 {
     if (blocks == 0)
         return hipSuccess;
+    const dim3 g(blocks), b(kThreads);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_cdef<1, 1>), dim3(blocks), dim3(kThreads), 0, s, frames, n); break;
-    case 1: hipLaunchKernelGGL((k_cdef<0, 0>), dim3(blocks), dim3(kThreads), 0, s, frames, n); break;
-    case 2: hipLaunchKernelGGL((k_cdef<1, 0>), dim3(blocks), dim3(kThreads), 0, s, frames, n); break;
-    default: hipLaunchKernelGGL((k_cdef<0, 1>), dim3(blocks), dim3(kThreads), 0, s, frames, n); break;
+    case 0: hipExtLaunchKernelGGL((k_cdef<1, 1>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    case 1: hipExtLaunchKernelGGL((k_cdef<0, 0>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    case 2: hipExtLaunchKernelGGL((k_cdef<1, 0>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    default: hipExtLaunchKernelGGL((k_cdef<0, 1>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
     }
     return hipGetLastError();
 }

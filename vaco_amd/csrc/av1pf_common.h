@@ -1,6 +1,7 @@
 // av1pf_common.h - device helpers shared by the stage's kernels.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "av1pf_device.h"

@@ -27,11 +27,14 @@
 #include "av1pf_internal.h"
 
 namespace av1pf {
-hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, hipStream_t s);
-hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s);
+hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, hipStream_t s,
+                       hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s,
+                           hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_lr(const LrPlane *planes, const uint32_t *list, const uint32_t counts[3],
-                     hipStream_t s);
-hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s);
+                     hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s,
+                         hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_cdef_cases(const uint16_t *inputs, const void *params, uint8_t *dirs,
                              uint32_t *vars, uint16_t *outs, uint32_t n, hipStream_t s);
 }  // namespace av1pf
@@ -213,7 +216,9 @@ static vaco_av1pf_status upload_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t 
         return VACO_AV1PF_OK;
     hipStream_t u = ctx->up_stream;
     HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_UPLOAD, (uint64_t)a->used * 2, u,
-                          [&] { return launch_upload(a->d, a->hd, a->used, u); }));
+                          [&](hipEvent_t e0, hipEvent_t e1) {
+                              return launch_upload(a->d, a->hd, a->used, u, e0, e1);
+                          }));
     HIPCHECK(hipEventRecord(a->up_done, u));
     HIPCHECK(hipStreamWaitEvent(s, a->up_done, 0));
     return VACO_AV1PF_OK;
@@ -820,17 +825,23 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
                                   c.h, hipMemcpyDeviceToDevice, s));
     for (int m = 0; m < 4; m++)
         if (dc[m])
-            HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, plan.cdef_bytes[m], s, [&] {
-                return launch_cdef(dc[m], (int)plan.cdef[m].size(), plan.cdef_blocks[m], m, s);
-            }));
+            HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, plan.cdef_bytes[m], s,
+                                  [&](hipEvent_t e0, hipEvent_t e1) {
+                                      return launch_cdef(dc[m], (int)plan.cdef[m].size(),
+                                                         plan.cdef_blocks[m], m, s, e0, e1);
+                                  }));
     if (dj)
-        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES, plan.bytes[VACO_AV1PF_KERNEL_SUPERRES], s, [&] {
-            return launch_superres(dj, (int)plan.sr.size(), plan.sr_blocks, s);
-        }));
+        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES, plan.bytes[VACO_AV1PF_KERNEL_SUPERRES],
+                              s, [&](hipEvent_t e0, hipEvent_t e1) {
+                                  return launch_superres(dj, (int)plan.sr.size(), plan.sr_blocks,
+                                                         s, e0, e1);
+                              }));
     if (dl)
         HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION,
                               plan.bytes[VACO_AV1PF_KERNEL_RESTORATION], s,
-                              [&] { return launch_lr(dl, dlist, nlist, s); }));
+                              [&](hipEvent_t e0, hipEvent_t e1) {
+                                  return launch_lr(dl, dlist, nlist, s, e0, e1);
+                              }));
     return VACO_AV1PF_OK;
 }
 
@@ -875,23 +886,23 @@ static hipError_t profile_collect(vaco_av1pf_ctx *ctx)
     return hipSuccess;
 }
 
+// launch(ev0, ev1) dispatches one kernel; while profiling, ev0 and ev1 are
+// set by the dispatch itself at its start and end (hipExtLaunchKernel), the
+// same interval a kernel trace reports.
 template <class F>
 static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, uint64_t bytes, hipStream_t s,
                                F launch)
 {
+    (void)s;
     if (!ctx->prof)
-        return launch();
+        return launch(nullptr, nullptr);
     if (ctx->prof_next + 2 > ctx->prof_pool.size()) {
         hipError_t e = profile_collect(ctx);
         if (e != hipSuccess)
             return e;
     }
     const uint32_t a = ctx->prof_next++, b = ctx->prof_next++;
-    hipError_t e = hipEventRecord(ctx->prof_pool[a], s);
-    if (e == hipSuccess)
-        e = launch();
-    if (e == hipSuccess)
-        e = hipEventRecord(ctx->prof_pool[b], s);
+    hipError_t e = launch(ctx->prof_pool[a], ctx->prof_pool[b]);
     if (e == hipSuccess)
         ctx->prof_recs.push_back({kernel, a, b, bytes});
     return e;

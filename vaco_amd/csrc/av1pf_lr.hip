@@ -370,13 +370,15 @@ __global__ __launch_bounds__(kLrThreads) void k_lr(const LrPlane *__restrict__ p
         lr_copy(t);
 }
 
+// ev0/ev1 (may be null) receive the dispatch's start and end.
 hipError_t launch_lr(const LrPlane *planes, const uint32_t *list, const uint32_t counts[3],
-                     hipStream_t s)
+                     hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     const uint32_t n = counts[0] + counts[1] + counts[2];
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_lr, dim3(n), dim3(kLrThreads), 0, s, planes, list, counts[2], counts[1]);
+    hipExtLaunchKernelGGL(k_lr, dim3(n), dim3(kLrThreads), 0, s, ev0, ev1, 0, planes, list,
+                          counts[2], counts[1]);
     return hipGetLastError();
 }
 

@@ -23,14 +23,16 @@ __global__ __launch_bounds__(kUpThreads) void k_upload(uint4 *__restrict__ dst,
         dst[i] = src[i];
 }
 
-hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s)
+// ev0/ev1 (may be null) receive the dispatch's start and end.
+hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s, hipEvent_t ev0,
+                         hipEvent_t ev1)
 {
     const uint32_t n16 = (uint32_t)((bytes + 15) / 16);
     if (n16 == 0)
         return hipSuccess;
     const uint32_t blocks = std::min<uint32_t>(kUpMaxBlocks, (n16 + kUpThreads - 1) / kUpThreads);
-    hipLaunchKernelGGL(k_upload, dim3(blocks), dim3(kUpThreads), 0, s, (uint4 *)dst,
-                       (const uint4 *)src, n16);
+    hipExtLaunchKernelGGL(k_upload, dim3(blocks), dim3(kUpThreads), 0, s, ev0, ev1, 0,
+                          (uint4 *)dst, (const uint4 *)src, n16);
     return hipGetLastError();
 }
 

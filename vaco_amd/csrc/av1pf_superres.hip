@@ -96,11 +96,13 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
     }
 }
 
-hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s)
+// ev0/ev1 (may be null) receive the dispatch's start and end.
+hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s, hipEvent_t ev0,
+                           hipEvent_t ev1)
 {
     if (blocks == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_superres, dim3(blocks), dim3(kSrThreads), 0, s, jobs, n);
+    hipExtLaunchKernelGGL(k_superres, dim3(blocks), dim3(kSrThreads), 0, s, ev0, ev1, 0, jobs, n);
     return hipGetLastError();
 }
 
