@@ -102,17 +102,18 @@ __device__ __forceinline__ uint32_t dir_cost_any(int d, const uint16_t *p, int s
 }
 
 // Filter parameters of one 8x8 block (one plane class), cdef.rs:35-54.
-// off[k] is the LDS offset of tap k (dir d1, d2; dir+2 d1, d2; dir+6 d1,
-// d2). The packed path reads sample pairs starting at ci + off and ci - off
-// as one aligned 32-bit word: for an odd offset that pair starts at an odd
-// index, so it is read from the shifted copy of the tile (index i stored
-// at i + 1), at offsets offp / offm.
-struct __align__(8) Blk {
+// off[k] is the LDS offset (in samples) of tap k (dir d1, d2; dir+2 d1, d2;
+// dir+6 d1, d2). The packed path reads sample pairs starting at ci + off and
+// ci - off as one aligned 32-bit word: for an odd offset that pair starts at
+// an odd index, so it is read from the shifted copy of the tile (index i
+// stored at i + 1). offp / offm are those two reads' byte offsets from the
+// pair's own address, resolved to the copy that makes them aligned.
+struct __align__(16) Blk {
+    int32_t offp[6], offm[6];
     int16_t off[6];
     uint8_t pri, sec;    // strengths after scaling / variance adjustment
     uint8_t shifts;      // damping - ilog2(strength), primary | secondary << 4
     uint8_t flags;       // 1: filter this block, 2: primary taps are (3, 3)
-    int16_t offp[6], offm[6];
 };
 
 __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int damping,
@@ -135,8 +136,8 @@ __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         const int o = B.off[k];
-        B.offp[k] = (int16_t)((o & 1) ? kShift + 1 + o : o);
-        B.offm[k] = (int16_t)((o & 1) ? kShift + 1 - o : -o);
+        B.offp[k] = 2 * ((o & 1) ? kShift + 1 + o : o);
+        B.offm[k] = 2 * ((o & 1) ? kShift + 1 - o : -o);
     }
     return B;
 }
@@ -184,10 +185,9 @@ __device__ __forceinline__ int filter_px(const uint16_t *t, int ci, const Blk &B
 }
 
 // Packed form of filter_px for two horizontally adjacent samples of one
-// block (interior superblocks: every tap available). Every quantity fits
-// int16: samples <= 4095, differences within +-4095, and the tap sum is at
-// most (4+2)*2*240 + (2+1)*4*64 = 3648, so each operation works on both
-// samples at once (v_pk_*_i16).
+// block. Every quantity fits int16: samples <= 4095, differences within
+// +-4095, and the tap sum is at most 2*(4+2)*240 + 4*(2+1)*64 = 3648, so
+// each operation works on both samples at once (v_pk_*_i16).
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
@@ -197,50 +197,77 @@ __device__ __forceinline__ s16x2 ld2(const uint16_t *t, int i)
     return *reinterpret_cast<const s16x2 *>(t + i);
 }
 
-// An unavailable tap (kUnavailable, -1 as int16) is skipped by the
-// reference (cdef.rs:189-191): no term, no part in the clamp range. Taking
-// the centre sample in its place has exactly that effect (constrain(0) = 0,
-// and x is already in [mn, mx]), so BORDER superblocks swap it in with one
-// bit select per tap.
-template <bool BORDER>
-__device__ __forceinline__ void tap2_pk(const uint16_t *t, int ci, int offp, int offm, s16x2 x,
-                                        s16x2 S, s16x2 sh, s16x2 w, s16x2 &sum, s16x2 &mn,
-                                        s16x2 &mx)
+// Samples i and i + 1 at byte address a of a tile (a multiple of 4).
+__device__ __forceinline__ s16x2 ldb(const char *a)
 {
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-        s16x2 p = ld2(t, ci + (s ? offp : offm));
-        if (BORDER) {
-            const uint32_t m = __builtin_bit_cast(uint32_t, p >> (s16x2)15);
-            const uint32_t v = (m & __builtin_bit_cast(uint32_t, x)) | (~m & __builtin_bit_cast(uint32_t, p));
-            p = __builtin_bit_cast(s16x2, v);
-        }
-        const s16x2 d = p - x;
-        const s16x2 ad = __builtin_elementwise_max(d, -d);
-        const s16x2 q = (s16x2)((u16x2)ad >> (u16x2)sh);
-        const s16x2 T = __builtin_elementwise_max(S - q, (s16x2)0);
-        sum += w * __builtin_elementwise_min(__builtin_elementwise_max(d, -T), T);
-        mn = __builtin_elementwise_min(mn, p);
-        mx = __builtin_elementwise_max(mx, p);
-    }
+    return *reinterpret_cast<const s16x2 *>(a);
 }
 
-template <bool BORDER>
-__device__ __forceinline__ s16x2 filter_pair(const uint16_t *t, int ci, const Blk &B)
+// filter_block (cdef.rs:142-205) for N pairs of horizontally adjacent
+// samples of one block, the pairs STRIDE samples apart, starting at tile
+// index ci (even). The N computations run interleaved, which fills the
+// dependency gaps of each chain.
+//
+// constrain(d) (cdef.rs:126-133) is sign(d) * min(|d|, T) with T = max(0,
+// S - (|d| >> shift)). With samples unsigned, a = p -sat x and b = x -sat p
+// (saturating at 0) give |d| = a | b, T = S -sat (|d| >> shift) and
+// constrain(d) = min(a, T) - min(b, T): one of a, b is zero.
+//
+// An unavailable tap (kUnavailable) is skipped by the reference
+// (cdef.rs:189-191): no term, no part in the clamp range. Taking the centre
+// sample in its place has exactly that effect (constrain(0) = 0, and x is
+// already in [mn, mx]), so BORDER superblocks swap it in with one bit
+// select per tap.
+template <bool BORDER, int N, int STRIDE>
+__device__ __forceinline__ void filter_pairs(const uint16_t *t, int ci, const Blk &B, s16x2 *res)
 {
-    const s16x2 x = ld2(t, ci);
-    s16x2 sum = (s16x2)0, mn = x, mx = x;
-    const s16x2 P = (s16x2)(short)B.pri, Q = (s16x2)(short)B.sec;
-    const s16x2 ps = (s16x2)(short)(B.shifts & 15), qs = (s16x2)(short)(B.shifts >> 4);
+    const char *a = reinterpret_cast<const char *>(t + ci);
+    u16x2 x[N], mn[N], mx[N];
+    s16x2 sum[N];
+#pragma unroll
+    for (int n = 0; n < N; n++) {
+        x[n] = __builtin_bit_cast(u16x2, ldb(a + 2 * STRIDE * n));
+        sum[n] = (s16x2)0;
+        mn[n] = mx[n] = x[n];
+    }
+    const u16x2 P = (u16x2)(unsigned short)B.pri, Q = (u16x2)(unsigned short)B.sec;
+    const u16x2 ps = (u16x2)(unsigned short)(B.shifts & 15), qs = (u16x2)(unsigned short)(B.shifts >> 4);
     const short t0 = (B.flags & 2) ? 3 : 4;
-    tap2_pk<BORDER>(t, ci, B.offp[0], B.offm[0], x, P, ps, (s16x2)t0, sum, mn, mx);
-    tap2_pk<BORDER>(t, ci, B.offp[1], B.offm[1], x, P, ps, (s16x2)(short)(6 - t0), sum, mn, mx);
-    tap2_pk<BORDER>(t, ci, B.offp[2], B.offm[2], x, Q, qs, (s16x2)2, sum, mn, mx);
-    tap2_pk<BORDER>(t, ci, B.offp[3], B.offm[3], x, Q, qs, (s16x2)1, sum, mn, mx);
-    tap2_pk<BORDER>(t, ci, B.offp[4], B.offm[4], x, Q, qs, (s16x2)2, sum, mn, mx);
-    tap2_pk<BORDER>(t, ci, B.offp[5], B.offm[5], x, Q, qs, (s16x2)1, sum, mn, mx);
-    const s16x2 y = x + ((sum + (s16x2)8 + (sum >> (s16x2)15)) >> (s16x2)4);
-    return __builtin_elementwise_min(__builtin_elementwise_max(y, mn), mx);
+    const s16x2 wk[6] = {(s16x2)t0, (s16x2)(short)(6 - t0), (s16x2)2, (s16x2)1, (s16x2)2, (s16x2)1};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const u16x2 S = k < 2 ? P : Q, sh = k < 2 ? ps : qs;
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int o = s ? B.offp[k] : B.offm[k];
+#pragma unroll
+            for (int n = 0; n < N; n++) {
+                u16x2 p = __builtin_bit_cast(u16x2, ldb(a + 2 * STRIDE * n + o));
+                if (BORDER) {
+                    const uint32_t m =
+                        __builtin_bit_cast(uint32_t, __builtin_bit_cast(s16x2, p) >> (s16x2)15);
+                    const uint32_t v = (m & __builtin_bit_cast(uint32_t, x[n])) |
+                                       (~m & __builtin_bit_cast(uint32_t, p));
+                    p = __builtin_bit_cast(u16x2, v);
+                }
+                const u16x2 up = __builtin_elementwise_sub_sat(p, x[n]);
+                const u16x2 dn = __builtin_elementwise_sub_sat(x[n], p);
+                const u16x2 T = __builtin_elementwise_sub_sat(S, (up | dn) >> sh);
+                const s16x2 c = __builtin_bit_cast(s16x2, __builtin_elementwise_min(up, T)) -
+                                __builtin_bit_cast(s16x2, __builtin_elementwise_min(dn, T));
+                sum[n] += wk[k] * c;
+                mn[n] = __builtin_elementwise_min(mn[n], p);
+                mx[n] = __builtin_elementwise_max(mx[n], p);
+            }
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < N; n++) {
+        const s16x2 xs = __builtin_bit_cast(s16x2, x[n]);
+        const s16x2 y = xs + ((sum[n] + (s16x2)8 + (sum[n] >> (s16x2)15)) >> (s16x2)4);  // cdef.rs:200-201
+        res[n] = __builtin_elementwise_min(__builtin_elementwise_max(y, __builtin_bit_cast(s16x2, mn[n])),
+                                           __builtin_bit_cast(s16x2, mx[n]));
+    }
 }
 
 // Stage rows ys .. ys+ROWS-1, columns xs .. xs+COLS-1 of a plane into an LDS
@@ -294,7 +321,7 @@ __device__ __forceinline__ bool stage_vec(uint16_t *dst, const uint16_t *src, in
 }
 
 // Filter (or pass through) every sample of a staged superblock and store
-// it, two horizontally adjacent samples per lane (filter_pair). Luma: 16
+// it, two horizontally adjacent samples per lane (filter_pairs). Luma: 16
 // lanes per 8x8 block, pairs in rows i0 and i0+4 at columns 2*j2, 2*j2+1.
 // Chroma: 16 lanes per block, pairs of U then V. BORDER: the superblock
 // touches a plane edge (unavailable taps; partial superblocks store only
@@ -318,12 +345,19 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
         const uint16_t *t = lt[0];
         uint16_t *dst = f.out[0];
         const int ost = f.out_stride[0], W0 = f.width[0], H0 = f.height[0];
+        const int ci0 = (2 + bi * 8 + i0) * kRow + 8 + bj * 8 + j;
+        s16x2 r2[2];
+        if (act) {
+            filter_pairs<BORDER, 2, 4 * kRow>(t, ci0, B, r2);
+        } else {
+            r2[0] = ld2(t, ci0);
+            r2[1] = ld2(t, ci0 + 4 * kRow);
+        }
 #pragma unroll
         for (int k = 0; k < 2; k++) {
             const int i = i0 + 4 * k;
             const int x = x0 + bj * 8 + j, y = y0 + bi * 8 + i;
-            const int ci = (2 + bi * 8 + i) * kRow + 8 + bj * 8 + j;
-            const s16x2 res = act ? filter_pair<BORDER>(t, ci, B) : ld2(t, ci);
+            const s16x2 res = r2[k];
             if (!BORDER || (x < W0 && y < H0)) {
                 uint16_t *o = dst + (size_t)y * ost + x;
                 o[0] = (uint16_t)res.x;
@@ -344,7 +378,11 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
             const int x = cx0 + bj * BW + j, y = cy0 + bi * BH + i;
             const int ci = (2 + bi * BH + i) * kRow + 8 + bj * BW + j;
             const uint16_t *t = lt[p];
-            const s16x2 res = act ? filter_pair<BORDER>(t, ci, B) : ld2(t, ci);
+            s16x2 res;
+            if (act)
+                filter_pairs<BORDER, 1, 0>(t, ci, B, &res);
+            else
+                res = ld2(t, ci);
             if (!BORDER || (x < W1 && y < H1)) {
                 uint16_t *o = f.out[p] + (size_t)y * f.out_stride[p] + x;
                 o[0] = (uint16_t)res.x;

@@ -48,9 +48,16 @@ struct CdefFrame {
 
 // ------------------------------------------------------------- superres --
 
-// One plane of one image: superres.rs:163-216. Workgroups cover
-// rows x segments of kSrSeg output samples.
+// One plane of one image: superres.rs:163-216. Workgroups cover blocks of
+// kSrRows rows x segments of kSrSeg output samples.
 constexpr int kSrSeg = 1024;
+constexpr int kSrRows = 8;
+
+// Workgroups of a superres job of n_rows rows and segs segments.
+inline uint32_t sr_job_blocks(uint32_t n_rows, uint32_t segs)
+{
+    return (n_rows + kSrRows - 1) / kSrRows * segs;
+}
 
 struct SrJob {
     const uint16_t *in;

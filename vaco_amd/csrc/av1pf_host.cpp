@@ -699,7 +699,7 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
                 j.start = g.sr_start[p];
                 j.block_begin = plan.sr_blocks;
                 j.max_sample = (uint16_t)((1u << d.bit_depth) - 1);
-                plan.sr_blocks += j.n_rows * j.segs;
+                plan.sr_blocks += sr_job_blocks(j.n_rows, j.segs);
                 plan.bytes[VACO_AV1PF_KERNEL_SUPERRES] +=
                     (uint64_t)j.n_rows * (g.sr_vis[p] + g.sr_out_w[p]) * 2;
                 plan.sr.push_back(j);
@@ -1399,7 +1399,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_upscale_plane(vaco_av1pf_ctx *ctx, const
     if (e == hipSuccess)
         e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-        e = launch_superres(dj, 1, j.n_rows * j.segs, ctx->stream);
+        e = launch_superres(dj, 1, sr_job_blocks(j.n_rows, j.segs), ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpy2DAsync(out->data, out->stride * 2, dout, cfg->output_width * 2,
                              cfg->output_width * 2, cfg->height, hipMemcpyDeviceToHost, ctx->stream);

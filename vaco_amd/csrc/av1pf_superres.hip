@@ -45,40 +45,47 @@ __constant__ int16_t kUpscale[64][8] = {
 };
 
 constexpr int kSrThreads = 256;
-constexpr int kSrPer = kSrSeg / kSrThreads;  // outputs per thread
+constexpr int kSrPer = kSrSeg / kSrThreads;  // output columns per thread
+constexpr int kSrWin = kSrSeg + 16;          // staged source samples per row
 
-// One row segment of kSrSeg output samples (superres.rs:173-214). The
-// source window the segment's taps read (edge-clamped to the padded width)
-// is staged in LDS once; it is exactly the union of the taps, because the
-// step is below one source sample, so the sample-range flag is raised
-// exactly when the reference would reject a tap.
+// One segment of kSrSeg output columns over up to kSrRows rows
+// (superres.rs:173-214). Positions, phases and filters depend on the column
+// only, so each thread derives them once for its columns and applies them to
+// every row. Each row's source window (the union of the segment's taps,
+// edge-clamped to the padded width) is staged in LDS; because the step is
+// below one source sample the window is exactly the samples the taps read,
+// so the sample-range flag is raised exactly when the reference would reject
+// a tap.
 __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict__ jobs, int n_jobs)
 {
     __shared__ __align__(16) int16_t tab[64 * 8];
-    __shared__ uint16_t win[kSrSeg + 16];
+    __shared__ uint16_t win[kSrRows][kSrWin];
     const SrJob &j = find_item(jobs, n_jobs, blockIdx.x);
     const uint32_t local = blockIdx.x - j.block_begin;
-    const uint32_t ri = local / j.segs;
-    const uint32_t seg = local - ri * j.segs;
-    const uint32_t y = j.rows ? j.rows[ri] : ri;
+    const uint32_t rb = local / j.segs;
+    const uint32_t seg = local - rb * j.segs;
+    const int r0 = (int)rb * kSrRows;
+    const int nr = min(kSrRows, (int)j.n_rows - r0);
     const int tid = threadIdx.x;
     const int x0 = (int)seg * kSrSeg;
     const int xn = min(x0 + kSrSeg, (int)j.out_width) - 1;
     const int lo = ((-(1 << 14) + j.start + x0 * j.step) >> 14) - 3;
     const int hi = ((-(1 << 14) + j.start + xn * j.step) >> 14) + 4;
-    const uint16_t *row = j.in + (size_t)y * j.in_stride;
     const int last = (int)j.in_width - 1;
     bool bad = false;
-    for (int i = tid; i <= hi - lo; i += kSrThreads) {
-        const int v = row[min(max(lo + i, 0), last)];
-        bad |= v > j.max_sample;
-        win[i] = (uint16_t)v;
+    for (int r = 0; r < nr; r++) {
+        const uint32_t y = j.rows ? j.rows[r0 + r] : (uint32_t)(r0 + r);
+        const uint16_t *row = j.in + (size_t)y * j.in_stride;
+        for (int i = tid; i <= hi - lo; i += kSrThreads) {
+            const int v = row[min(max(lo + i, 0), last)];
+            bad |= v > j.max_sample;
+            win[r][i] = (uint16_t)v;
+        }
     }
     for (int i = tid; i < 64 * 8; i += kSrThreads)
         tab[i] = kUpscale[i >> 3][i & 7];
     if (__syncthreads_or(bad) && tid == 0)
         raise_status(j.status, kFlagSampleRange);
-    uint16_t *out = j.out + (size_t)y * j.out_stride;
 #pragma unroll
     for (int k = 0; k < kSrPer; k++) {
         const int x = x0 + tid + k * kSrThreads;
@@ -86,13 +93,20 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
             break;
         const int src_x = -(1 << 14) + j.start + x * j.step;
         const int base = (src_x >> 14) - 3 - lo;
-        const int16_t *f = &tab[((src_x & 0x3fff) >> 8) * 8];
-        int sum = 0;
+        const int16_t *fp = &tab[((src_x & 0x3fff) >> 8) * 8];
+        int f[8];
 #pragma unroll
         for (int t = 0; t < 8; t++)
-            sum += win[base + t] * f[t];
-        const int r = (sum + 64) >> 7;
-        out[x] = (uint16_t)min(max(r, 0), (int)j.max_sample);
+            f[t] = fp[t];
+        for (int r = 0; r < nr; r++) {
+            const uint32_t y = j.rows ? j.rows[r0 + r] : (uint32_t)(r0 + r);
+            int sum = 0;
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                sum += win[r][base + t] * f[t];
+            const int v = (sum + 64) >> 7;
+            j.out[(size_t)y * j.out_stride + x] = (uint16_t)min(max(v, 0), (int)j.max_sample);
+        }
     }
 }
 
