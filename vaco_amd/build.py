@@ -15,6 +15,9 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # keep hipcc from forming packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # split LDS accesses the compiler cannot prove aligned into aligned ones:
+    # merged unaligned ds_read_b64/b96 stall the LDS pipeline on gfx950
+    "-Xclang", "-target-feature", "-Xclang", "-unaligned-access-mode",
     "-Wall", "-Wno-unused-command-line-argument",
 ]
 
