@@ -60,6 +60,7 @@ def measure_mixed_batch(av1pf, hip, workloads, reps: int, n: int = 64):
     ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
     per_frame = [ctx.device_call(f, ptr(dev_in)[k], ptr(out_pf)[k]) for k, f in enumerate(frames)]
     batch = ctx.batch_call(frames, ptr(dev_in), ptr(out_b))
+    tuning = ctx.autotune(frames, ptr(dev_in), ptr(out_b), reps=2)  # untimed
     for c in per_frame:
         c()
     batch()
@@ -92,6 +93,7 @@ def measure_mixed_batch(av1pf, hip, workloads, reps: int, n: int = 64):
         "batched_ms_per_batch": round(t_b * 1e3, 3),
         "bit_exact_vs_per_frame": bool(exact),
         "reps": reps,
+        "tuning": tuning,
     }
 
 
@@ -236,6 +238,8 @@ def measure_per_frame(av1pf, hip, cfg, n_frames, steps, warmup):
     hip.synchronize()
     calls = [ctx.device_call(f, [t.ptr for t in dev_in[k]], [t.ptr for t in dev_out[k]])
              for k, f in enumerate(frames)]
+    tuning = ctx.autotune(frames[:1], [[t.ptr for t in dev_in[0]]], [[t.ptr for t in dev_out[0]]],
+                          reps=3)  # untimed
     for i in range(warmup):
         calls[i % len(calls)]()
     ctx.finish()
@@ -276,6 +280,7 @@ def measure_per_frame(av1pf, hip, cfg, n_frames, steps, warmup):
         "host_enqueue_ms_per_step": round((t_enq - t0) * 1e3 / steps, 4),
         "host_call_us_median": round(host_us[len(host_us) // 2], 2),
         "kernels": kernel_table(prof, steps),
+        "tuning": tuning,
         "_frames": frames,
         "_outputs": outputs,
     }
@@ -295,6 +300,7 @@ def measure_4k10(av1pf, hip, workloads, reps: int, n: int = 16):
     ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
     calls = [ctx.device_call(f, ptr(dev_in)[k], ptr(out_pf)[k]) for k, f in enumerate(frames)]
     batch = ctx.batch_call(frames, ptr(dev_in), ptr(out_b))
+    tuning = ctx.autotune(frames, ptr(dev_in), ptr(out_b), reps=1)  # untimed
     for c in calls:
         c()
     batch()
@@ -335,6 +341,7 @@ def measure_4k10(av1pf, hip, workloads, reps: int, n: int = 16):
         "batched_mpix_s": round(mpix / t_b, 2),
         "stage_hbm_frac_per_frame": round(total_bytes / t_pf / 1e9 / HBM_PEAK_GBS, 5),
         "kernels_per_frame": kern,
+        "tuning": tuning,
         "bit_exact_frame0_vs_oracle": bool(exact0),
         "batched_equals_per_frame": bool(same),
     }
@@ -443,6 +450,7 @@ def main():
             "parallelism": "single GPU",
         },
         "host_enqueue_ms_per_step": per_frame["host_enqueue_ms_per_step"],
+        "tuning": per_frame["tuning"],
         "kernels": kern,
         "roofline": {
             "bound": "hbm",

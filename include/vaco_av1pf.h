@@ -241,6 +241,32 @@ vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32_t n,
                                           const vaco_av1pf_picture *ins,
                                           vaco_av1pf_picture *outs, void *stream);
 
+/* ------------------------------------------------------------ tuning --- */
+
+/* No reference counterpart: the launch variants an autotuner chooses
+ * between. Every variant computes the same output bit for bit; they differ
+ * only in work order and workgroup shape. */
+typedef struct vaco_av1pf_tuning {
+    uint8_t superres_rows;   /* rows per superres workgroup: 4, 8 or 16 */
+    uint8_t cdef_xcd_order;  /* 1: superblocks numbered XCD-contiguously */
+    uint8_t lr_order;        /* 0: restoration tiles by unit type, self-guided first;
+                                1: in plane order, numbered XCD-contiguously */
+    uint8_t reserved;        /* 0 */
+} vaco_av1pf_tuning;
+
+vaco_av1pf_status vaco_av1pf_get_tuning(const vaco_av1pf_ctx *ctx, vaco_av1pf_tuning *out);
+/* INVALID_ARGUMENT for values outside the ranges above. Applies to calls
+ * made after it returns. */
+vaco_av1pf_status vaco_av1pf_set_tuning(vaco_av1pf_ctx *ctx, const vaco_av1pf_tuning *tuning);
+/* Runs `n` frames (device planes, as vaco_av1pf_batch_device) `reps` times
+ * with each variant of each kernel, times the kernels by their dispatch
+ * events, keeps the fastest variant per kernel as the context's tuning and
+ * returns it in *out. Synchronous; the outputs hold the frames' results. */
+vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n,
+                                      const vaco_av1pf_frame_desc *descs,
+                                      const vaco_av1pf_picture *ins, vaco_av1pf_picture *outs,
+                                      uint32_t reps, vaco_av1pf_tuning *out);
+
 /* ----------------------------------------------------------- serving --- */
 
 /* No reference peer: a request loop over many streams on one device. The

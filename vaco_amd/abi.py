@@ -125,6 +125,15 @@ class FrameDesc(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("superres_rows", C.c_uint8),
+        ("cdef_xcd_order", C.c_uint8),
+        ("lr_order", C.c_uint8),
+        ("reserved", C.c_uint8),
+    ]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_uint64),

@@ -113,6 +113,10 @@ def lib():
                                               C.c_void_p]
         L.vaco_av1pf_profile.argtypes = [ctxp, C.c_int]
         L.vaco_av1pf_profile_read.argtypes = [ctxp, P(C.c_double), P(C.c_uint64), P(C.c_uint64)]
+        L.vaco_av1pf_get_tuning.argtypes = [ctxp, P(abi.Tuning)]
+        L.vaco_av1pf_set_tuning.argtypes = [ctxp, P(abi.Tuning)]
+        L.vaco_av1pf_autotune.argtypes = [ctxp, C.c_uint32, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture),
+                                          C.c_uint32, P(abi.Tuning)]
         L.vaco_av1pf_server_create.argtypes = [ctxp, P(abi.ServerConfig), P(C.c_void_p)]
         L.vaco_av1pf_server_submit.argtypes = [C.c_void_p, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture),
                                                C.c_uint64]
@@ -124,7 +128,8 @@ def lib():
         L.vaco_av1pf_now_ns.restype = C.c_uint64
         L.vaco_av1pf_server_frame_bytes.argtypes = [P(abi.FrameDesc)]
         L.vaco_av1pf_server_frame_bytes.restype = C.c_uint64
-        for name in ("vaco_av1pf_server_create", "vaco_av1pf_server_submit", "vaco_av1pf_server_drain",
+        for name in ("vaco_av1pf_get_tuning", "vaco_av1pf_set_tuning", "vaco_av1pf_autotune",
+                     "vaco_av1pf_server_create", "vaco_av1pf_server_submit", "vaco_av1pf_server_drain",
                      "vaco_av1pf_batch_device", "vaco_av1pf_profile", "vaco_av1pf_profile_read", "vaco_av1pf_create", "vaco_av1pf_frame", "vaco_av1pf_frame_device", "vaco_av1pf_finish",
                      "vaco_av1pf_cdef_blocks", "vaco_av1pf_upscale_plane", "vaco_av1pf_restore_plane",
                      "vaco_av1pf_frame_output_geometry", "vaco_av1pf_frame_input_geometry",
@@ -283,6 +288,45 @@ class Context:
 
         call.keep = (list(frames), descs, pins, pouts)
         return call
+
+    # ---- tuning
+
+    TUNING_FIELDS = ("superres_rows", "cdef_xcd_order", "lr_order")
+
+    @property
+    def tuning(self) -> dict:
+        t = abi.Tuning()
+        self._check(self._lib.vaco_av1pf_get_tuning(self.handle, C.byref(t)))
+        return {k: getattr(t, k) for k in self.TUNING_FIELDS}
+
+    @tuning.setter
+    def tuning(self, values: dict):
+        t = abi.Tuning()
+        self._check(self._lib.vaco_av1pf_get_tuning(self.handle, C.byref(t)))
+        for k, v in values.items():
+            if k not in self.TUNING_FIELDS:
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        st = self._lib.vaco_av1pf_set_tuning(self.handle, C.byref(t))
+        if st != abi.OK:
+            raise Av1Error(st, f"invalid tuning {values}")
+
+    def autotune(self, frames, in_ptrs, out_ptrs, reps: int = 3) -> dict:
+        """Time every launch variant on these frames (device planes) and keep
+        the fastest per kernel; returns the chosen tuning."""
+        n = len(frames)
+        descs = (abi.FrameDesc * n)()
+        pins, pouts = (abi.Picture * n)(), (abi.Picture * n)()
+        for k, f in enumerate(frames):
+            descs[k] = f.desc
+            geo_out = f.output_geometry()
+            for i in range(f.nplanes):
+                h, w = f.planes[i].shape
+                pins[k].plane[i] = abi.device_plane(in_ptrs[k][i], w, h)
+                pouts[k].plane[i] = abi.device_plane(out_ptrs[k][i], geo_out[i][1], geo_out[i][0])
+        t = abi.Tuning()
+        self._check(self._lib.vaco_av1pf_autotune(self.handle, n, descs, pins, pouts, reps, C.byref(t)))
+        return {k: getattr(t, k) for k in self.TUNING_FIELDS}
 
     def batch_device(self, frames, in_ptrs, out_ptrs, stream: int | None = None):
         self.batch_call(frames, in_ptrs, out_ptrs, stream)()

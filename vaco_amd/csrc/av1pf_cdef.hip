@@ -218,25 +218,17 @@ __device__ __forceinline__ s16x2 ldb(const char *a)
 // sample in its place has exactly that effect (constrain(0) = 0, and x is
 // already in [mn, mx]), so BORDER superblocks swap it in with one bit
 // select per tap.
-template <bool BORDER, int N, int STRIDE>
-__device__ __forceinline__ void filter_pairs(const uint16_t *t, int ci, const Blk &B, s16x2 *res)
+// Taps k0 .. k1-1 (both signs) of filter_pairs. FULL = false: the tap
+// direction's strength is zero, so its terms vanish (constrain() returns 0,
+// cdef.rs:127-129) but its samples still bound the clamp range.
+template <bool BORDER, bool FULL, int N, int STRIDE, int K0, int K1>
+__device__ __forceinline__ void apply_taps(const char *a, const Blk &B, const u16x2 *x, u16x2 S,
+                                           u16x2 sh, s16x2 w0, s16x2 w1, s16x2 *sum, u16x2 *mn,
+                                           u16x2 *mx)
 {
-    const char *a = reinterpret_cast<const char *>(t + ci);
-    u16x2 x[N], mn[N], mx[N];
-    s16x2 sum[N];
 #pragma unroll
-    for (int n = 0; n < N; n++) {
-        x[n] = __builtin_bit_cast(u16x2, ldb(a + 2 * STRIDE * n));
-        sum[n] = (s16x2)0;
-        mn[n] = mx[n] = x[n];
-    }
-    const u16x2 P = (u16x2)(unsigned short)B.pri, Q = (u16x2)(unsigned short)B.sec;
-    const u16x2 ps = (u16x2)(unsigned short)(B.shifts & 15), qs = (u16x2)(unsigned short)(B.shifts >> 4);
-    const short t0 = (B.flags & 2) ? 3 : 4;
-    const s16x2 wk[6] = {(s16x2)t0, (s16x2)(short)(6 - t0), (s16x2)2, (s16x2)1, (s16x2)2, (s16x2)1};
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const u16x2 S = k < 2 ? P : Q, sh = k < 2 ? ps : qs;
+    for (int k = K0; k < K1; k++) {
+        const s16x2 w = ((k - K0) & 1) ? w1 : w0;
 #pragma unroll
         for (int s = 0; s < 2; s++) {
             const int o = s ? B.offp[k] : B.offm[k];
@@ -250,17 +242,48 @@ __device__ __forceinline__ void filter_pairs(const uint16_t *t, int ci, const Bl
                                        (~m & __builtin_bit_cast(uint32_t, p));
                     p = __builtin_bit_cast(u16x2, v);
                 }
-                const u16x2 up = __builtin_elementwise_sub_sat(p, x[n]);
-                const u16x2 dn = __builtin_elementwise_sub_sat(x[n], p);
-                const u16x2 T = __builtin_elementwise_sub_sat(S, (up | dn) >> sh);
-                const s16x2 c = __builtin_bit_cast(s16x2, __builtin_elementwise_min(up, T)) -
-                                __builtin_bit_cast(s16x2, __builtin_elementwise_min(dn, T));
-                sum[n] += wk[k] * c;
+                if (FULL) {
+                    const u16x2 up = __builtin_elementwise_sub_sat(p, x[n]);
+                    const u16x2 dn = __builtin_elementwise_sub_sat(x[n], p);
+                    const u16x2 T = __builtin_elementwise_sub_sat(S, (up | dn) >> sh);
+                    const s16x2 c = __builtin_bit_cast(s16x2, __builtin_elementwise_min(up, T)) -
+                                    __builtin_bit_cast(s16x2, __builtin_elementwise_min(dn, T));
+                    sum[n] += w * c;
+                }
                 mn[n] = __builtin_elementwise_min(mn[n], p);
                 mx[n] = __builtin_elementwise_max(mx[n], p);
             }
         }
     }
+}
+
+// pri_on / sec_on: the superblock's preset has a nonzero primary / secondary
+// strength (uniform over the workgroup, so the branches are scalar).
+template <bool BORDER, int N, int STRIDE>
+__device__ __forceinline__ void filter_pairs(const uint16_t *t, int ci, const Blk &B, s16x2 *res,
+                                             bool pri_on, bool sec_on)
+{
+    const char *a = reinterpret_cast<const char *>(t + ci);
+    u16x2 x[N], mn[N], mx[N];
+    s16x2 sum[N];
+#pragma unroll
+    for (int n = 0; n < N; n++) {
+        x[n] = __builtin_bit_cast(u16x2, ldb(a + 2 * STRIDE * n));
+        sum[n] = (s16x2)0;
+        mn[n] = mx[n] = x[n];
+    }
+    const u16x2 P = (u16x2)(unsigned short)B.pri, Q = (u16x2)(unsigned short)B.sec;
+    const u16x2 ps = (u16x2)(unsigned short)(B.shifts & 15), qs = (u16x2)(unsigned short)(B.shifts >> 4);
+    const short t0 = (B.flags & 2) ? 3 : 4;
+    const s16x2 w0 = (s16x2)t0, w1 = (s16x2)(short)(6 - t0);
+    if (pri_on)
+        apply_taps<BORDER, true, N, STRIDE, 0, 2>(a, B, x, P, ps, w0, w1, sum, mn, mx);
+    else
+        apply_taps<BORDER, false, N, STRIDE, 0, 2>(a, B, x, P, ps, w0, w1, sum, mn, mx);
+    if (sec_on)
+        apply_taps<BORDER, true, N, STRIDE, 2, 6>(a, B, x, Q, qs, (s16x2)2, (s16x2)1, sum, mn, mx);
+    else
+        apply_taps<BORDER, false, N, STRIDE, 2, 6>(a, B, x, Q, qs, (s16x2)2, (s16x2)1, sum, mn, mx);
 #pragma unroll
     for (int n = 0; n < N; n++) {
         const s16x2 xs = __builtin_bit_cast(s16x2, x[n]);
@@ -330,8 +353,12 @@ __device__ __forceinline__ bool stage_vec(uint16_t *dst, const uint16_t *src, in
 template <bool BORDER, int SX, int SY>
 __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*lt)[kTile],
                                           const Blk (*blk)[64], const uint8_t *bact, bool any,
-                                          int x0, int y0)
+                                          int x0, int y0, int ix)
 {
+    // Strengths are per superblock preset (the luma primary is only ever
+    // lowered per block, never raised from zero).
+    const bool ypri = f.y_pri[ix] != 0, ysec = f.y_sec[ix] != 0;
+    const bool cpri = f.uv_pri[ix] != 0, csec = f.uv_sec[ix] != 0;
     constexpr int BW = 8 >> SX, BH = 8 >> SY, PPP = BW * BH / 2, CPX = BW * BH / 16;
     const int tid = threadIdx.x;
     const int b = tid >> 4, q = tid & 15;
@@ -348,7 +375,7 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
         const int ci0 = (2 + bi * 8 + i0) * kRow + 8 + bj * 8 + j;
         s16x2 r2[2];
         if (act) {
-            filter_pairs<BORDER, 2, 4 * kRow>(t, ci0, B, r2);
+            filter_pairs<BORDER, 2, 4 * kRow>(t, ci0, B, r2, ypri, ysec);
         } else {
             r2[0] = ld2(t, ci0);
             r2[1] = ld2(t, ci0 + 4 * kRow);
@@ -380,7 +407,7 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
             const uint16_t *t = lt[p];
             s16x2 res;
             if (act)
-                filter_pairs<BORDER, 1, 0>(t, ci, B, &res);
+                filter_pairs<BORDER, 1, 0>(t, ci, B, &res, cpri, csec);
             else
                 res = ld2(t, ci);
             if (!BORDER || (x < W1 && y < H1)) {
@@ -394,7 +421,8 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
 
 // One 64x64 luma superblock of one frame, all planes (decode.rs:772-856).
 // SX, SY: chroma subsampling (monochrome frames use <0, 0> with 1 plane).
-template <int SX, int SY>
+// XCD: superblocks numbered XCD-contiguously (xcd_remap).
+template <int SX, int SY, bool XCD>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cdef(const CdefFrame *__restrict__ frames, int n_frames)
 {
     constexpr int CW = 64 >> SX, CH = 64 >> SY, BW = 8 >> SX, BH = 8 >> SY;
@@ -404,7 +432,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     __shared__ uint32_t cost[8][64];
     __shared__ uint8_t bact[64];
 
-    const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t wg = XCD ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const CdefFrame &f = find_item(frames, n_frames, wg);
     const uint32_t sb = wg - f.block_begin;
     const int sby = (int)(sb / f.sb_cols), sbx = (int)(sb - (uint32_t)sby * f.sb_cols);
@@ -499,9 +527,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
 
     const bool any = idx >= 0;
     if (border)
-        filter_sb<true, SX, SY>(f, lt, blk, bact, any, x0, y0);
+        filter_sb<true, SX, SY>(f, lt, blk, bact, any, x0, y0, idx & 7);
     else
-        filter_sb<false, SX, SY>(f, lt, blk, bact, any, x0, y0);
+        filter_sb<false, SX, SY>(f, lt, blk, bact, any, x0, y0, idx & 7);
 }
 
 // Kernel-level cases (cdef::find_direction + cdef::filter_block on one
@@ -562,20 +590,31 @@ __global__ __launch_bounds__(64) void k_cdef_cases(const uint16_t *__restrict__ 
     outs[(size_t)cs * 64 + lane] = res;
 }
 
-// mode: 0 = 4:2:0, 1 = 4:4:4 or monochrome, 2 = 4:2:2, 3 = 4:4:0. ev0/ev1
-// (may be null) receive the dispatch's start and end.
-hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, hipStream_t s,
-                       hipEvent_t ev0, hipEvent_t ev1)
+template <bool XCD>
+static void launch_cdef_mode(const CdefFrame *frames, int n, uint32_t blocks, int mode,
+                             hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+{
+    const dim3 g(blocks), b(kThreads);
+    switch (mode) {
+    case 0: hipExtLaunchKernelGGL((k_cdef<1, 1, XCD>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    case 1: hipExtLaunchKernelGGL((k_cdef<0, 0, XCD>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    case 2: hipExtLaunchKernelGGL((k_cdef<1, 0, XCD>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    default: hipExtLaunchKernelGGL((k_cdef<0, 1, XCD>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    }
+}
+
+// mode: 0 = 4:2:0, 1 = 4:4:4 or monochrome, 2 = 4:2:2, 3 = 4:4:0. xcd:
+// XCD-contiguous superblock order. ev0/ev1 (may be null) receive the
+// dispatch's start and end.
+hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, bool xcd,
+                       hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (blocks == 0)
         return hipSuccess;
-    const dim3 g(blocks), b(kThreads);
-    switch (mode) {
-    case 0: hipExtLaunchKernelGGL((k_cdef<1, 1>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
-    case 1: hipExtLaunchKernelGGL((k_cdef<0, 0>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
-    case 2: hipExtLaunchKernelGGL((k_cdef<1, 0>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
-    default: hipExtLaunchKernelGGL((k_cdef<0, 1>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
-    }
+    if (xcd)
+        launch_cdef_mode<true>(frames, n, blocks, mode, s, ev0, ev1);
+    else
+        launch_cdef_mode<false>(frames, n, blocks, mode, s, ev0, ev1);
     return hipGetLastError();
 }
 

@@ -49,14 +49,13 @@ struct CdefFrame {
 // ------------------------------------------------------------- superres --
 
 // One plane of one image: superres.rs:163-216. Workgroups cover blocks of
-// kSrRows rows x segments of kSrSeg output samples.
+// R rows (a launch parameter) x segments of kSrSeg output samples.
 constexpr int kSrSeg = 1024;
-constexpr int kSrRows = 8;
 
 // Workgroups of a superres job of n_rows rows and segs segments.
-inline uint32_t sr_job_blocks(uint32_t n_rows, uint32_t segs)
+inline uint32_t sr_job_blocks(uint32_t n_rows, uint32_t segs, uint32_t rows_per_block)
 {
-    return (n_rows + kSrRows - 1) / kSrRows * segs;
+    return (n_rows + rows_per_block - 1) / rows_per_block * segs;
 }
 
 struct SrJob {

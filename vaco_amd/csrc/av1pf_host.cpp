@@ -27,11 +27,11 @@
 #include "av1pf_internal.h"
 
 namespace av1pf {
-hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, hipStream_t s,
-                       hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s,
+hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, bool xcd,
+                       hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, int rows, hipStream_t s,
                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-hipError_t launch_lr(const LrPlane *planes, const uint32_t *list, const uint32_t counts[3],
+hipError_t launch_lr(const LrPlane *planes, const uint32_t *list, uint32_t n, bool xcd,
                      hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s,
                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
@@ -104,6 +104,7 @@ struct vaco_av1pf_ctx {
         uint32_t a, b;
         uint64_t bytes;
     };
+    vaco_av1pf_tuning tune = {8, 1, 0, 0};  // launch variants (vaco_av1pf_autotune)
     bool prof = false;
     std::vector<hipEvent_t> prof_pool;
     uint32_t prof_next = 0;
@@ -561,6 +562,7 @@ struct Plan {
         uint32_t w, h;
     };
     std::vector<Copy> copies;  // frames in which no stage runs
+    vaco_av1pf_tuning tune = {8, 1, 0, 0};  // launch variants this plan is built for
     // algorithmic bytes per kernel id (samples read once + written once, 2 B)
     uint64_t bytes[VACO_AV1PF_KERNEL_COUNT] = {};
     uint64_t cdef_bytes[4] = {};  // CDEF by chroma layout
@@ -699,7 +701,7 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
                 j.start = g.sr_start[p];
                 j.block_begin = plan.sr_blocks;
                 j.max_sample = (uint16_t)((1u << d.bit_depth) - 1);
-                plan.sr_blocks += sr_job_blocks(j.n_rows, j.segs);
+                plan.sr_blocks += sr_job_blocks(j.n_rows, j.segs, plan.tune.superres_rows);
                 plan.bytes[VACO_AV1PF_KERNEL_SUPERRES] +=
                     (uint64_t)j.n_rows * (g.sr_vis[p] + g.sr_out_w[p]) * 2;
                 plan.sr.push_back(j);
@@ -771,7 +773,9 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
                             d.lr_units[p][(*rows)[ri].unit_row * L.unit_cols + (*cols)[ci].unit_col].kind;
                         kind = k == VACO_AV1PF_UNIT_WIENER ? 1 : k == VACO_AV1PF_UNIT_SELF_GUIDED ? 2 : 0;
                     }
-                    plan.lr_tiles[kind].push_back(item << 22 | (ri * L.n_cols + ci));
+                    // plane order (one list) or by kind (three lists)
+                    plan.lr_tiles[plan.tune.lr_order ? 0 : kind].push_back(item << 22 |
+                                                                           (ri * L.n_cols + ci));
                     plan.bytes[VACO_AV1PF_KERNEL_RESTORATION] +=
                         (uint64_t)(*rows)[ri].h * (*cols)[ci].w * 4;
                 }
@@ -828,19 +832,21 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
             HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, plan.cdef_bytes[m], s,
                                   [&](hipEvent_t e0, hipEvent_t e1) {
                                       return launch_cdef(dc[m], (int)plan.cdef[m].size(),
-                                                         plan.cdef_blocks[m], m, s, e0, e1);
+                                                         plan.cdef_blocks[m], m,
+                                                         plan.tune.cdef_xcd_order != 0, s, e0, e1);
                                   }));
     if (dj)
         HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES, plan.bytes[VACO_AV1PF_KERNEL_SUPERRES],
                               s, [&](hipEvent_t e0, hipEvent_t e1) {
                                   return launch_superres(dj, (int)plan.sr.size(), plan.sr_blocks,
-                                                         s, e0, e1);
+                                                         plan.tune.superres_rows, s, e0, e1);
                               }));
     if (dl)
         HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION,
                               plan.bytes[VACO_AV1PF_KERNEL_RESTORATION], s,
                               [&](hipEvent_t e0, hipEvent_t e1) {
-                                  return launch_lr(dl, dlist, nlist, s, e0, e1);
+                                  return launch_lr(dl, dlist, nlist[0] + nlist[1] + nlist[2],
+                                                   plan.tune.lr_order != 0, s, e0, e1);
                               }));
     return VACO_AV1PF_OK;
 }
@@ -857,6 +863,7 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
     ws.base = ctx->img[1][0];
     ws.cap = ctx->plane_cap * 9;
     Plan plan;
+    plan.tune = ctx->tune;
     vaco_av1pf_status st = plan_frame(ctx, d, g, in, out, a, status, ws, plan);
     if (st != VACO_AV1PF_OK)
         return st;
@@ -1255,6 +1262,7 @@ vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
     ws.base = ctx->ws;
     ws.cap = ctx->ws_cap;
     Plan plan;
+    plan.tune = ctx->tune;
     for (uint32_t i = 0; i < n; i++) {
         DevPlane din[3], dout[3];
         for (int p = 0; p < geo[i].nplanes; p++) {
@@ -1302,6 +1310,109 @@ extern "C" vaco_av1pf_status vaco_av1pf_finish(vaco_av1pf_ctx *ctx)
     ctx->deferred = VACO_AV1PF_OK;
     ctx->deferred_msg = "";
     return r;
+}
+
+// ------------------------------------------------------------ tuning --
+
+static bool tuning_ok(const vaco_av1pf_tuning &t)
+{
+    return (t.superres_rows == 4 || t.superres_rows == 8 || t.superres_rows == 16) &&
+           t.cdef_xcd_order <= 1 && t.lr_order <= 1 && t.reserved == 0;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_get_tuning(const vaco_av1pf_ctx *ctx, vaco_av1pf_tuning *out)
+{
+    if (!ctx || !out)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    *out = ctx->tune;
+    return VACO_AV1PF_OK;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_set_tuning(vaco_av1pf_ctx *ctx, const vaco_av1pf_tuning *t)
+{
+    if (!ctx || !t || !tuning_ok(*t))
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    ctx->tune = *t;
+    return VACO_AV1PF_OK;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n,
+                                                 const vaco_av1pf_frame_desc *descs,
+                                                 const vaco_av1pf_picture *ins,
+                                                 vaco_av1pf_picture *outs, uint32_t reps,
+                                                 vaco_av1pf_tuning *out)
+{
+    if (!ctx || !out || reps == 0 || n == 0)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    vaco_av1pf_status st = vaco_av1pf_finish(ctx);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    const bool was_profiling = ctx->prof;
+    if ((st = vaco_av1pf_profile(ctx, 1)) != VACO_AV1PF_OK)
+        return st;
+    double ms[VACO_AV1PF_KERNEL_COUNT];
+    uint64_t launches[VACO_AV1PF_KERNEL_COUNT];
+    // Mean time of `kernel` per batch with tuning t (one untimed batch first).
+    auto measure = [&](const vaco_av1pf_tuning &t, int kernel, double *res) {
+        ctx->tune = t;
+        vaco_av1pf_status r = av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr);
+        if (r == VACO_AV1PF_OK)
+            r = vaco_av1pf_finish(ctx);
+        if (r == VACO_AV1PF_OK)
+            r = vaco_av1pf_profile_read(ctx, nullptr, nullptr, nullptr);
+        for (uint32_t i = 0; i < reps && r == VACO_AV1PF_OK; i++)
+            r = av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr);
+        if (r == VACO_AV1PF_OK)
+            r = vaco_av1pf_finish(ctx);
+        if (r == VACO_AV1PF_OK)
+            r = vaco_av1pf_profile_read(ctx, ms, launches, nullptr);
+        *res = launches[kernel] ? ms[kernel] / reps : 0.0;
+        return r;
+    };
+    vaco_av1pf_tuning best = ctx->tune, t;
+    double tm, best_ms;
+    // superres rows per workgroup
+    best_ms = -1;
+    for (uint8_t rows : {4, 8, 16}) {
+        t = best;
+        t.superres_rows = rows;
+        if ((st = measure(t, VACO_AV1PF_KERNEL_SUPERRES, &tm)) != VACO_AV1PF_OK)
+            break;
+        if (tm > 0 && (best_ms < 0 || tm < best_ms)) {  // kernels absent from the batch time 0
+            best_ms = tm;
+            best.superres_rows = rows;
+        }
+    }
+    // CDEF superblock order
+    best_ms = -1;
+    for (uint8_t v = 0; v < 2 && st == VACO_AV1PF_OK; v++) {
+        t = best;
+        t.cdef_xcd_order = v;
+        if ((st = measure(t, VACO_AV1PF_KERNEL_CDEF, &tm)) != VACO_AV1PF_OK)
+            break;
+        if (tm > 0 && (best_ms < 0 || tm < best_ms)) {  // kernels absent from the batch time 0
+            best_ms = tm;
+            best.cdef_xcd_order = v;
+        }
+    }
+    // restoration tile order
+    best_ms = -1;
+    for (uint8_t v = 0; v < 2 && st == VACO_AV1PF_OK; v++) {
+        t = best;
+        t.lr_order = v;
+        if ((st = measure(t, VACO_AV1PF_KERNEL_RESTORATION, &tm)) != VACO_AV1PF_OK)
+            break;
+        if (tm > 0 && (best_ms < 0 || tm < best_ms)) {  // kernels absent from the batch time 0
+            best_ms = tm;
+            best.lr_order = v;
+        }
+    }
+    ctx->tune = best;
+    const vaco_av1pf_status ps = vaco_av1pf_profile(ctx, was_profiling ? 1 : 0);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    *out = best;
+    return ps;
 }
 
 // ------------------------------------------------- kernel-level entries --
@@ -1399,7 +1510,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_upscale_plane(vaco_av1pf_ctx *ctx, const
     if (e == hipSuccess)
         e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-        e = launch_superres(dj, 1, sr_job_blocks(j.n_rows, j.segs), ctx->stream);
+        e = launch_superres(dj, 1, sr_job_blocks(j.n_rows, j.segs, 8), 8, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpy2DAsync(out->data, out->stride * 2, dout, cfg->output_width * 2,
                              cfg->output_width * 2, cfg->height, hipMemcpyDeviceToHost, ctx->stream);
@@ -1517,7 +1628,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     if (e == hipSuccess)
         e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-        e = launch_lr(dl, dlist, nlist, ctx->stream);
+        e = launch_lr(dl, dlist, nlist[0] + nlist[1] + nlist[2], false, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpy2DAsync(out->data, out->stride * 2, dout, cfg->width * 2, cfg->width * 2,
                              cfg->height, hipMemcpyDeviceToHost, ctx->stream);

@@ -7,10 +7,12 @@
 //
 // A tile is at most 64x64 output samples inside one restoration stripe and
 // one restoration unit, so its filter type is uniform. The host sorts tiles
-// by type and one launch of k_lr (512 threads) covers all of them: the
-// workgroups first take the self-guided tiles, then the Wiener tiles, then
-// the copies, so the longest tiles start first and the three kinds share
-// the machine. Each kind is a branch-free path of its own:
+// by type and one launch of k_lr (512 threads) covers all of them; each
+// workgroup branches once on its tile's unit type. The host orders the list
+// (tuning): either self-guided tiles, then Wiener, then copies, so the
+// longest tiles start first, or plane order numbered XCD-contiguously, so
+// neighbouring tiles share halo lines in one L2. Each kind is a branch-free
+// path of its own:
 //   copy        RestorationUnit::None and planes without restoration
 //   Wiener      ~19 KB LDS: staged source, 16-bit row pass, column pass
 //   self-guided ~39 KB LDS: staged source, column sums of x and x^2, box
@@ -41,9 +43,9 @@ struct Tile {
     LrUnit u;
 };
 
-__device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, const uint32_t *list)
+__device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, const uint32_t *list, uint32_t wg)
 {
-    const uint32_t e = list[blockIdx.x];
+    const uint32_t e = list[wg];
     Tile t;
     t.P = &planes[e >> 22];
     const uint32_t local = e & 0x3fffffu;
@@ -353,32 +355,34 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
     }
 }
 
-// One launch over every tile of the plan: list holds the self-guided tiles,
-// then the Wiener tiles, then the copies (n_sgr + n_wiener + n_copy
-// entries, one per workgroup).
+// One launch over every tile of the plan, one workgroup per list entry.
+// XCD: workgroup b takes entry xcd_remap(b) (for lists in plane order).
+template <bool XCD>
 __global__ __launch_bounds__(kLrThreads) void k_lr(const LrPlane *__restrict__ planes,
-                                                   const uint32_t *__restrict__ list, uint32_t n_sgr,
-                                                   uint32_t n_wiener)
+                                                   const uint32_t *__restrict__ list)
 {
     __shared__ __align__(16) uint8_t lds[kSgrLds > kWienerLds ? kSgrLds : kWienerLds];
-    const Tile t = fetch_tile(planes, list);
-    if (blockIdx.x < n_sgr)
+    const uint32_t wg = XCD ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const Tile t = fetch_tile(planes, list, wg);
+    if (t.u.kind == kUnitSelfGuided)
         lr_sgr(t, lds);
-    else if (blockIdx.x < n_sgr + n_wiener)
+    else if (t.u.kind == kUnitWiener)
         lr_wiener(t, lds);
     else
         lr_copy(t);
 }
 
-// ev0/ev1 (may be null) receive the dispatch's start and end.
-hipError_t launch_lr(const LrPlane *planes, const uint32_t *list, const uint32_t counts[3],
+// n tiles; xcd: list in plane order, numbered XCD-contiguously. ev0/ev1
+// (may be null) receive the dispatch's start and end.
+hipError_t launch_lr(const LrPlane *planes, const uint32_t *list, uint32_t n, bool xcd,
                      hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
-    const uint32_t n = counts[0] + counts[1] + counts[2];
     if (n == 0)
         return hipSuccess;
-    hipExtLaunchKernelGGL(k_lr, dim3(n), dim3(kLrThreads), 0, s, ev0, ev1, 0, planes, list,
-                          counts[2], counts[1]);
+    if (xcd)
+        hipExtLaunchKernelGGL(k_lr<true>, dim3(n), dim3(kLrThreads), 0, s, ev0, ev1, 0, planes, list);
+    else
+        hipExtLaunchKernelGGL(k_lr<false>, dim3(n), dim3(kLrThreads), 0, s, ev0, ev1, 0, planes, list);
     return hipGetLastError();
 }
 

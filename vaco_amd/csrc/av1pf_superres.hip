@@ -48,7 +48,7 @@ constexpr int kSrThreads = 256;
 constexpr int kSrPer = kSrSeg / kSrThreads;  // output columns per thread
 constexpr int kSrWin = kSrSeg + 16;          // staged source samples per row
 
-// One segment of kSrSeg output columns over up to kSrRows rows
+// One segment of kSrSeg output columns over up to R rows
 // (superres.rs:173-214). Positions, phases and filters depend on the column
 // only, so each thread derives them once for its columns and applies them to
 // every row. Each row's source window (the union of the segment's taps,
@@ -56,16 +56,17 @@ constexpr int kSrWin = kSrSeg + 16;          // staged source samples per row
 // below one source sample the window is exactly the samples the taps read,
 // so the sample-range flag is raised exactly when the reference would reject
 // a tap.
+template <int R>
 __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict__ jobs, int n_jobs)
 {
     __shared__ __align__(16) int16_t tab[64 * 8];
-    __shared__ uint16_t win[kSrRows][kSrWin];
+    __shared__ uint16_t win[R][kSrWin];
     const SrJob &j = find_item(jobs, n_jobs, blockIdx.x);
     const uint32_t local = blockIdx.x - j.block_begin;
     const uint32_t rb = local / j.segs;
     const uint32_t seg = local - rb * j.segs;
-    const int r0 = (int)rb * kSrRows;
-    const int nr = min(kSrRows, (int)j.n_rows - r0);
+    const int r0 = (int)rb * R;
+    const int nr = min(R, (int)j.n_rows - r0);
     const int tid = threadIdx.x;
     const int x0 = (int)seg * kSrSeg;
     const int xn = min(x0 + kSrSeg, (int)j.out_width) - 1;
@@ -110,13 +111,20 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
     }
 }
 
-// ev0/ev1 (may be null) receive the dispatch's start and end.
-hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, hipStream_t s, hipEvent_t ev0,
-                           hipEvent_t ev1)
+// rows: rows per workgroup (4, 8 or 16; the jobs' block_begin counts use
+// the same value). ev0/ev1 (may be null) receive the dispatch's start and end.
+hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, int rows, hipStream_t s,
+                           hipEvent_t ev0, hipEvent_t ev1)
 {
     if (blocks == 0)
         return hipSuccess;
-    hipExtLaunchKernelGGL(k_superres, dim3(blocks), dim3(kSrThreads), 0, s, ev0, ev1, 0, jobs, n);
+    const dim3 g(blocks), b(kSrThreads);
+    if (rows == 4)
+        hipExtLaunchKernelGGL(k_superres<4>, g, b, 0, s, ev0, ev1, 0, jobs, n);
+    else if (rows == 16)
+        hipExtLaunchKernelGGL(k_superres<16>, g, b, 0, s, ev0, ev1, 0, jobs, n);
+    else
+        hipExtLaunchKernelGGL(k_superres<8>, g, b, 0, s, ev0, ev1, 0, jobs, n);
     return hipGetLastError();
 }
 
