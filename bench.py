@@ -255,14 +255,34 @@ def measure_per_frame(av1pf, hip, cfg, n_frames, steps, warmup):
     t1 = time.perf_counter()
 
     # Host cost of one call with the staging ring empty (no waits on the GPU).
-    host_us = []
-    for rep in range(8):
-        for i in range(6):
-            t = time.perf_counter()
-            calls[i % len(calls)]()
-            host_us.append((time.perf_counter() - t) * 1e6)
-        ctx.finish()
-    host_us.sort()
+    def host_cost():
+        us = []
+        for rep in range(8):
+            for i in range(6):
+                t = time.perf_counter()
+                calls[i % len(calls)]()
+                us.append((time.perf_counter() - t) * 1e6)
+            ctx.finish()
+        return sorted(us)[len(us) // 2]
+
+    host_us = host_cost()
+
+    # The same steps with each call's kernel chain replayed as a hipGraph.
+    ctx.use_graphs(True)
+    for i in range(warmup):
+        calls[i % len(calls)]()
+    ctx.finish()
+    hip.synchronize()
+    g0 = time.perf_counter()
+    for i in range(steps):
+        calls[i % len(calls)]()
+    ctx.finish()
+    hip.synchronize()
+    g1 = time.perf_counter()
+    graphs = {"mpix_s": round(frames[0].output_luma_pixels / 1e6 * steps / (g1 - g0), 2),
+              "ms_per_step": round((g1 - g0) * 1e3 / steps, 4),
+              "host_call_us_median": round(host_cost(), 2)}
+    ctx.use_graphs(False)
 
     # Kernel durations from HIP events on the library's stream (a separate
     # pass over the same steps, so event records do not perturb the timing).
@@ -278,7 +298,8 @@ def measure_per_frame(av1pf, hip, cfg, n_frames, steps, warmup):
         "mpix_s": round(mpix * steps / (t1 - t0), 2),
         "ms_per_step": round((t1 - t0) * 1e3 / steps, 4),
         "host_enqueue_ms_per_step": round((t_enq - t0) * 1e3 / steps, 4),
-        "host_call_us_median": round(host_us[len(host_us) // 2], 2),
+        "host_call_us_median": round(host_us, 2),
+        "graph_replay": graphs,
         "kernels": kernel_table(prof, steps),
         "tuning": tuning,
         "_frames": frames,
@@ -450,6 +471,8 @@ def main():
             "parallelism": "single GPU",
         },
         "host_enqueue_ms_per_step": per_frame["host_enqueue_ms_per_step"],
+        "host_call_us_median": per_frame["host_call_us_median"],
+        "graph_replay": per_frame["graph_replay"],
         "tuning": per_frame["tuning"],
         "kernels": kern,
         "roofline": {

@@ -241,6 +241,15 @@ vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32_t n,
                                           const vaco_av1pf_picture *ins,
                                           vaco_av1pf_picture *outs, void *stream);
 
+/* ------------------------------------------------------------ graphs --- */
+
+/* No reference counterpart. With graphs enabled, the kernel chain of a
+ * frame-device or batched call is captured once per launch shape (frame
+ * geometry and staging slot) as a hipGraph and replayed by later calls of
+ * the same shape: one graph launch instead of a launch per kernel. Output
+ * is identical either way. Not used while profiling. */
+vaco_av1pf_status vaco_av1pf_use_graphs(vaco_av1pf_ctx *ctx, int enable);
+
 /* ------------------------------------------------------------ tuning --- */
 
 /* No reference counterpart: the launch variants an autotuner chooses

@@ -113,6 +113,7 @@ def lib():
                                               C.c_void_p]
         L.vaco_av1pf_profile.argtypes = [ctxp, C.c_int]
         L.vaco_av1pf_profile_read.argtypes = [ctxp, P(C.c_double), P(C.c_uint64), P(C.c_uint64)]
+        L.vaco_av1pf_use_graphs.argtypes = [ctxp, C.c_int]
         L.vaco_av1pf_get_tuning.argtypes = [ctxp, P(abi.Tuning)]
         L.vaco_av1pf_set_tuning.argtypes = [ctxp, P(abi.Tuning)]
         L.vaco_av1pf_autotune.argtypes = [ctxp, C.c_uint32, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture),
@@ -128,7 +129,7 @@ def lib():
         L.vaco_av1pf_now_ns.restype = C.c_uint64
         L.vaco_av1pf_server_frame_bytes.argtypes = [P(abi.FrameDesc)]
         L.vaco_av1pf_server_frame_bytes.restype = C.c_uint64
-        for name in ("vaco_av1pf_get_tuning", "vaco_av1pf_set_tuning", "vaco_av1pf_autotune",
+        for name in ("vaco_av1pf_use_graphs", "vaco_av1pf_get_tuning", "vaco_av1pf_set_tuning", "vaco_av1pf_autotune",
                      "vaco_av1pf_server_create", "vaco_av1pf_server_submit", "vaco_av1pf_server_drain",
                      "vaco_av1pf_batch_device", "vaco_av1pf_profile", "vaco_av1pf_profile_read", "vaco_av1pf_create", "vaco_av1pf_frame", "vaco_av1pf_frame_device", "vaco_av1pf_finish",
                      "vaco_av1pf_cdef_blocks", "vaco_av1pf_upscale_plane", "vaco_av1pf_restore_plane",
@@ -288,6 +289,10 @@ class Context:
 
         call.keep = (list(frames), descs, pins, pouts)
         return call
+
+    def use_graphs(self, enable: bool = True):
+        """Replay each call's kernel chain as a hipGraph per launch shape."""
+        self._check(self._lib.vaco_av1pf_use_graphs(self.handle, 1 if enable else 0))
 
     # ---- tuning
 

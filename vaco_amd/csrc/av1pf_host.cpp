@@ -64,6 +64,14 @@ struct Arena {
     // frames whose status must be checked before reuse
     std::vector<vaco_av1pf_frame_desc> descs;
     std::vector<vaco_av1pf_picture> ins;
+    // Instantiated kernel chains of calls that used this slot, keyed by
+    // their launch parameters (vaco_av1pf_use_graphs). Only this slot's calls
+    // launch them, so an entry is idle whenever the slot is taken.
+    struct Graph {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
 
     template <class T>
     T *take(size_t n, T **dev)
@@ -105,6 +113,7 @@ struct vaco_av1pf_ctx {
         uint64_t bytes;
     };
     vaco_av1pf_tuning tune = {8, 1, 0, 0};  // launch variants (vaco_av1pf_autotune)
+    bool use_graphs = false;                // replay kernel chains as hipGraphs
     bool prof = false;
     std::vector<hipEvent_t> prof_pool;
     uint32_t prof_next = 0;
@@ -322,6 +331,8 @@ extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
             hipEventDestroy(ctx->slot[i].done);
         if (ctx->slot[i].up_done)
             hipEventDestroy(ctx->slot[i].up_done);
+        for (auto &gr : ctx->slot[i].graphs)
+            hipGraphExecDestroy(gr.exec);
     }
     if (ctx->h_status)
         hipHostFree(ctx->h_status);
@@ -827,27 +838,73 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
     for (const auto &c : plan.copies)
         HIPCHECK(hipMemcpy2DAsync(c.dst.data, c.dst.stride * 2, c.src.data, c.src.stride * 2, c.w * 2,
                                   c.h, hipMemcpyDeviceToDevice, s));
-    for (int m = 0; m < 4; m++)
-        if (dc[m])
-            HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, plan.cdef_bytes[m], s,
+    const uint32_t n_tiles = nlist[0] + nlist[1] + nlist[2];
+    auto launch_all = [&]() -> vaco_av1pf_status {
+        for (int m = 0; m < 4; m++)
+            if (dc[m])
+                HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, plan.cdef_bytes[m], s,
+                                      [&](hipEvent_t e0, hipEvent_t e1) {
+                                          return launch_cdef(dc[m], (int)plan.cdef[m].size(),
+                                                             plan.cdef_blocks[m], m,
+                                                             plan.tune.cdef_xcd_order != 0, s, e0,
+                                                             e1);
+                                      }));
+        if (dj)
+            HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES,
+                                  plan.bytes[VACO_AV1PF_KERNEL_SUPERRES], s,
                                   [&](hipEvent_t e0, hipEvent_t e1) {
-                                      return launch_cdef(dc[m], (int)plan.cdef[m].size(),
-                                                         plan.cdef_blocks[m], m,
-                                                         plan.tune.cdef_xcd_order != 0, s, e0, e1);
+                                      return launch_superres(dj, (int)plan.sr.size(), plan.sr_blocks,
+                                                             plan.tune.superres_rows, s, e0, e1);
                                   }));
-    if (dj)
-        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES, plan.bytes[VACO_AV1PF_KERNEL_SUPERRES],
-                              s, [&](hipEvent_t e0, hipEvent_t e1) {
-                                  return launch_superres(dj, (int)plan.sr.size(), plan.sr_blocks,
-                                                         plan.tune.superres_rows, s, e0, e1);
-                              }));
-    if (dl)
-        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION,
-                              plan.bytes[VACO_AV1PF_KERNEL_RESTORATION], s,
-                              [&](hipEvent_t e0, hipEvent_t e1) {
-                                  return launch_lr(dl, dlist, nlist[0] + nlist[1] + nlist[2],
-                                                   plan.tune.lr_order != 0, s, e0, e1);
-                              }));
+        if (dl)
+            HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION,
+                                  plan.bytes[VACO_AV1PF_KERNEL_RESTORATION], s,
+                                  [&](hipEvent_t e0, hipEvent_t e1) {
+                                      return launch_lr(dl, dlist, n_tiles, plan.tune.lr_order != 0, s,
+                                                       e0, e1);
+                                  }));
+        return VACO_AV1PF_OK;
+    };
+    if (!ctx->use_graphs || ctx->prof)
+        return launch_all();
+
+    // Graph replay: the chain's launch parameters (descriptor addresses in
+    // this slot, grid sizes, variants) identify its graph. Frames of one
+    // geometry in one slot reuse it; the descriptors it reads are rewritten
+    // by each call's upload.
+    std::vector<uint64_t> key;
+    for (int m = 0; m < 4; m++)
+        key.insert(key.end(), {(uint64_t)(uintptr_t)dc[m], plan.cdef_blocks[m], plan.cdef[m].size()});
+    key.insert(key.end(), {(uint64_t)(uintptr_t)dj, plan.sr_blocks, plan.sr.size(),
+                           (uint64_t)(uintptr_t)dl, (uint64_t)(uintptr_t)dlist, n_tiles,
+                           plan.tune.superres_rows, plan.tune.cdef_xcd_order, plan.tune.lr_order});
+    hipGraphExec_t exec = nullptr;
+    for (auto &gr : a->graphs)
+        if (gr.key == key)
+            exec = gr.exec;
+    if (!exec) {
+        HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const vaco_av1pf_status st2 = launch_all();
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(s, &g);
+        if (st2 != VACO_AV1PF_OK) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return st2;
+        }
+        if (ee != hipSuccess)
+            return hip_fail(ctx, ee);
+        const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess)
+            return hip_fail(ctx, ei);
+        if (a->graphs.size() >= 4) {
+            HIPCHECK(hipGraphExecDestroy(a->graphs.front().exec));
+            a->graphs.erase(a->graphs.begin());
+        }
+        a->graphs.push_back({key, exec});
+    }
+    HIPCHECK(hipGraphLaunch(exec, s));
     return VACO_AV1PF_OK;
 }
 
@@ -1310,6 +1367,16 @@ extern "C" vaco_av1pf_status vaco_av1pf_finish(vaco_av1pf_ctx *ctx)
     ctx->deferred = VACO_AV1PF_OK;
     ctx->deferred_msg = "";
     return r;
+}
+
+// ------------------------------------------------------------ graphs --
+
+extern "C" vaco_av1pf_status vaco_av1pf_use_graphs(vaco_av1pf_ctx *ctx, int enable)
+{
+    if (!ctx)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    ctx->use_graphs = enable != 0;
+    return VACO_AV1PF_OK;
 }
 
 // ------------------------------------------------------------ tuning --

@@ -73,6 +73,10 @@ class DeviceArray:
         assert a.nbytes == self.nbytes
         check(rt().hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), self.nbytes, H2D), "hipMemcpy H2D")
 
+    def zero(self):
+        """Clear the allocation (synchronous copy of zeros from the host)."""
+        self.upload(np.zeros(self.shape, self.dtype))
+
     def download(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
         check(rt().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes, D2H), "hipMemcpy D2H")
