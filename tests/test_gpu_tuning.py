@@ -20,6 +20,9 @@ def say(*a):
     sys.stdout.flush()
 
 
+DEFAULT = {"superres_rows": 8, "cdef_xcd_order": 1, "lr_order": 0, "lr_tile_rows": 64}
+
+
 @pytest.fixture(scope="module")
 def ctx():
     c = av1pf.Context(3840, 2176)
@@ -60,20 +63,21 @@ def check(dev_out, want, label):
 
 def test_gpu_tuning_defaults_and_validation(ctx):
     say("tuning: defaults and rejection")
-    assert ctx.tuning == {"superres_rows": 8, "cdef_xcd_order": 1, "lr_order": 0}
-    for bad in ({"superres_rows": 5}, {"cdef_xcd_order": 2}, {"lr_order": 3}):
+    assert ctx.tuning == DEFAULT
+    for bad in ({"superres_rows": 5}, {"cdef_xcd_order": 2}, {"lr_order": 3}, {"lr_tile_rows": 48}):
         with pytest.raises(av1pf.Av1Error) as e:
             ctx.tuning = bad
         assert e.value.status == abi.INVALID_ARGUMENT
     assert ctx.tuning["superres_rows"] == 8
 
 
-@pytest.mark.parametrize("rows,cdef_xcd,lr_order", list(itertools.product((4, 8, 16), (0, 1), (0, 1))))
-def test_gpu_every_launch_variant_matches_oracle(ctx, batch, rows, cdef_xcd, lr_order):
+@pytest.mark.parametrize("rows,cdef_xcd,lr_order,lr_rows",
+                         list(itertools.product((4, 8, 16), (0, 1), (0, 1), (64, 32, 16))))
+def test_gpu_every_launch_variant_matches_oracle(ctx, batch, rows, cdef_xcd, lr_order, lr_rows):
     frames, want, dev_in, dev_out = batch
-    label = f"rows={rows} cdef_xcd={cdef_xcd} lr_order={lr_order}"
+    label = f"rows={rows} cdef_xcd={cdef_xcd} lr_order={lr_order} lr_rows={lr_rows}"
     say("variant", label)
-    ctx.tuning = {"superres_rows": rows, "cdef_xcd_order": cdef_xcd, "lr_order": lr_order}
+    ctx.tuning = {"superres_rows": rows, "cdef_xcd_order": cdef_xcd, "lr_order": lr_order, "lr_tile_rows": lr_rows}
     try:
         ctx.batch_device(frames, ptrs(dev_in), ptrs(dev_out))
         ctx.finish()
@@ -83,7 +87,7 @@ def test_gpu_every_launch_variant_matches_oracle(ctx, batch, rows, cdef_xcd, lr_
         ctx.finish()
         check(dev_out[:3], want[:3], "per-frame " + label)
     finally:
-        ctx.tuning = {"superres_rows": 8, "cdef_xcd_order": 1, "lr_order": 0}
+        ctx.tuning = DEFAULT
 
 
 def test_gpu_autotune_returns_a_valid_choice_and_exact_outputs(ctx, batch):
@@ -91,9 +95,10 @@ def test_gpu_autotune_returns_a_valid_choice_and_exact_outputs(ctx, batch):
     say("autotune")
     t = ctx.autotune(frames, ptrs(dev_in), ptrs(dev_out), reps=2)
     assert t["superres_rows"] in (4, 8, 16) and t["cdef_xcd_order"] in (0, 1) and t["lr_order"] in (0, 1)
+    assert t["lr_tile_rows"] in (64, 32, 16)
     assert ctx.tuning == t
     check(dev_out, want, "after autotune")
     ctx.batch_device(frames, ptrs(dev_in), ptrs(dev_out))
     ctx.finish()
     check(dev_out, want, f"tuned {t}")
-    ctx.tuning = {"superres_rows": 8, "cdef_xcd_order": 1, "lr_order": 0}
+    ctx.tuning = DEFAULT

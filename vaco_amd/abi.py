@@ -130,7 +130,7 @@ class Tuning(C.Structure):
         ("superres_rows", C.c_uint8),
         ("cdef_xcd_order", C.c_uint8),
         ("lr_order", C.c_uint8),
-        ("reserved", C.c_uint8),
+        ("lr_tile_rows", C.c_uint8),
     ]
 
 

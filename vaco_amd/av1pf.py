@@ -296,7 +296,7 @@ class Context:
 
     # ---- tuning
 
-    TUNING_FIELDS = ("superres_rows", "cdef_xcd_order", "lr_order")
+    TUNING_FIELDS = ("superres_rows", "cdef_xcd_order", "lr_order", "lr_tile_rows")
 
     @property
     def tuning(self) -> dict:

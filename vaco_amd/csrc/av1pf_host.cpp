@@ -112,7 +112,7 @@ struct vaco_av1pf_ctx {
         uint32_t a, b;
         uint64_t bytes;
     };
-    vaco_av1pf_tuning tune = {8, 1, 0, 0};  // launch variants (vaco_av1pf_autotune)
+    vaco_av1pf_tuning tune = {8, 1, 0, 64};  // launch variants (vaco_av1pf_autotune)
     bool use_graphs = false;                // replay kernel chains as hipGraphs
     bool prof = false;
     std::vector<hipEvent_t> prof_pool;
@@ -148,6 +148,7 @@ static vaco_av1pf_status hip_fail(vaco_av1pf_ctx *ctx, hipError_t e)
     } while (0)
 
 static vaco_av1pf_status resolve_slot(vaco_av1pf_ctx *ctx, Arena &a);
+static size_t arena_need(const vaco_av1pf_frame_desc &d, const FrameGeo &g);
 
 // (Re)allocate a slot's staging buffers at `bytes` (a multiple of 256). The
 // host side is coherent pinned memory that k_upload reads directly.
@@ -573,7 +574,7 @@ struct Plan {
         uint32_t w, h;
     };
     std::vector<Copy> copies;  // frames in which no stage runs
-    vaco_av1pf_tuning tune = {8, 1, 0, 0};  // launch variants this plan is built for
+    vaco_av1pf_tuning tune = {8, 1, 0, 64};  // launch variants this plan is built for
     // algorithmic bytes per kernel id (samples read once + written once, 2 B)
     uint64_t bytes[VACO_AV1PF_KERNEL_COUNT] = {};
     uint64_t cdef_bytes[4] = {};  // CDEF by chroma layout
@@ -732,7 +733,17 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             const bool none = d.lr_type[p] == VACO_AV1PF_RESTORE_NONE;
             std::vector<LrRowSeg> rows_none;
             std::vector<LrColSeg> cols_none;
-            const std::vector<LrRowSeg> *rows = &lr_rows[p];
+            // Tiles: row segments cut to the tuned height (the stripe and unit
+            // of each piece are its segment's).
+            std::vector<LrRowSeg> rows_cut;
+            for (const LrRowSeg &r : lr_rows[p])
+                for (int y = 0; y < r.h; y += plan.tune.lr_tile_rows) {
+                    LrRowSeg c = r;
+                    c.y0 = r.y0 + y;
+                    c.h = std::min(r.h - y, (int)plan.tune.lr_tile_rows);
+                    rows_cut.push_back(c);
+                }
+            const std::vector<LrRowSeg> *rows = &rows_cut;
             const std::vector<LrColSeg> *cols = &lr_cols[p];
             if (none) {
                 // copy_visible_plane (decode.rs:673-681): 64x64 copy tiles
@@ -1070,6 +1081,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame(vaco_av1pf_ctx *ctx, const vaco_av
     Arena *a;
     if ((st = take_slot(ctx, &a)) != VACO_AV1PF_OK)
         return st;
+    if ((st = grow_slot(ctx, a, arena_need(*desc, g))) != VACO_AV1PF_OK)
+        return st;
     DevPlane din[3], dout[3];
     for (int p = 0; p < g.nplanes; p++) {
         din[p] = DevPlane{ctx->img[0][p], g.in_w[p]};
@@ -1136,6 +1149,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
     Arena *a;
     if ((st = take_slot(ctx, &a)) != VACO_AV1PF_OK)
         return st;
+    if ((st = grow_slot(ctx, a, arena_need(*desc, g))) != VACO_AV1PF_OK)
+        return st;
     DevPlane din[3], dout[3];
     for (int p = 0; p < g.nplanes; p++) {
         din[p] = DevPlane{in->plane[p].data, in->plane[p].stride};
@@ -1188,16 +1203,20 @@ static vaco_av1pf_status resolve_slot(vaco_av1pf_ctx *ctx, Arena &a)
 }
 
 // Staging bytes one frame can need (side information, segment lists,
-// descriptors), an upper bound.
+// descriptors), an upper bound. Every TAKE rounds up to 256 bytes.
 static size_t arena_need(const vaco_av1pf_frame_desc &d, const FrameGeo &g)
 {
-    size_t n = 4096;
+    size_t n = 16384;  // descriptors (CdefFrame, SrJob, LrPlane) and rounding
     if (g.cdef)
         n += (size_t)g.idx_rows * d.cdef_stride + (size_t)(g.mi_rows / 2) * (g.mi_cols / 2) + 1024;
     for (int p = 0; p < g.nplanes; p++) {
+        // row pieces: at most one cut per 16 rows plus stripe and unit cuts
+        // (every 32 rows at least); columns: 64-wide pieces plus unit cuts
+        const size_t rows = (size_t)g.lr_h[p] / 8 + 8, cols = (size_t)g.lr_w[p] / 32 + 4;
         n += (size_t)d.lr_unit_count[p] * sizeof(LrUnit) + 512;
-        n += ((size_t)g.lr_h[p] / 16 + 4) * sizeof(LrRowSeg) + ((size_t)g.lr_w[p] / 16 + 4) * sizeof(LrColSeg);
-        n += ((size_t)g.lr_h[p] / 4 + 16) * sizeof(uint32_t);
+        n += rows * sizeof(LrRowSeg) + cols * sizeof(LrColSeg) + 512;
+        n += rows * cols * sizeof(uint32_t) + 256;            // tile list
+        n += ((size_t)g.lr_h[p] / 4 + 16) * sizeof(uint32_t);  // superres border rows
     }
     return n;
 }
@@ -1384,7 +1403,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_use_graphs(vaco_av1pf_ctx *ctx, int enab
 static bool tuning_ok(const vaco_av1pf_tuning &t)
 {
     return (t.superres_rows == 4 || t.superres_rows == 8 || t.superres_rows == 16) &&
-           t.cdef_xcd_order <= 1 && t.lr_order <= 1 && t.reserved == 0;
+           t.cdef_xcd_order <= 1 && t.lr_order <= 1 &&
+           (t.lr_tile_rows == 64 || t.lr_tile_rows == 32 || t.lr_tile_rows == 16);
 }
 
 extern "C" vaco_av1pf_status vaco_av1pf_get_tuning(const vaco_av1pf_ctx *ctx, vaco_av1pf_tuning *out)
@@ -1472,6 +1492,20 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
         if (tm > 0 && (best_ms < 0 || tm < best_ms)) {  // kernels absent from the batch time 0
             best_ms = tm;
             best.lr_order = v;
+        }
+    }
+    // restoration tile height
+    best_ms = -1;
+    for (uint8_t rows : {64, 32, 16}) {
+        if (st != VACO_AV1PF_OK)
+            break;
+        t = best;
+        t.lr_tile_rows = rows;
+        if ((st = measure(t, VACO_AV1PF_KERNEL_RESTORATION, &tm)) != VACO_AV1PF_OK)
+            break;
+        if (tm > 0 && (best_ms < 0 || tm < best_ms)) {  // kernels absent from the batch time 0
+            best_ms = tm;
+            best.lr_tile_rows = rows;
         }
     }
     ctx->tune = best;
