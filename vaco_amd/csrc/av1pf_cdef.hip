@@ -20,10 +20,10 @@
 namespace av1pf {
 
 // DIRECTIONS, cdef.rs:56-66, as (row, column).
-__constant__ int8_t kDirDy[8][2] = {{-1, -2}, {0, -1}, {0, 0}, {0, 1},
-                                    {1, 2},   {1, 2},  {1, 2}, {1, 2}};
-__constant__ int8_t kDirDx[8][2] = {{1, 2}, {1, 2}, {1, 2}, {1, 2},
-                                    {1, 2}, {0, 1}, {0, 0}, {0, -1}};
+constexpr int8_t kDirDy[8][2] = {{-1, -2}, {0, -1}, {0, 0}, {0, 1},
+                                 {1, 2},   {1, 2},  {1, 2}, {1, 2}};
+constexpr int8_t kDirDx[8][2] = {{1, 2}, {1, 2}, {1, 2}, {1, 2},
+                                 {1, 2}, {0, 1}, {0, 0}, {0, -1}};
 
 // Line-partition bin of pixel (i, j) for direction d (cdef.rs:88-97).
 constexpr int bin_of(int d, int i, int j)
@@ -49,6 +49,25 @@ constexpr int bin_count(int d, int b)
 }
 
 constexpr int kRow = 80;         // LDS tile row: columns px0-8 .. px0+71
+
+// LDS offsets (dy * kRow + dx) of DIRECTIONS[d][k] for directions 4h .. 4h+3,
+// 16 bits each, so a lane selects its direction's offset with register
+// operations instead of a memory load.
+constexpr uint64_t dir_offsets(int k, int h)
+{
+    uint64_t v = 0;
+    for (int i = 0; i < 4; i++)
+        v |= (uint64_t)(uint16_t)(int16_t)(kDirDy[4 * h + i][k] * kRow + kDirDx[4 * h + i][k]) << (16 * i);
+    return v;
+}
+
+__device__ __forceinline__ int tap_offset(int dir, int k)
+{
+    constexpr uint64_t t00 = dir_offsets(0, 0), t01 = dir_offsets(0, 1);
+    constexpr uint64_t t10 = dir_offsets(1, 0), t11 = dir_offsets(1, 1);
+    const uint64_t t = k ? (dir < 4 ? t10 : t11) : (dir < 4 ? t00 : t01);
+    return (int16_t)(uint16_t)(t >> (16 * (dir & 3)));
+}
 constexpr int kTile = 68 * kRow; // one plane's tile
 constexpr int kShift = 3 * kTile; // tile p + kShift: plane p shifted by one sample
 constexpr int kThreads = 1024;
@@ -117,7 +136,7 @@ struct __align__(16) Blk {
 };
 
 __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int damping,
-                                        int shift, int row)
+                                        int shift)
 {
     Blk B;
     B.pri = (uint8_t)min(pri, 255u);
@@ -129,9 +148,9 @@ __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int
     const int d2 = (dir + 2) & 7, d6 = (dir + 6) & 7;
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-        B.off[k] = (int16_t)(kDirDy[dir][k] * row + kDirDx[dir][k]);
-        B.off[2 + k] = (int16_t)(kDirDy[d2][k] * row + kDirDx[d2][k]);
-        B.off[4 + k] = (int16_t)(kDirDy[d6][k] * row + kDirDx[d6][k]);
+        B.off[k] = (int16_t)tap_offset(dir, k);
+        B.off[2 + k] = (int16_t)tap_offset(d2, k);
+        B.off[4 + k] = (int16_t)tap_offset(d6, k);
     }
 #pragma unroll
     for (int k = 0; k < 6; k++) {
@@ -506,7 +525,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
                         adj = (pri * (4 + lg) + 8) >> 4;
                     }
                     B = make_blk(pri ? best : 0, adj, (uint32_t)f.y_sec[ix] << shift,
-                                 (uint8_t)(f.damping + shift), shift, kRow);
+                                 (uint8_t)(f.damping + shift), shift);
                 } else {
                     const uint32_t pri = (uint32_t)f.uv_pri[ix] << shift;
                     // chroma_direction, cdef.rs:222-230: nibble `best` of
@@ -517,7 +536,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
                     else if (!SX && SY)
                         cdir = (0x06432221u >> (4 * best)) & 7;
                     B = make_blk(pri ? cdir : 0, pri, (uint32_t)f.uv_sec[ix] << shift,
-                                 (uint8_t)(f.damping + shift - 1), shift, kRow);
+                                 (uint8_t)(f.damping + shift - 1), shift);
                 }
             }
             blk[cls][b] = B;
@@ -582,7 +601,7 @@ __global__ __launch_bounds__(64) void k_cdef_cases(const uint16_t *__restrict__ 
         dirs[cs] = (uint8_t)best;
         vars[cs] = (bc - cost[(best + 4) & 7]) >> 10;
     }
-    const Blk B = make_blk(P.direction, P.primary, P.secondary, P.damping, shift, kRow);
+    const Blk B = make_blk(P.direction, P.primary, P.secondary, P.damping, shift);
     const int r = lane >> 3, c = lane & 7;
     uint16_t res = 0;
     if (r < (int)P.height && c < (int)P.width)
