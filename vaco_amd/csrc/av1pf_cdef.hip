@@ -335,24 +335,29 @@ __device__ __forceinline__ bool stage(uint16_t *dst, const uint16_t *src, int W,
     return bad;
 }
 
-// stage() for interior superblocks of planes with 8-byte aligned rows: the
-// window widened to start at column xs - 2 and loaded four samples at a
-// time (the two extra columns on each side are never read by a tap). Also
-// fills the shifted copy.
-template <int ROWS, int COLS>
-__device__ __forceinline__ bool stage_vec(uint16_t *dst, const uint16_t *src, int st, int xs, int ys,
-                                          int maxv)
+// stage() for planes with 8-byte aligned rows: the window widened to start
+// at column xs - 2 and loaded four samples at a time (the two extra columns
+// on each side are never read by a tap). Also fills the shifted copy. Group
+// columns are multiples of 4 and plane widths are multiples of 4, so a group
+// lies wholly inside or wholly outside the plane; outside (BORDER only) it
+// holds kUnavailable.
+template <int ROWS, int COLS, bool BORDER>
+__device__ __forceinline__ bool stage_vec(uint16_t *dst, const uint16_t *src, int W, int H, int st,
+                                          int xs, int ys, int maxv)
 {
     constexpr int V = (COLS + 4) / 4;  // 4-sample groups per row
     bool bad = false;
 #pragma unroll 2
     for (int e = threadIdx.x; e < ROWS * V; e += kThreads) {
         const int r = e / V, v = e - r * V;
-        const uint2 q =
-            *reinterpret_cast<const uint2 *>(src + (size_t)(ys + r) * st + xs - 2 + 4 * v);
-        const uint32_t m = max(max(q.x & 0xffffu, q.x >> 16), max(q.y & 0xffffu, q.y >> 16));
-        bad |= (int)m > maxv;
-        uint16_t *t = dst + r * kRow + 4 + 4 * v;
+        const int y = ys + r, x = xs - 2 + 4 * v;
+        uint2 q = {0xffffffffu, 0xffffffffu};
+        if (!BORDER || (y >= 0 && y < H && x >= 0 && x < W)) {
+            q = *reinterpret_cast<const uint2 *>(src + (size_t)y * st + x);
+            const uint32_t m = max(max(q.x & 0xffffu, q.x >> 16), max(q.y & 0xffffu, q.y >> 16));
+            bad |= (int)m > maxv;
+        }
+        uint16_t *t = static_cast<uint16_t *>(__builtin_assume_aligned(dst + r * kRow + 4 + 4 * v, 8));
         *reinterpret_cast<uint2 *>(t) = q;
         uint16_t *u = t + kShift + 1;  // sample i of the tile at i + 1
         u[0] = (uint16_t)q.x;
@@ -470,7 +475,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     }
     bool bad = false;
     const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
-    if (border || !f.vec_ok) {
+    if (!f.vec_ok) {
         bad = stage<68, 68>(lt[0], f.in[0], W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border, maxv);
         if (nplanes > 1) {
             bad |= stage<CH + 4, CW + 4>(lt[1], f.in[1], f.width[1], f.height[1], f.in_stride[1],
@@ -479,13 +484,23 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
                                          cx, cy, border, maxv);
         }
     } else {
-        // Interior: rows y0-2 .. y0+65 and columns x0-4 .. x0+67 lie inside the
-        // plane, because padded sizes are multiples of 8 and the superblock
-        // does not touch the last 8 rows or columns (likewise for chroma).
-        bad = stage_vec<68, 68>(lt[0], f.in[0], f.in_stride[0], x0 - 2, y0 - 2, maxv);
-        if (nplanes > 1) {
-            bad |= stage_vec<CH + 4, CW + 4>(lt[1], f.in[1], f.in_stride[1], cx, cy, maxv);
-            bad |= stage_vec<CH + 4, CW + 4>(lt[2], f.in[2], f.in_stride[2], cx, cy, maxv);
+        // Interior superblocks: rows y0-2 .. y0+65 and columns x0-4 .. x0+67
+        // lie inside the plane, because padded sizes are multiples of 8 and
+        // the superblock does not touch the last 8 rows or columns (likewise
+        // for chroma), so no bounds are checked.
+        const int W1 = nplanes > 1 ? (int)f.width[1] : 0, H1 = nplanes > 1 ? (int)f.height[1] : 0;
+        if (border) {
+            bad = stage_vec<68, 68, true>(lt[0], f.in[0], W0, H0, f.in_stride[0], x0 - 2, y0 - 2, maxv);
+            if (nplanes > 1) {
+                bad |= stage_vec<CH + 4, CW + 4, true>(lt[1], f.in[1], W1, H1, f.in_stride[1], cx, cy, maxv);
+                bad |= stage_vec<CH + 4, CW + 4, true>(lt[2], f.in[2], W1, H1, f.in_stride[2], cx, cy, maxv);
+            }
+        } else {
+            bad = stage_vec<68, 68, false>(lt[0], f.in[0], W0, H0, f.in_stride[0], x0 - 2, y0 - 2, maxv);
+            if (nplanes > 1) {
+                bad |= stage_vec<CH + 4, CW + 4, false>(lt[1], f.in[1], W1, H1, f.in_stride[1], cx, cy, maxv);
+                bad |= stage_vec<CH + 4, CW + 4, false>(lt[2], f.in[2], W1, H1, f.in_stride[2], cx, cy, maxv);
+            }
         }
     }
     if (__syncthreads_or(bad) && tid == 0)
