@@ -241,6 +241,42 @@ vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32_t n,
                                           const vaco_av1pf_picture *ins,
                                           vaco_av1pf_picture *outs, void *stream);
 
+/* ------------------------------------------------ parameter producers --- */
+
+/* The uncompressed-header syntax that produces the stage's frame-level
+ * parameters, for decoders without their own header parser. Bits are read
+ * MSB first; reads past the end return zeros and set `overrun`
+ * (vaco-bitstream/src/reader.rs:72-210). */
+typedef struct vaco_av1pf_bit_reader {
+    const uint8_t *data;
+    uint64_t size_bits;
+    uint64_t pos;      /* bits consumed, including any past the end */
+    uint32_t overrun;  /* 1 once a read went past the end */
+} vaco_av1pf_bit_reader;
+
+/* The sequence-header fields the syntax depends on. */
+typedef struct vaco_av1pf_seq_info {
+    uint8_t enable_cdef, enable_restoration, enable_superres;
+    uint8_t use_128x128_superblock;
+    uint8_t subsampling_x, subsampling_y;
+} vaco_av1pf_seq_info;
+
+void vaco_av1pf_bits_init(vaco_av1pf_bit_reader *r, const uint8_t *data, size_t len);
+uint32_t vaco_av1pf_bits_get(vaco_av1pf_bit_reader *r, uint32_t n);
+/* frame_header.rs:1034-1072 (cdef_params): fills *out (zeroed, enabled = 0
+ * when CDEF is off, which reads no bits) and returns cdef_bits. */
+uint32_t vaco_av1pf_parse_cdef_params(vaco_av1pf_bit_reader *r, const vaco_av1pf_seq_info *seq,
+                                      int coded_lossless, int allow_intrabc, uint32_t num_planes,
+                                      vaco_av1pf_cdef_params *out);
+/* frame_header.rs:1074-1123 (lr_params): types as VACO_AV1PF_RESTORE_*
+ * values, unit sizes in samples (0 when restoration is unused). */
+void vaco_av1pf_parse_lr_params(vaco_av1pf_bit_reader *r, const vaco_av1pf_seq_info *seq,
+                                int all_lossless, int allow_intrabc, uint32_t num_planes,
+                                uint8_t types[3], uint32_t unit_size[3]);
+/* frame_header.rs:687-704 (superres_params): returns the coded width. */
+uint32_t vaco_av1pf_parse_superres_params(vaco_av1pf_bit_reader *r, const vaco_av1pf_seq_info *seq,
+                                          uint32_t upscaled_width, uint8_t *use_superres);
+
 /* ------------------------------------------------------------ graphs --- */
 
 /* No reference counterpart. With graphs enabled, the kernel chain of a

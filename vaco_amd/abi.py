@@ -125,6 +125,26 @@ class FrameDesc(C.Structure):
     ]
 
 
+class BitReader(C.Structure):
+    _fields_ = [
+        ("data", C.c_void_p),
+        ("size_bits", C.c_uint64),
+        ("pos", C.c_uint64),
+        ("overrun", C.c_uint32),
+    ]
+
+
+class SeqInfo(C.Structure):
+    _fields_ = [
+        ("enable_cdef", C.c_uint8),
+        ("enable_restoration", C.c_uint8),
+        ("enable_superres", C.c_uint8),
+        ("use_128x128_superblock", C.c_uint8),
+        ("subsampling_x", C.c_uint8),
+        ("subsampling_y", C.c_uint8),
+    ]
+
+
 class Tuning(C.Structure):
     _fields_ = [
         ("superres_rows", C.c_uint8),

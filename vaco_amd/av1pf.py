@@ -114,6 +114,19 @@ def lib():
         L.vaco_av1pf_profile.argtypes = [ctxp, C.c_int]
         L.vaco_av1pf_profile_read.argtypes = [ctxp, P(C.c_double), P(C.c_uint64), P(C.c_uint64)]
         L.vaco_av1pf_use_graphs.argtypes = [ctxp, C.c_int]
+        L.vaco_av1pf_bits_init.argtypes = [P(abi.BitReader), C.c_void_p, C.c_size_t]
+        L.vaco_av1pf_bits_init.restype = None
+        L.vaco_av1pf_bits_get.argtypes = [P(abi.BitReader), C.c_uint32]
+        L.vaco_av1pf_bits_get.restype = C.c_uint32
+        L.vaco_av1pf_parse_cdef_params.argtypes = [P(abi.BitReader), P(abi.SeqInfo), C.c_int, C.c_int,
+                                                   C.c_uint32, P(abi.CdefParams)]
+        L.vaco_av1pf_parse_cdef_params.restype = C.c_uint32
+        L.vaco_av1pf_parse_lr_params.argtypes = [P(abi.BitReader), P(abi.SeqInfo), C.c_int, C.c_int,
+                                                 C.c_uint32, P(C.c_uint8), P(C.c_uint32)]
+        L.vaco_av1pf_parse_lr_params.restype = None
+        L.vaco_av1pf_parse_superres_params.argtypes = [P(abi.BitReader), P(abi.SeqInfo), C.c_uint32,
+                                                       P(C.c_uint8)]
+        L.vaco_av1pf_parse_superres_params.restype = C.c_uint32
         L.vaco_av1pf_get_tuning.argtypes = [ctxp, P(abi.Tuning)]
         L.vaco_av1pf_set_tuning.argtypes = [ctxp, P(abi.Tuning)]
         L.vaco_av1pf_autotune.argtypes = [ctxp, C.c_uint32, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture),
@@ -474,6 +487,56 @@ def frame_bytes(frame: Frame) -> int:
 
 def now_ns() -> int:
     return lib().vaco_av1pf_now_ns()
+
+
+# ------------------------------------------------ parameter producers --
+
+class Bits:
+    """An MSB-first bit reader over `data` (vaco_av1pf_bit_reader)."""
+
+    def __init__(self, data: bytes):
+        self._buf = (C.c_uint8 * max(1, len(data))).from_buffer_copy(bytes(data) or b"\0")
+        self.r = abi.BitReader()
+        lib().vaco_av1pf_bits_init(C.byref(self.r), C.cast(self._buf, C.c_void_p) if data else None, len(data))
+
+    @property
+    def pos(self) -> int:
+        return self.r.pos
+
+    @property
+    def overrun(self) -> bool:
+        return bool(self.r.overrun)
+
+    def get(self, n: int) -> int:
+        return lib().vaco_av1pf_bits_get(C.byref(self.r), n)
+
+
+def seq_info(cdef=False, restoration=False, superres=False, sb128=False, ssx=True, ssy=True) -> abi.SeqInfo:
+    return abi.SeqInfo(int(cdef), int(restoration), int(superres), int(sb128), int(ssx), int(ssy))
+
+
+def parse_cdef_params(bits: Bits, seq: abi.SeqInfo, coded_lossless=False, allow_intrabc=False, num_planes=3):
+    """-> (cdef_bits, CdefParams)"""
+    out = abi.CdefParams()
+    n = lib().vaco_av1pf_parse_cdef_params(C.byref(bits.r), C.byref(seq), int(coded_lossless),
+                                           int(allow_intrabc), num_planes, C.byref(out))
+    return n, out
+
+
+def parse_lr_params(bits: Bits, seq: abi.SeqInfo, all_lossless=False, allow_intrabc=False, num_planes=3):
+    """-> (types[3], unit_size[3])"""
+    t = (C.c_uint8 * 3)()
+    u = (C.c_uint32 * 3)()
+    lib().vaco_av1pf_parse_lr_params(C.byref(bits.r), C.byref(seq), int(all_lossless), int(allow_intrabc),
+                                     num_planes, t, u)
+    return list(t), list(u)
+
+
+def parse_superres_params(bits: Bits, seq: abi.SeqInfo, upscaled_width: int):
+    """-> (coded_width, use_superres)"""
+    use = C.c_uint8()
+    w = lib().vaco_av1pf_parse_superres_params(C.byref(bits.r), C.byref(seq), upscaled_width, C.byref(use))
+    return w, bool(use.value)
 
 
 # --------------------------------------------------------- host helpers --
