@@ -241,6 +241,33 @@ vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32_t n,
                                           const vaco_av1pf_picture *ins,
                                           vaco_av1pf_picture *outs, void *stream);
 
+/* ------------------------------------------------------ frame output --- */
+
+/* Output planes in the layout of the decoder's Frame (pic_to_frame,
+ * decode.rs:3795-3852): one byte per sample at 8-bit, two (little-endian)
+ * above; rows stride_bytes apart. */
+typedef struct vaco_av1pf_frame_planes {
+    void *data[3];
+    uint32_t stride_bytes[3];
+} vaco_av1pf_frame_planes;
+
+/* pic_to_frame's plane sizes (decode.rs:3821-3829): luma upscaled_width x
+ * coded_height, chroma rounded up; and the bytes per sample. */
+vaco_av1pf_status vaco_av1pf_frame_layout(const vaco_av1pf_frame_desc *desc, uint32_t width[3],
+                                          uint32_t height[3], uint32_t *bytes_per_sample);
+
+/* decode.rs:554-560 as one call: the stage on device planes, with the final
+ * stage storing straight into frame planes as pic_to_frame and its blit
+ * (decode.rs:3863-3891) would write them: cropped to the frame, converted to
+ * bytes at 8-bit. This removes the conversion pass and halves the output
+ * bytes of 8-bit frames. Frame samples blit would not write (a chroma column
+ * or row that restoration's rounded-down planes do not reach) are left as
+ * they are. Asynchronous like vaco_av1pf_frame_device. */
+vaco_av1pf_status vaco_av1pf_frame_device_to_frame(vaco_av1pf_ctx *ctx,
+                                                   const vaco_av1pf_frame_desc *desc,
+                                                   const vaco_av1pf_picture *in,
+                                                   const vaco_av1pf_frame_planes *out, void *stream);
+
 /* ------------------------------------------------ parameter producers --- */
 
 /* The uncompressed-header syntax that produces the stage's frame-level

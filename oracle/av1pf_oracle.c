@@ -973,3 +973,37 @@ int orc_frame(const vaco_av1pf_frame_desc *d, const vaco_av1pf_picture *in,
         opic_free(&pre);
     return rc;
 }
+
+/* ------------------------------------------------------------------------
+ * pic_to_frame (decode.rs:3795-3852) and blit (decode.rs:3863-3891)
+ * ---------------------------------------------------------------------- */
+void orc_pic_to_frame(const vaco_av1pf_frame_desc *d, const vaco_av1pf_picture *pic,
+                      uint8_t *const out[3], const uint32_t stride_bytes[3])
+{
+    const int64_t w = d->upscaled_width, h = d->coded_height;
+    const int64_t sx = d->subsampling_x, sy = d->subsampling_y;
+    const int two_bytes = d->bit_depth > 8;
+    const int np = d->monochrome ? 1 : 3;
+    int p;
+    for (p = 0; p < np; p++) {
+        /* decode.rs:3827-3829: chroma plane size by div_ceil */
+        const int64_t fw = p ? (w + (1 << sx) - 1) >> sx : w;
+        const int64_t fh = p ? (h + (1 << sy) - 1) >> sy : h;
+        const oplane src = op_from(&pic->plane[p]);
+        /* blit over min(frame, plane) (decode.rs:3831-3850) */
+        const int64_t bw = fw < src.w ? fw : src.w, bh = fh < src.h ? fh : src.h;
+        int64_t x, y;
+        for (y = 0; y < bh; y++) {
+            uint8_t *row = out[p] + (size_t)y * stride_bytes[p];
+            for (x = 0; x < bw; x++) {
+                const uint16_t v = op_get(&src, x, y);
+                if (two_bytes) {
+                    row[2 * x] = (uint8_t)(v & 0xff);
+                    row[2 * x + 1] = (uint8_t)(v >> 8);
+                } else {
+                    row[x] = v <= 255 ? (uint8_t)v : 0; /* u8::try_from(v).unwrap_or(0) */
+                }
+            }
+        }
+    }
+}

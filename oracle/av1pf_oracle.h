@@ -18,6 +18,8 @@
  *   orc_adjust_strength                    closed form, exhaustive log2 buckets
  *   orc_chroma_direction                   spec table, exhaustive
  *   orc_lr_unit_counts / orc_lr_unit_index reference unit-geometry test vectors
+ *   orc_pic_to_frame                       closed form of blit (crop, byte order)
+ *                                          on hand-checked planes
  *   orc_frame                              invariants + brute force against the
  *                                          pinned block/plane functions;
  *                                          "parity unpinned" for the frame glue
@@ -66,6 +68,14 @@ int orc_frame_output_geometry(const vaco_av1pf_frame_desc *d, uint32_t w[3],
  * orc_frame_output_geometry reports; they are written in full. */
 int orc_frame(const vaco_av1pf_frame_desc *d, const vaco_av1pf_picture *in,
               vaco_av1pf_picture *out, uint64_t *fuel, const char **err);
+
+/* pic_to_frame with blit, decode.rs:3795-3891: the final picture `pic`
+ * (orc_frame's output) written into frame planes of upscaled_width x
+ * coded_height (chroma rounded up), one byte per sample at 8-bit and two
+ * little-endian bytes above, rows stride_bytes apart. Each plane writes its
+ * min(frame, picture) region; other frame bytes are left as they are. */
+void orc_pic_to_frame(const vaco_av1pf_frame_desc *d, const vaco_av1pf_picture *pic,
+                      uint8_t *const out[3], const uint32_t stride_bytes[3]);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,9 @@ def lib():
         L.orc_restore_plane.argtypes = [C.POINTER(abi.Plane), C.POINTER(abi.Plane), C.POINTER(abi.LrPlaneConfig), C.POINTER(abi.LrUnit), C.c_size_t, C.POINTER(abi.Plane), E]
         L.orc_frame_output_geometry.argtypes = [C.POINTER(abi.FrameDesc), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_frame.argtypes = [C.POINTER(abi.FrameDesc), C.POINTER(abi.Picture), C.POINTER(abi.Picture), C.POINTER(C.c_uint64), E]
+        L.orc_pic_to_frame.argtypes = [C.POINTER(abi.FrameDesc), C.POINTER(abi.Picture), C.c_void_p * 3,
+                                       C.c_uint32 * 3]
+        L.orc_pic_to_frame.restype = None
         for f in ("orc_find_direction", "orc_filter_block", "orc_upscale_plane", "orc_restore_plane", "orc_frame", "orc_frame_output_geometry"):
             getattr(L, f).restype = C.c_int
         _lib = L
@@ -141,3 +144,15 @@ def frame(desc: abi.FrameDesc, planes, fuel: int | None = None):
     f = C.c_uint64(fuel) if fuel is not None else None
     _check(lib().orc_frame(C.byref(desc), C.byref(pin), C.byref(pout), C.byref(f) if f is not None else None, C.byref(err)), err)
     return outs
+
+
+def pic_to_frame(desc: abi.FrameDesc, pic_planes, frame_geometry, bytes_per_sample):
+    """decode.rs:3795-3891: the final picture as frame planes (zero-filled
+    first, as Frame::alloc_video leaves them), uint8 or uint16."""
+    dt = np.uint8 if bytes_per_sample == 1 else np.uint16
+    out = [np.zeros(g, dt) for g in frame_geometry]
+    pic = abi.picture_of([np.ascontiguousarray(p, dtype=np.uint16) for p in pic_planes])
+    ptrs = (C.c_void_p * 3)(*[o.ctypes.data for o in out] + [None] * (3 - len(out)))
+    strides = (C.c_uint32 * 3)(*[o.shape[1] * o.itemsize for o in out] + [0] * (3 - len(out)))
+    lib().orc_pic_to_frame(C.byref(desc), C.byref(pic), ptrs, strides)
+    return out

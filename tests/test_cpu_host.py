@@ -252,3 +252,37 @@ def test_out_of_range_sample_is_reported_by_the_first_stage_reading_it():
     with pytest.raises(O.OracleError) as e:
         O.frame(f.desc, f.planes)
     assert e.value.message == "CDEF sample exceeds bit depth"
+
+
+def test_pic_to_frame_oracle_crops_and_packs_like_blit():
+    # decode.rs:3795-3891 on hand-checkable planes: a padded 8-bit picture is
+    # cropped to coded size, an out-of-range 8-bit sample becomes 0
+    # (u8::try_from(..).unwrap_or(0)), 10-bit samples are two little-endian
+    # bytes, and a frame plane larger than the picture plane keeps its extra
+    # row untouched.
+    d = abi.FrameDesc()
+    d.coded_width, d.upscaled_width, d.coded_height = 5, 5, 3
+    d.bit_depth, d.subsampling_x, d.subsampling_y = 8, 1, 1
+    y = np.arange(8 * 4, dtype=np.uint16).reshape(4, 8) * 9  # padded 8x4
+    y[1, 2] = 300
+    u = np.array([[1, 2, 3, 4]], dtype=np.uint16)                # 1 row: frame chroma has 2
+    v = np.array([[5, 6, 7, 8]], dtype=np.uint16)
+    fr = O.pic_to_frame(d, [y, u, v], [(3, 5), (2, 3), (2, 3)], 1)
+    want_y = (np.arange(8 * 4).reshape(4, 8) * 9)[:3, :5].astype(np.uint8)
+    want_y[1, 2] = 0
+    assert np.array_equal(fr[0], want_y)
+    assert fr[1].tolist() == [[1, 2, 3], [0, 0, 0]] and fr[2].tolist() == [[5, 6, 7], [0, 0, 0]]
+    d.bit_depth = 10
+    y10 = np.full((4, 8), 0x3A5, np.uint16)
+    fr = O.pic_to_frame(d, [y10, u, v], [(3, 5), (2, 3), (2, 3)], 2)
+    assert fr[0].dtype == np.uint16 and (fr[0] == 0x3A5).all()
+    assert fr[0].view(np.uint8)[0, :2].tolist() == [0xA5, 0x03]
+
+
+def test_frame_layout_matches_pic_to_frame_sizes():
+    f = F.random_frame(seed=3, coded_w=93, coded_h=75, upscaled_w=161)
+    geo, bps = av1pf.frame_layout(f)
+    assert geo == [(75, 161), (38, 81), (38, 81)] and bps == 1
+    g = F.random_frame(seed=4, coded_w=77, coded_h=45, mono=True, bd=10)
+    geo, bps = av1pf.frame_layout(g)
+    assert geo == [(45, 77)] and bps == 2

@@ -125,6 +125,10 @@ class FrameDesc(C.Structure):
     ]
 
 
+class FramePlanes(C.Structure):
+    _fields_ = [("data", C.c_void_p * 3), ("stride_bytes", C.c_uint32 * 3)]
+
+
 class BitReader(C.Structure):
     _fields_ = [
         ("data", C.c_void_p),

@@ -114,6 +114,9 @@ def lib():
         L.vaco_av1pf_profile.argtypes = [ctxp, C.c_int]
         L.vaco_av1pf_profile_read.argtypes = [ctxp, P(C.c_double), P(C.c_uint64), P(C.c_uint64)]
         L.vaco_av1pf_use_graphs.argtypes = [ctxp, C.c_int]
+        L.vaco_av1pf_frame_layout.argtypes = [P(abi.FrameDesc), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+        L.vaco_av1pf_frame_device_to_frame.argtypes = [ctxp, P(abi.FrameDesc), P(abi.Picture),
+                                                       P(abi.FramePlanes), C.c_void_p]
         L.vaco_av1pf_bits_init.argtypes = [P(abi.BitReader), C.c_void_p, C.c_size_t]
         L.vaco_av1pf_bits_init.restype = None
         L.vaco_av1pf_bits_get.argtypes = [P(abi.BitReader), C.c_uint32]
@@ -142,7 +145,7 @@ def lib():
         L.vaco_av1pf_now_ns.restype = C.c_uint64
         L.vaco_av1pf_server_frame_bytes.argtypes = [P(abi.FrameDesc)]
         L.vaco_av1pf_server_frame_bytes.restype = C.c_uint64
-        for name in ("vaco_av1pf_use_graphs", "vaco_av1pf_get_tuning", "vaco_av1pf_set_tuning", "vaco_av1pf_autotune",
+        for name in ("vaco_av1pf_frame_layout", "vaco_av1pf_frame_device_to_frame", "vaco_av1pf_use_graphs", "vaco_av1pf_get_tuning", "vaco_av1pf_set_tuning", "vaco_av1pf_autotune",
                      "vaco_av1pf_server_create", "vaco_av1pf_server_submit", "vaco_av1pf_server_drain",
                      "vaco_av1pf_batch_device", "vaco_av1pf_profile", "vaco_av1pf_profile_read", "vaco_av1pf_create", "vaco_av1pf_frame", "vaco_av1pf_frame_device", "vaco_av1pf_finish",
                      "vaco_av1pf_cdef_blocks", "vaco_av1pf_upscale_plane", "vaco_av1pf_restore_plane",
@@ -179,6 +182,15 @@ class Frame:
     @property
     def output_luma_pixels(self) -> int:
         return int(self.desc.upscaled_width) * int(self.desc.coded_height)
+
+
+def frame_layout(frame: "Frame"):
+    """pic_to_frame's planes for `frame`: ([(height, width)], bytes_per_sample)."""
+    w, h, b = (C.c_uint32 * 3)(), (C.c_uint32 * 3)(), C.c_uint32()
+    st = lib().vaco_av1pf_frame_layout(C.byref(frame.desc), w, h, C.byref(b))
+    if st != abi.OK:
+        raise Av1Error(st, "vaco_av1pf_frame_layout failed")
+    return [(h[i], w[i]) for i in range(frame.nplanes)], b.value
 
 
 def generate(cfg: GenConfig, frame_index: int = 0) -> Frame:
@@ -302,6 +314,20 @@ class Context:
 
         call.keep = (list(frames), descs, pins, pouts)
         return call
+
+    def frame_device_to_frame(self, frame: "Frame", in_ptrs, out_ptrs, strides_bytes, stream: int | None = None):
+        """The stage on device planes, written as pic_to_frame's frame planes
+        (vaco_av1pf_frame_device_to_frame); see frame_layout()."""
+        pin = abi.Picture()
+        for i in range(frame.nplanes):
+            h, w = frame.planes[i].shape
+            pin.plane[i] = abi.device_plane(in_ptrs[i], w, h)
+        fo = abi.FramePlanes()
+        for i in range(frame.nplanes):
+            fo.data[i] = out_ptrs[i]
+            fo.stride_bytes[i] = strides_bytes[i]
+        self._check(self._lib.vaco_av1pf_frame_device_to_frame(self.handle, C.byref(frame.desc), C.byref(pin),
+                                                               C.byref(fo), C.c_void_p(stream or 0)))
 
     def use_graphs(self, enable: bool = True):
         """Replay each call's kernel chain as a hipGraph per launch shape."""

@@ -367,6 +367,24 @@ __device__ __forceinline__ bool stage_vec(uint16_t *dst, const uint16_t *src, in
     return bad;
 }
 
+// Store a pair of output samples at element index i of a plane of u16
+// samples, or of bytes when u8 (8-bit frame output); the second sample only
+// when `second` (a crop to an odd width can end between the two).
+__device__ __forceinline__ void store_pair(uint16_t *plane, size_t i, s16x2 v, bool u8, bool second)
+{
+    if (u8) {
+        uint8_t *o = reinterpret_cast<uint8_t *>(plane) + i;
+        o[0] = (uint8_t)v.x;
+        if (second)
+            o[1] = (uint8_t)v.y;
+    } else {
+        uint16_t *o = plane + i;
+        o[0] = (uint16_t)v.x;
+        if (second)
+            o[1] = (uint16_t)v.y;
+    }
+}
+
 // Filter (or pass through) every sample of a staged superblock and store
 // it, two horizontally adjacent samples per lane (filter_pairs). Luma: 16
 // lanes per 8x8 block, pairs in rows i0 and i0+4 at columns 2*j2, 2*j2+1.
@@ -395,7 +413,7 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
             B = blk[0][b];
         const uint16_t *t = lt[0];
         uint16_t *dst = f.out[0];
-        const int ost = f.out_stride[0], W0 = f.width[0], H0 = f.height[0];
+        const int ost = f.out_stride[0], W0 = f.crop_w[0], H0 = f.crop_h[0];
         const int ci0 = (2 + bi * 8 + i0) * kRow + 8 + bj * 8 + j;
         s16x2 r2[2];
         if (act) {
@@ -409,18 +427,15 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
             const int i = i0 + 4 * k;
             const int x = x0 + bj * 8 + j, y = y0 + bi * 8 + i;
             const s16x2 res = r2[k];
-            if (!BORDER || (x < W0 && y < H0)) {
-                uint16_t *o = dst + (size_t)y * ost + x;
-                o[0] = (uint16_t)res.x;
-                o[1] = (uint16_t)res.y;
-            }
+            if (!BORDER || (x < W0 && y < H0))
+                store_pair(dst, (size_t)y * ost + x, res, f.out_u8, x + 1 < W0);
         }
     }
     if (f.nplanes > 1) {
         Blk B = {};
         if (act)
             B = blk[1][b];
-        const int cx0 = x0 >> SX, cy0 = y0 >> SY, W1 = f.width[1], H1 = f.height[1];
+        const int cx0 = x0 >> SX, cy0 = y0 >> SY, W1 = f.crop_w[1], H1 = f.crop_h[1];
 #pragma unroll
         for (int k = 0; k < CPX; k++) {
             const int pi = q + 16 * k;
@@ -434,11 +449,8 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
                 filter_pairs<BORDER, 1, 0>(t, ci, B, &res, cpri, csec);
             else
                 res = ld2(t, ci);
-            if (!BORDER || (x < W1 && y < H1)) {
-                uint16_t *o = f.out[p] + (size_t)y * f.out_stride[p] + x;
-                o[0] = (uint16_t)res.x;
-                o[1] = (uint16_t)res.y;
-            }
+            if (!BORDER || (x < W1 && y < H1))
+                store_pair(f.out[p], (size_t)y * f.out_stride[p] + x, res, f.out_u8, x + 1 < W1);
         }
     }
 }

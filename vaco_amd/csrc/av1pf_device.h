@@ -43,6 +43,8 @@ struct CdefFrame {
     uint8_t bit_depth, ss_x, ss_y, nplanes;
     uint8_t damping;                   // CdefDamping (includes +3)
     uint8_t vec_ok;                    // inputs 8-byte aligned, strides multiples of 4
+    uint8_t out_u8;                    // out[] holds one byte per sample (8-bit frame output)
+    uint32_t crop_w[3], crop_h[3];     // stores limited to this region of each output plane
     uint8_t y_pri[8], y_sec[8], uv_pri[8], uv_sec[8];
 };
 
@@ -72,7 +74,8 @@ struct SrJob {
     int32_t start;                     // initial subpel position
     uint32_t block_begin;
     uint16_t max_sample;
-    uint16_t pad;
+    uint8_t out_u8;                    // out holds one byte per sample (8-bit frame output)
+    uint8_t pad;
 };
 
 // -------------------------------------------------------- restoration --
@@ -114,7 +117,7 @@ struct LrPlane {
     uint32_t block_begin;
     uint8_t bit_depth, ss_y;
     uint8_t vec_ok;                    // before/after 8-byte aligned, strides multiples of 4
-    uint8_t pad;
+    uint8_t out_u8;                    // out holds one byte per sample (8-bit frame output)
 };
 
 }  // namespace av1pf

@@ -35,6 +35,8 @@ hipError_t launch_lr(const LrPlane *planes, const uint32_t *list, uint32_t n, bo
                      hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s,
                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_to_bytes(const uint16_t *src, uint32_t src_stride, uint8_t *dst,
+                           uint32_t dst_stride, uint32_t w, uint32_t h, hipStream_t s);
 hipError_t launch_cdef_cases(const uint16_t *inputs, const void *params, uint8_t *dirs,
                              uint32_t *vars, uint16_t *outs, uint32_t n, hipStream_t s);
 }  // namespace av1pf
@@ -442,9 +444,14 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame_output_geometry(const vaco_av1pf_f
 
 namespace {
 
+// A device plane. Frame outputs (vaco_av1pf_frame_device_to_frame) set u8
+// for one byte per sample (stride then counts bytes) and crop_w/crop_h to
+// the frame plane's size: the final stage stores only that region.
 struct DevPlane {
     uint16_t *data;
     uint32_t stride;
+    uint8_t u8 = 0;
+    uint32_t crop_w = 0, crop_h = 0;  // 0: no crop
 };
 
 // Row segments: intersections of restoration stripes (restoration.rs:218-231)
@@ -572,7 +579,7 @@ struct Plan {
     struct Copy {
         DevPlane src, dst;
         uint32_t w, h;
-    };
+    };  // w x h samples; dst.u8: to bytes
     std::vector<Copy> copies;  // frames in which no stage runs
     vaco_av1pf_tuning tune = {8, 1, 0, 64};  // launch variants this plan is built for
     // algorithmic bytes per kernel id (samples read once + written once, 2 B)
@@ -598,8 +605,14 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
         cdef_out[p] = cur[p] = pre[p] = in[p];
     if (!g.cdef && !g.superres && !g.lr) {
         // Nothing runs: the stage's output is its input (decode.rs:741-748).
-        for (int p = 0; p < np; p++)
-            plan.copies.push_back({in[p], out[p], g.in_w[p], g.in_h[p]});
+        for (int p = 0; p < np; p++) {
+            uint32_t w = g.in_w[p], h = g.in_h[p];
+            if (out[p].crop_w) {
+                w = std::min(w, out[p].crop_w);
+                h = std::min(h, out[p].crop_h);
+            }
+            plan.copies.push_back({in[p], out[p], w, h});
+        }
         return VACO_AV1PF_OK;
     }
 
@@ -627,7 +640,14 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             f.height[p] = g.in_h[p];
             f.in_stride[p] = in[p].stride;
             f.out_stride[p] = cdef_out[p].stride;
+            f.crop_w[p] = g.in_w[p];
+            f.crop_h[p] = g.in_h[p];
+            if (cdef_out[p].crop_w) {
+                f.crop_w[p] = std::min(f.crop_w[p], cdef_out[p].crop_w);
+                f.crop_h[p] = std::min(f.crop_h[p], cdef_out[p].crop_h);
+            }
         }
+        f.out_u8 = cdef_out[0].u8;
         f.cdef_idx = didx;
         f.skip8 = dskip;
         f.idx_stride = d.cdef_stride;
@@ -691,7 +711,7 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
                     nrows = (uint32_t)rows.size();
                 }
                 DevPlane src = img == 0 ? cur[p] : in[p];
-                DevPlane dst = out[p];
+                DevPlane dst = out[p];  // the frame plane equals the upscaled plane
                 if (g.lr) {
                     dst = DevPlane{ws.take((size_t)g.sr_out_w[p] * g.sr_h[p]), g.sr_out_w[p]};
                     if (!dst.data)
@@ -713,6 +733,7 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
                 j.start = g.sr_start[p];
                 j.block_begin = plan.sr_blocks;
                 j.max_sample = (uint16_t)((1u << d.bit_depth) - 1);
+                j.out_u8 = dst.u8;
                 plan.sr_blocks += sr_job_blocks(j.n_rows, j.segs, plan.tune.superres_rows);
                 plan.bytes[VACO_AV1PF_KERNEL_SUPERRES] +=
                     (uint64_t)j.n_rows * (g.sr_vis[p] + g.sr_out_w[p]) * 2;
@@ -785,6 +806,7 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             L.ss_y = (uint8_t)(p && d.subsampling_y);
             L.vec_ok = !((uintptr_t)L.before & 7) && !((uintptr_t)L.after & 7) &&
                        !(L.before_stride & 3) && !(L.after_stride & 3);
+            L.out_u8 = out[p].u8;  // restored planes lie inside the frame planes
             plan.lr_blocks += L.n_rows * L.n_cols;
             const uint32_t item = (uint32_t)plan.lr.size();
             for (uint32_t ri = 0; ri < L.n_rows; ri++)
@@ -846,9 +868,14 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
     vaco_av1pf_status st = upload_slot(ctx, a, s);
     if (st != VACO_AV1PF_OK)
         return st;
-    for (const auto &c : plan.copies)
-        HIPCHECK(hipMemcpy2DAsync(c.dst.data, c.dst.stride * 2, c.src.data, c.src.stride * 2, c.w * 2,
-                                  c.h, hipMemcpyDeviceToDevice, s));
+    for (const auto &c : plan.copies) {
+        if (c.dst.u8)
+            HIPCHECK(launch_to_bytes(c.src.data, c.src.stride, (uint8_t *)c.dst.data, c.dst.stride,
+                                     c.w, c.h, s));
+        else
+            HIPCHECK(hipMemcpy2DAsync(c.dst.data, c.dst.stride * 2, c.src.data, c.src.stride * 2,
+                                      c.w * 2, c.h, hipMemcpyDeviceToDevice, s));
+    }
     const uint32_t n_tiles = nlist[0] + nlist[1] + nlist[2];
     auto launch_all = [&]() -> vaco_av1pf_status {
         for (int m = 0; m < 4; m++)
@@ -1112,19 +1139,13 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame(vaco_av1pf_ctx *ctx, const vaco_av
     return VACO_AV1PF_OK;
 }
 
-extern "C" vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
-                                                     const vaco_av1pf_frame_desc *desc,
-                                                     const vaco_av1pf_picture *in,
-                                                     vaco_av1pf_picture *out, void *stream)
+// The device path of one frame into output planes `dout` (u16 picture
+// planes, or frame planes of pic_to_frame's layout).
+static vaco_av1pf_status frame_device_to(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_desc *desc,
+                                         const FrameGeo &g, const vaco_av1pf_picture *in,
+                                         const DevPlane dout[3], void *stream)
 {
-    if (!ctx || !desc || !in || !out)
-        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "null argument");
-    ctx->err = "";
-    FrameGeo g;
-    frame_geometry(*desc, &g);
-    vaco_av1pf_status st = check_planes(ctx, g, desc->cdef_stride, in, out);
-    if (st != VACO_AV1PF_OK)
-        return st;
+    vaco_av1pf_status st;
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     if (!params_clean(*desc, g)) {
         // Parameter errors need the samples to order them exactly.
@@ -1151,11 +1172,9 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
         return st;
     if ((st = grow_slot(ctx, a, arena_need(*desc, g))) != VACO_AV1PF_OK)
         return st;
-    DevPlane din[3], dout[3];
-    for (int p = 0; p < g.nplanes; p++) {
+    DevPlane din[3];
+    for (int p = 0; p < g.nplanes; p++)
         din[p] = DevPlane{in->plane[p].data, in->plane[p].stride};
-        dout[p] = DevPlane{out->plane[p].data, out->plane[p].stride};
-    }
     st = enqueue_frame(ctx, *desc, g, din, dout, a, a->status, s);
     if (st != VACO_AV1PF_OK)
         return st;
@@ -1165,6 +1184,83 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
     a->descs.assign(1, *desc);
     a->ins.assign(1, *in);
     return VACO_AV1PF_OK;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
+                                                     const vaco_av1pf_frame_desc *desc,
+                                                     const vaco_av1pf_picture *in,
+                                                     vaco_av1pf_picture *out, void *stream)
+{
+    if (!ctx || !desc || !in || !out)
+        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "null argument");
+    ctx->err = "";
+    FrameGeo g;
+    frame_geometry(*desc, &g);
+    vaco_av1pf_status st = check_planes(ctx, g, desc->cdef_stride, in, out);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    DevPlane dout[3];
+    for (int p = 0; p < g.nplanes; p++)
+        dout[p] = DevPlane{out->plane[p].data, out->plane[p].stride};
+    return frame_device_to(ctx, desc, g, in, dout, stream);
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_frame_layout(const vaco_av1pf_frame_desc *d,
+                                                     uint32_t width[3], uint32_t height[3],
+                                                     uint32_t *bytes_per_sample)
+{
+    if (!d || !width || !height || !bytes_per_sample)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    // pic_to_frame, decode.rs:3821-3829: upscaled_width x coded_height, chroma
+    // rounded up.
+    const uint32_t w = d->upscaled_width, h = d->coded_height;
+    const uint32_t np = d->monochrome ? 1 : 3;
+    for (uint32_t p = 0; p < 3; p++) {
+        const bool chroma = p > 0;
+        width[p] = p < np ? (chroma ? (w + d->subsampling_x) >> d->subsampling_x : w) : 0;
+        height[p] = p < np ? (chroma ? (h + d->subsampling_y) >> d->subsampling_y : h) : 0;
+    }
+    *bytes_per_sample = d->bit_depth > 8 ? 2 : 1;
+    return VACO_AV1PF_OK;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_frame_device_to_frame(vaco_av1pf_ctx *ctx,
+                                                              const vaco_av1pf_frame_desc *desc,
+                                                              const vaco_av1pf_picture *in,
+                                                              const vaco_av1pf_frame_planes *out,
+                                                              void *stream)
+{
+    if (!ctx || !desc || !in || !out)
+        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "null argument");
+    ctx->err = "";
+    FrameGeo g;
+    frame_geometry(*desc, &g);
+    // Input planes are checked as for the picture output; the output is the
+    // frame layout instead of the final picture.
+    vaco_av1pf_picture probe;
+    memset(&probe, 0, sizeof probe);
+    for (int p = 0; p < g.nplanes; p++) {
+        probe.plane[p].data = (uint16_t *)out->data[p];
+        probe.plane[p].width = g.out_w[p];
+        probe.plane[p].height = g.out_h[p];
+        probe.plane[p].stride = g.out_w[p];
+    }
+    vaco_av1pf_status st = check_planes(ctx, g, desc->cdef_stride, in, &probe);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    uint32_t fw[3], fh[3], bps;
+    vaco_av1pf_frame_layout(desc, fw, fh, &bps);
+    DevPlane dout[3];
+    for (int p = 0; p < g.nplanes; p++) {
+        if (!out->data[p] || out->stride_bytes[p] < fw[p] * bps || out->stride_bytes[p] % bps)
+            return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "frame plane geometry");
+        dout[p].data = (uint16_t *)out->data[p];
+        dout[p].stride = out->stride_bytes[p] / bps;
+        dout[p].u8 = bps == 1;
+        dout[p].crop_w = fw[p];
+        dout[p].crop_h = fh[p];
+    }
+    return frame_device_to(ctx, desc, g, in, dout, stream);
 }
 
 // Settle a slot's deferred check: if its frame raised the sample-range flag,

@@ -58,6 +58,16 @@ __device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, const uint32_t
     return t;
 }
 
+// Store output sample v at element index i: u16, or one byte per sample for
+// an 8-bit frame output.
+__device__ __forceinline__ void put(const LrPlane &P, size_t i, int v)
+{
+    if (P.out_u8)
+        reinterpret_cast<uint8_t *>(P.out)[i] = (uint8_t)v;
+    else
+        P.out[i] = (uint16_t)v;
+}
+
 // Stage Source::get (restoration.rs:233-244) over rows y0-3 .. y0+th+2 and
 // columns x0-3 .. x0+tw+2 into S (row stride kS). Returns whether a sample
 // above the bit depth was read.
@@ -134,7 +144,7 @@ __device__ __forceinline__ void lr_copy(const Tile &t)
         if (r < th && c < tw) {
             const int v = P.after[(size_t)(y0 + r) * P.after_stride + x0 + c];
             bad |= v > maxv;
-            P.out[(size_t)(y0 + r) * P.out_stride + x0 + c] = (uint16_t)min(v, maxv);
+            put(P, (size_t)(y0 + r) * P.out_stride + x0 + c, min(v, maxv));
         }
     }
     if (__syncthreads_or(bad) && threadIdx.x == 0)
@@ -186,7 +196,7 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
         int s = v0 * (m[0] + m[6 * 64]) + v1 * (m[64] + m[5 * 64]) + v2 * (m[2 * 64] + m[4 * 64]) +
                 v3 * m[3 * 64];
         s = round2i(s, 14 - round_h);
-        P.out[(size_t)(y0 + r) * P.out_stride + x0 + c] = (uint16_t)min(max(s, 0), maxv);
+        put(P, (size_t)(y0 + r) * P.out_stride + x0 + c, min(max(s, 0), maxv));
     }
 }
 
@@ -351,7 +361,7 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
         }
         int v = w0 * f0[i] + w1 * (uu << 4) + (128 - w0 - w1) * f1;
         v = (v + (1 << 10)) >> 11;
-        P.out[(size_t)(y0 + r) * P.out_stride + x0 + c] = (uint16_t)min(max(v, 0), maxv);
+        put(P, (size_t)(y0 + r) * P.out_stride + x0 + c, min(max(v, 0), maxv));
     }
 }
 

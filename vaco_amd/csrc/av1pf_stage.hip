@@ -36,4 +36,25 @@ hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s
     return hipGetLastError();
 }
 
+// w x h samples of a u16 plane to one byte each (8-bit frame output of a
+// frame in which no stage runs). One thread per sample.
+__global__ __launch_bounds__(256) void k_to_bytes(const uint16_t *__restrict__ src,
+                                                  uint32_t src_stride, uint8_t *__restrict__ dst,
+                                                  uint32_t dst_stride, uint32_t w, uint32_t h)
+{
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x < w && y < h)
+        dst[(size_t)y * dst_stride + x] = (uint8_t)src[(size_t)y * src_stride + x];
+}
+
+hipError_t launch_to_bytes(const uint16_t *src, uint32_t src_stride, uint8_t *dst,
+                           uint32_t dst_stride, uint32_t w, uint32_t h, hipStream_t s)
+{
+    if (w == 0 || h == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_to_bytes, dim3((w + 255) / 256, h), dim3(256), 0, s, src, src_stride, dst,
+                       dst_stride, w, h);
+    return hipGetLastError();
+}
+
 }  // namespace av1pf

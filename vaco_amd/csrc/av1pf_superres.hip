@@ -105,8 +105,12 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
 #pragma unroll
             for (int t = 0; t < 8; t++)
                 sum += win[r][base + t] * f[t];
-            const int v = (sum + 64) >> 7;
-            j.out[(size_t)y * j.out_stride + x] = (uint16_t)min(max(v, 0), (int)j.max_sample);
+            const int v = min(max((sum + 64) >> 7, 0), (int)j.max_sample);
+            const size_t o = (size_t)y * j.out_stride + x;
+            if (j.out_u8)
+                reinterpret_cast<uint8_t *>(j.out)[o] = (uint8_t)v;
+            else
+                j.out[o] = (uint16_t)v;
         }
     }
 }
