@@ -284,6 +284,29 @@ def measure_per_frame(av1pf, hip, cfg, n_frames, steps, warmup):
               "host_call_us_median": round(host_cost(), 2)}
     ctx.use_graphs(False)
 
+    # The same steps written as the decoder's output frames (pic_to_frame
+    # layout; one byte per sample at 8-bit), checked against the picture.
+    geo_f, bps = av1pf.frame_layout(frames[0])
+    dt = np.uint8 if bps == 1 else np.uint16
+    fout = [[hip.DeviceArray(g, dt) for g in geo_f] for _ in frames]
+    fcalls = [ctx.frame_output_call(f, [t.ptr for t in dev_in[k]], [t.ptr for t in fout[k]],
+                                    [g[1] * bps for g in geo_f]) for k, f in enumerate(frames)]
+    for i in range(warmup):
+        fcalls[i % len(fcalls)]()
+    ctx.finish()
+    hip.synchronize()
+    f0 = time.perf_counter()
+    for i in range(steps):
+        fcalls[i % len(fcalls)]()
+    ctx.finish()
+    hip.synchronize()
+    f1 = time.perf_counter()
+    same = all(np.array_equal(fout[k][p].download(), dev_out[k][p].download()[:geo_f[p][0], :geo_f[p][1]].astype(dt))
+               for k in range(len(frames)) for p in range(len(geo_f)))
+    frame_out = {"mpix_s": round(frames[0].output_luma_pixels / 1e6 * steps / (f1 - f0), 2),
+                 "ms_per_step": round((f1 - f0) * 1e3 / steps, 4),
+                 "bytes_per_sample": bps, "equals_picture": bool(same)}
+
     # Kernel durations from HIP events on the library's stream (a separate
     # pass over the same steps, so event records do not perturb the timing).
     ctx.profile(True)
@@ -300,6 +323,7 @@ def measure_per_frame(av1pf, hip, cfg, n_frames, steps, warmup):
         "host_enqueue_ms_per_step": round((t_enq - t0) * 1e3 / steps, 4),
         "host_call_us_median": round(host_us, 2),
         "graph_replay": graphs,
+        "frame_output": frame_out,
         "kernels": kernel_table(prof, steps),
         "tuning": tuning,
         "_frames": frames,
@@ -473,6 +497,7 @@ def main():
         "host_enqueue_ms_per_step": per_frame["host_enqueue_ms_per_step"],
         "host_call_us_median": per_frame["host_call_us_median"],
         "graph_replay": per_frame["graph_replay"],
+        "frame_output": per_frame["frame_output"],
         "tuning": per_frame["tuning"],
         "kernels": kern,
         "roofline": {

@@ -315,9 +315,10 @@ class Context:
         call.keep = (list(frames), descs, pins, pouts)
         return call
 
-    def frame_device_to_frame(self, frame: "Frame", in_ptrs, out_ptrs, strides_bytes, stream: int | None = None):
-        """The stage on device planes, written as pic_to_frame's frame planes
-        (vaco_av1pf_frame_device_to_frame); see frame_layout()."""
+    def frame_output_call(self, frame: "Frame", in_ptrs, out_ptrs, strides_bytes, stream: int | None = None):
+        """A zero-argument callable running the stage on device planes into
+        pic_to_frame's frame planes (vaco_av1pf_frame_device_to_frame); see
+        frame_layout(). The ctypes arguments are built once."""
         pin = abi.Picture()
         for i in range(frame.nplanes):
             h, w = frame.planes[i].shape
@@ -326,8 +327,20 @@ class Context:
         for i in range(frame.nplanes):
             fo.data[i] = out_ptrs[i]
             fo.stride_bytes[i] = strides_bytes[i]
-        self._check(self._lib.vaco_av1pf_frame_device_to_frame(self.handle, C.byref(frame.desc), C.byref(pin),
-                                                               C.byref(fo), C.c_void_p(stream or 0)))
+        fn, h = self._lib.vaco_av1pf_frame_device_to_frame, self.handle
+        d, pi, po, st = C.byref(frame.desc), C.byref(pin), C.byref(fo), C.c_void_p(stream or 0)
+        check = self._check
+
+        def call():
+            rc = fn(h, d, pi, po, st)
+            if rc:
+                check(rc)
+
+        call.keep = (frame, pin, fo)
+        return call
+
+    def frame_device_to_frame(self, frame: "Frame", in_ptrs, out_ptrs, strides_bytes, stream: int | None = None):
+        self.frame_output_call(frame, in_ptrs, out_ptrs, strides_bytes, stream)()
 
     def use_graphs(self, enable: bool = True):
         """Replay each call's kernel chain as a hipGraph per launch shape."""
