@@ -316,7 +316,7 @@ __device__ __forceinline__ void filter_pairs(const uint16_t *t, int ci, const Bl
 // tile at column offset 6, and into the shifted copy at dst + kShift;
 // positions outside the plane hold kUnavailable.
 template <int ROWS, int COLS>
-__device__ __forceinline__ bool stage(uint16_t *dst, const uint16_t *src, int W, int H, int st,
+__device__ __forceinline__ bool stage(uint16_t *dst, gptr<const uint16_t> src, int W, int H, int st,
                                       int xs, int ys, bool border, int maxv)
 {
     bool bad = false;
@@ -342,7 +342,7 @@ __device__ __forceinline__ bool stage(uint16_t *dst, const uint16_t *src, int W,
 // lies wholly inside or wholly outside the plane; outside (BORDER only) it
 // holds kUnavailable.
 template <int ROWS, int COLS, bool BORDER>
-__device__ __forceinline__ bool stage_vec(uint16_t *dst, const uint16_t *src, int W, int H, int st,
+__device__ __forceinline__ bool stage_vec(uint16_t *dst, gptr<const uint16_t> src, int W, int H, int st,
                                           int xs, int ys, int maxv)
 {
     constexpr int V = (COLS + 4) / 4;  // 4-sample groups per row
@@ -353,7 +353,8 @@ __device__ __forceinline__ bool stage_vec(uint16_t *dst, const uint16_t *src, in
         const int y = ys + r, x = xs - 2 + 4 * v;
         uint2 q = {0xffffffffu, 0xffffffffu};
         if (!BORDER || (y >= 0 && y < H && x >= 0 && x < W)) {
-            q = *reinterpret_cast<const uint2 *>(src + (size_t)y * st + x);
+            const uint64_t w = *reinterpret_cast<gptr<const uint64_t>>(src + (size_t)y * st + x);
+            q = {(uint32_t)w, (uint32_t)(w >> 32)};
             const uint32_t m = max(max(q.x & 0xffffu, q.x >> 16), max(q.y & 0xffffu, q.y >> 16));
             bad |= (int)m > maxv;
         }
@@ -370,15 +371,15 @@ __device__ __forceinline__ bool stage_vec(uint16_t *dst, const uint16_t *src, in
 // Store a pair of output samples at element index i of a plane of u16
 // samples, or of bytes when u8 (8-bit frame output); the second sample only
 // when `second` (a crop to an odd width can end between the two).
-__device__ __forceinline__ void store_pair(uint16_t *plane, size_t i, s16x2 v, bool u8, bool second)
+__device__ __forceinline__ void store_pair(gptr<uint16_t> plane, size_t i, s16x2 v, bool u8, bool second)
 {
     if (u8) {
-        uint8_t *o = reinterpret_cast<uint8_t *>(plane) + i;
+        gptr<uint8_t> o = reinterpret_cast<gptr<uint8_t>>(plane) + i;
         o[0] = (uint8_t)v.x;
         if (second)
             o[1] = (uint8_t)v.y;
     } else {
-        uint16_t *o = plane + i;
+        gptr<uint16_t> o = plane + i;
         o[0] = (uint16_t)v.x;
         if (second)
             o[1] = (uint16_t)v.y;
@@ -412,7 +413,7 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
         if (act)
             B = blk[0][b];
         const uint16_t *t = lt[0];
-        uint16_t *dst = f.out[0];
+        gptr<uint16_t> dst = glb(f.out[0]);
         const int ost = f.out_stride[0], W0 = f.crop_w[0], H0 = f.crop_h[0];
         const int ci0 = (2 + bi * 8 + i0) * kRow + 8 + bj * 8 + j;
         s16x2 r2[2];
@@ -450,7 +451,7 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
             else
                 res = ld2(t, ci);
             if (!BORDER || (x < W1 && y < H1))
-                store_pair(f.out[p], (size_t)y * f.out_stride[p] + x, res, f.out_u8, x + 1 < W1);
+                store_pair(glb(f.out[p]), (size_t)y * f.out_stride[p] + x, res, f.out_u8, x + 1 < W1);
         }
     }
 }
@@ -461,8 +462,7 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
 template <int SX, int SY, bool XCD>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cdef(const CdefFrame *__restrict__ frames, int n_frames)
 {
-    constexpr int CW = 64 >> SX, CH = 64 >> SY, BW = 8 >> SX, BH = 8 >> SY;
-    constexpr int CPX = BW * BH / 16;  // chroma samples per thread per plane
+    constexpr int CW = 64 >> SX, CH = 64 >> SY;
     __shared__ __align__(16) uint16_t lt[6][kTile];  // planes, then their shifted copies
     __shared__ Blk blk[2][64];
     __shared__ uint32_t cost[8][64];
@@ -477,22 +477,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     const int nplanes = f.nplanes;
     const int x0 = sbx * 64, y0 = sby * 64;
     const int W0 = f.width[0], H0 = f.height[0];
-    const int idx = f.cdef_idx[sby * f.idx_stride + sbx];
+    const int idx = glb(f.cdef_idx)[sby * f.idx_stride + sbx];
     const bool border = x0 == 0 || y0 == 0 || x0 + 64 >= W0 || y0 + 64 >= H0;
 
     if (tid < 64) {  // decode.rs:774-777, 892-896
         const int bx = x0 + (tid & 7) * 8, by = y0 + (tid >> 3) * 8;
         bact[tid] = idx >= 0 && bx < W0 && by < H0 &&
-                    f.skip8[(size_t)(by >> 3) * f.skip_stride + (bx >> 3)] == 0;
+                    glb(f.skip8)[(size_t)(by >> 3) * f.skip_stride + (bx >> 3)] == 0;
     }
     bool bad = false;
     const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
     if (!f.vec_ok) {
-        bad = stage<68, 68>(lt[0], f.in[0], W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border, maxv);
+        bad = stage<68, 68>(lt[0], glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border, maxv);
         if (nplanes > 1) {
-            bad |= stage<CH + 4, CW + 4>(lt[1], f.in[1], f.width[1], f.height[1], f.in_stride[1],
+            bad |= stage<CH + 4, CW + 4>(lt[1], glb(f.in[1]), f.width[1], f.height[1], f.in_stride[1],
                                          cx, cy, border, maxv);
-            bad |= stage<CH + 4, CW + 4>(lt[2], f.in[2], f.width[2], f.height[2], f.in_stride[2],
+            bad |= stage<CH + 4, CW + 4>(lt[2], glb(f.in[2]), f.width[2], f.height[2], f.in_stride[2],
                                          cx, cy, border, maxv);
         }
     } else {
@@ -502,16 +502,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
         // for chroma), so no bounds are checked.
         const int W1 = nplanes > 1 ? (int)f.width[1] : 0, H1 = nplanes > 1 ? (int)f.height[1] : 0;
         if (border) {
-            bad = stage_vec<68, 68, true>(lt[0], f.in[0], W0, H0, f.in_stride[0], x0 - 2, y0 - 2, maxv);
+            bad = stage_vec<68, 68, true>(lt[0], glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2, y0 - 2, maxv);
             if (nplanes > 1) {
-                bad |= stage_vec<CH + 4, CW + 4, true>(lt[1], f.in[1], W1, H1, f.in_stride[1], cx, cy, maxv);
-                bad |= stage_vec<CH + 4, CW + 4, true>(lt[2], f.in[2], W1, H1, f.in_stride[2], cx, cy, maxv);
+                bad |= stage_vec<CH + 4, CW + 4, true>(lt[1], glb(f.in[1]), W1, H1, f.in_stride[1], cx, cy, maxv);
+                bad |= stage_vec<CH + 4, CW + 4, true>(lt[2], glb(f.in[2]), W1, H1, f.in_stride[2], cx, cy, maxv);
             }
         } else {
-            bad = stage_vec<68, 68, false>(lt[0], f.in[0], W0, H0, f.in_stride[0], x0 - 2, y0 - 2, maxv);
+            bad = stage_vec<68, 68, false>(lt[0], glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2, y0 - 2, maxv);
             if (nplanes > 1) {
-                bad |= stage_vec<CH + 4, CW + 4, false>(lt[1], f.in[1], W1, H1, f.in_stride[1], cx, cy, maxv);
-                bad |= stage_vec<CH + 4, CW + 4, false>(lt[2], f.in[2], W1, H1, f.in_stride[2], cx, cy, maxv);
+                bad |= stage_vec<CH + 4, CW + 4, false>(lt[1], glb(f.in[1]), W1, H1, f.in_stride[1], cx, cy, maxv);
+                bad |= stage_vec<CH + 4, CW + 4, false>(lt[2], glb(f.in[2]), W1, H1, f.in_stride[2], cx, cy, maxv);
             }
         }
     }

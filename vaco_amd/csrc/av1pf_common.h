@@ -10,6 +10,27 @@ namespace av1pf {
 
 __device__ __forceinline__ int ilog2u(uint32_t v) { return 31 - __clz(v); }
 
+// Pointers into device memory. A pointer read out of a work-item struct is
+// generic to the compiler, which then emits flat loads and stores; a flat
+// access counts against the LDS wait counter as well, so an LDS wait issued
+// after a flat store also waits for the store. Accesses through gptr are
+// global_* instructions. (The host pass of a .hip file only parses device
+// functions, and copies of address-space-qualified structs do not resolve
+// there, so the alias is a plain pointer in that pass.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+#else
+template <class T>
+using gptr = T *;
+#endif
+
+template <class T>
+__device__ __forceinline__ gptr<T> glb(T *p)
+{
+    return (gptr<T>)p;
+}
+
 // Round2 of the AV1 specification (restoration.rs:251-257) for int32.
 __device__ __forceinline__ int round2i(int v, int s)
 {

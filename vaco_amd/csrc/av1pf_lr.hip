@@ -50,22 +50,39 @@ __device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, const uint32_t
     t.P = &planes[e >> 22];
     const uint32_t local = e & 0x3fffffu;
     const uint32_t rsi = local / t.P->n_cols;
-    t.rs = t.P->rows[rsi];
-    t.cs = t.P->cols[local - rsi * t.P->n_cols];
+    t.rs = glb(t.P->rows)[rsi];
+    t.cs = glb(t.P->cols)[local - rsi * t.P->n_cols];
     t.u.kind = kUnitNone;
     if (t.P->units)
-        t.u = t.P->units[t.rs.unit_row * t.P->unit_cols + t.cs.unit_col];
+        t.u = glb(t.P->units)[t.rs.unit_row * t.P->unit_cols + t.cs.unit_col];
     return t;
 }
 
-// Store output sample v at element index i: u16, or one byte per sample for
-// an 8-bit frame output.
-__device__ __forceinline__ void put(const LrPlane &P, size_t i, int v)
+// Output of tile t: its first sample's address (u16 or, for an 8-bit frame
+// output, byte elements) and the row stride, so that per-sample addresses
+// are 32-bit offsets from it.
+struct TileOut {
+    gptr<uint16_t> base16;
+    gptr<uint8_t> base8;
+    uint32_t stride;
+    bool u8;
+};
+
+__device__ __forceinline__ TileOut tile_out(const LrPlane &P, int x0, int y0)
 {
-    if (P.out_u8)
-        reinterpret_cast<uint8_t *>(P.out)[i] = (uint8_t)v;
+    const size_t o = (size_t)y0 * P.out_stride + x0;
+    return TileOut{glb(P.out) + o, reinterpret_cast<gptr<uint8_t>>(glb(P.out)) + o, P.out_stride,
+                   P.out_u8 != 0};
+}
+
+// Store output sample v at row r, column c of the tile.
+__device__ __forceinline__ void put(const TileOut &o, int r, int c, int v)
+{
+    const uint32_t i = __umul24((uint32_t)r, o.stride) + (uint32_t)c;
+    if (o.u8)
+        o.base8[i] = (uint8_t)v;
     else
-        P.out[i] = (uint16_t)v;
+        o.base16[i] = (uint16_t)v;
 }
 
 // Stage Source::get (restoration.rs:233-244) over rows y0-3 .. y0+th+2 and
@@ -89,14 +106,15 @@ __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int max
             if (r >= srows)
                 continue;
             const int yc = min(max(y0 - 3 + r, 0), H - 1);
-            const uint16_t *row;
+            gptr<const uint16_t> row;
             if (yc < ss)
-                row = P.before + (size_t)max(yc, ss - 2) * P.before_stride;
+                row = glb(P.before) + (size_t)max(yc, ss - 2) * P.before_stride;
             else if (yc > se)
-                row = P.before + (size_t)min(yc, se + 2) * P.before_stride;
+                row = glb(P.before) + (size_t)min(yc, se + 2) * P.before_stride;
             else
-                row = P.after + (size_t)yc * P.after_stride;
-            const uint2 q = *reinterpret_cast<const uint2 *>(row + x0 - 4 + 4 * v);
+                row = glb(P.after) + (size_t)yc * P.after_stride;
+            const uint64_t w = *reinterpret_cast<gptr<const uint64_t>>(row + x0 - 4 + 4 * v);
+            const uint2 q = {(uint32_t)w, (uint32_t)(w >> 32)};
             const uint32_t m = max(max(q.x & 0xffffu, q.x >> 16), max(q.y & 0xffffu, q.y >> 16));
             bad |= (int)m > maxv;
             uint16_t *d = S + r * kS + 4 * v - 1;  // column c of S is x0 - 3 + c
@@ -117,11 +135,11 @@ __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int max
         const int yc = min(max(y0 - 3 + r, 0), H - 1);
         int v;
         if (yc < ss)
-            v = P.before[(size_t)max(yc, ss - 2) * P.before_stride + xc];
+            v = glb(P.before)[(size_t)max(yc, ss - 2) * P.before_stride + xc];
         else if (yc > se)
-            v = P.before[(size_t)min(yc, se + 2) * P.before_stride + xc];
+            v = glb(P.before)[(size_t)min(yc, se + 2) * P.before_stride + xc];
         else
-            v = P.after[(size_t)yc * P.after_stride + xc];
+            v = glb(P.after)[(size_t)yc * P.after_stride + xc];
         bad |= v > maxv;
         S[r * kS + c] = (uint16_t)v;
     }
@@ -137,14 +155,15 @@ __device__ __forceinline__ void lr_copy(const Tile &t)
     const LrPlane &P = *t.P;
     const int maxv = (1 << P.bit_depth) - 1;
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
+    const TileOut o = tile_out(P, x0, y0);
     bool bad = false;
 #pragma unroll 4
     for (int e = threadIdx.x; e < 64 * 64; e += kLrThreads) {
         const int r = e >> 6, c = e & 63;
         if (r < th && c < tw) {
-            const int v = P.after[(size_t)(y0 + r) * P.after_stride + x0 + c];
+            const int v = glb(P.after)[(size_t)(y0 + r) * P.after_stride + x0 + c];
             bad |= v > maxv;
-            put(P, (size_t)(y0 + r) * P.out_stride + x0 + c, min(v, maxv));
+            put(o, r, c, min(v, maxv));
         }
     }
     if (__syncthreads_or(bad) && threadIdx.x == 0)
@@ -162,6 +181,7 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
     const LrPlane &P = *t.P;
     const int bd = P.bit_depth, maxv = (1 << bd) - 1;
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
+    const TileOut o = tile_out(P, x0, y0);
     const int tid = threadIdx.x;
     if (__syncthreads_or(stage_source<kLrThreads>(S, t, maxv)) && tid == 0)
         raise_status(P.status, kFlagSampleRange);
@@ -196,7 +216,7 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
         int s = v0 * (m[0] + m[6 * 64]) + v1 * (m[64] + m[5 * 64]) + v2 * (m[2 * 64] + m[4 * 64]) +
                 v3 * m[3 * 64];
         s = round2i(s, 14 - round_h);
-        put(P, (size_t)(y0 + r) * P.out_stride + x0 + c, min(max(s, 0), maxv));
+        put(o, r, c, min(max(s, 0), maxv));
     }
 }
 
@@ -206,21 +226,26 @@ constexpr int kSgrThreads = kLrThreads;
 constexpr int kSgrPx = 64 * 64 / kSgrThreads;                          // per thread
 
 // A/B of one box from its sums (restoration.rs:314-339), packed as
-// b2 << 9 | a2 (a2 <= 256; b2 < 2^21 for 8/10/12-bit samples). sumsq <=
-// 25 * 4095^2 < 2^31; p * scale and (256 - a2) * sum * recip are formed in
-// 64 bits.
+// b2 << 9 | a2 (a2 <= 256; b2 < 2^21 for 8/10/12-bit samples), in exact
+// 32-bit arithmetic:
+//  - sumsq <= 25 * 4095^2 < 2^31; a < 2^21 and d < 2^17, so a * N and d * d
+//    are 24-bit products below 2^31;
+//  - z >= 255 (a2 = 256) exactly when p >= thr = ceil((255 * 2^20 - 2^19) /
+//    scale), so p is clamped to thr first: q * scale < 2^28 + scale;
+//  - (256 - a2) * recip <= 255 * 455 < 2^24 and sum <= 25 * 4095 < 2^24, and
+//    their product plus 2^11 stays below 2^32 (largest case 255 * 164 *
+//    102375 + 2^11 = 4281324548).
 template <int N>
-__device__ __forceinline__ uint32_t ab_of(int sum, int sumsq, int bd, uint32_t scale,
+__device__ __forceinline__ uint32_t ab_of(int sum, int sumsq, int bd, uint32_t scale, uint32_t thr,
                                           const uint16_t *xbx1)
 {
     constexpr uint32_t recip = ((1u << 12) + N / 2) / N;
-    const int a = round2i(sumsq, 2 * (bd - 8));
-    const int d = round2i(sum, bd - 8);
-    const int p = max(a * N - d * d, 0);
-    const uint64_t z = ((uint64_t)(uint32_t)p * scale + (1u << 19)) >> 20;
-    const uint32_t a2 = z >= 255 ? 256u : xbx1[(int)z];
-    const uint32_t b2 =
-        (uint32_t)(((uint64_t)(256 - a2) * (uint32_t)sum * recip + (1u << 11)) >> 12);
+    const uint32_t a = (uint32_t)round2i(sumsq, 2 * (bd - 8));
+    const uint32_t d = (uint32_t)round2i(sum, bd - 8);
+    const int p = max((int)__umul24(a, (uint32_t)N) - (int)__umul24(d, d), 0);
+    const uint32_t z = (min((uint32_t)p, thr) * scale + (1u << 19)) >> 20;
+    const uint32_t a2 = z >= 255 ? 256u : xbx1[z];
+    const uint32_t b2 = (__umul24((256 - a2) * recip, (uint32_t)sum) + (1u << 11)) >> 12;
     return (b2 << 9) | a2;
 }
 
@@ -237,6 +262,7 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
     constexpr int KC = ((R == 2 ? 34 : 66) * kS + kSgrThreads - 1) / kSgrThreads;
     const uint32_t n2e = (uint32_t)(N * N * eps);
     const uint32_t scale = ((1u << 20) + n2e / 2) / n2e;
+    const uint32_t thr = ((255u << 20) - (1u << 19) + scale - 1) / scale;
     const int crows = th + 2, ccols = tw + 2, scols = tw + 6;
     const int cr0 = R == 2 ? (y0 & 1) : 0;  // first centre row with y0-1+cr odd
     const int nr = (crows - cr0 + STEP - 1) / STEP;
@@ -273,7 +299,7 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
             s += VS[cr * kS + cc + 2 + dx];
             q += VQ[cr * kS + cc + 2 + dx];
         }
-        ab[i] = ab_of<N>(s, q, bd, scale, xbx1);
+        ab[i] = ab_of<N>(s, q, bd, scale, thr, xbx1);
     }
     __syncthreads();
 #pragma unroll
@@ -300,6 +326,7 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
     const LrPlane &P = *t.P;
     const int bd = P.bit_depth, maxv = (1 << bd) - 1;
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
+    const TileOut o = tile_out(P, x0, y0);
     const int tid = threadIdx.x;
     if (tid < 256) {  // x / (x + 1) table, restoration.rs:330-336
         const int z = tid;
@@ -361,7 +388,7 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
         }
         int v = w0 * f0[i] + w1 * (uu << 4) + (128 - w0 - w1) * f1;
         v = (v + (1 << 10)) >> 11;
-        put(P, (size_t)(y0 + r) * P.out_stride + x0 + c, min(max(v, 0), maxv));
+        put(o, r, c, min(max(v, 0), maxv));
     }
 }
 

@@ -75,8 +75,8 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
     const int last = (int)j.in_width - 1;
     bool bad = false;
     for (int r = 0; r < nr; r++) {
-        const uint32_t y = j.rows ? j.rows[r0 + r] : (uint32_t)(r0 + r);
-        const uint16_t *row = j.in + (size_t)y * j.in_stride;
+        const uint32_t y = j.rows ? glb(j.rows)[r0 + r] : (uint32_t)(r0 + r);
+        gptr<const uint16_t> row = glb(j.in) + (size_t)y * j.in_stride;
         for (int i = tid; i <= hi - lo; i += kSrThreads) {
             const int v = row[min(max(lo + i, 0), last)];
             bad |= v > j.max_sample;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
         for (int t = 0; t < 8; t++)
             f[t] = fp[t];
         for (int r = 0; r < nr; r++) {
-            const uint32_t y = j.rows ? j.rows[r0 + r] : (uint32_t)(r0 + r);
+            const uint32_t y = j.rows ? glb(j.rows)[r0 + r] : (uint32_t)(r0 + r);
             int sum = 0;
 #pragma unroll
             for (int t = 0; t < 8; t++)
@@ -108,9 +108,9 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
             const int v = min(max((sum + 64) >> 7, 0), (int)j.max_sample);
             const size_t o = (size_t)y * j.out_stride + x;
             if (j.out_u8)
-                reinterpret_cast<uint8_t *>(j.out)[o] = (uint8_t)v;
+                reinterpret_cast<gptr<uint8_t>>(glb(j.out))[o] = (uint8_t)v;
             else
-                j.out[o] = (uint16_t)v;
+                glb(j.out)[o] = (uint16_t)v;
         }
     }
 }
