@@ -304,6 +304,74 @@ void vaco_av1pf_parse_lr_params(vaco_av1pf_bit_reader *r, const vaco_av1pf_seq_i
 uint32_t vaco_av1pf_parse_superres_params(vaco_av1pf_bit_reader *r, const vaco_av1pf_seq_info *seq,
                                           uint32_t upscaled_width, uint8_t *use_superres);
 
+/* The AV1 symbol decoder (spec 8.2; crates/signal/vaco-codec-msac/src/
+ * av1.rs:27-190), one per tile: the entropy-coded reader of the per-block
+ * side info below (cdef_idx, restoration units). */
+typedef struct vaco_av1pf_symbol_decoder {
+    vaco_av1pf_bit_reader bits;  /* raw bits of the tile */
+    uint32_t value;              /* SymbolValue */
+    uint32_t range;              /* SymbolRange */
+    int64_t max_bits;            /* SymbolMaxBits; negative once padding is read */
+    uint32_t disable_cdf_update;
+} vaco_av1pf_symbol_decoder;
+
+/* init_symbol(size), av1.rs:48-65. */
+void vaco_av1pf_sd_init(vaco_av1pf_symbol_decoder *sd, const uint8_t *data, size_t size,
+                        int disable_cdf_update);
+/* read_symbol, av1.rs:80-123: cdf holds the n ascending thresholds
+ * (cdf[n-1] == 32768) and the adaptation counter cdf[n]; n >= 2. The cdf
+ * adapts in place unless disable_cdf_update (av1.rs:195-216). */
+uint32_t vaco_av1pf_sd_read_symbol(vaco_av1pf_symbol_decoder *sd, uint16_t *cdf, uint32_t n);
+/* read_bool / read_literal(n) (MSB first, low 32 bits kept), av1.rs:129-143. */
+uint32_t vaco_av1pf_sd_read_bool(vaco_av1pf_symbol_decoder *sd);
+uint32_t vaco_av1pf_sd_read_literal(vaco_av1pf_symbol_decoder *sd, uint32_t n);
+/* exit_symbol, av1.rs:152-157: skip the tile's unread trailing bits. */
+void vaco_av1pf_sd_exit(vaco_av1pf_symbol_decoder *sd);
+/* More than the 14 padding bits consumed, or a raw read past the end
+ * (av1.rs:161-165). */
+int vaco_av1pf_sd_overrun(const vaco_av1pf_symbol_decoder *sd);
+
+/* Per-tile state of the restoration-unit syntax: its three adaptive CDFs
+ * (tables/default_cdf.rs:4725-4731, cdf.rs:128-130) and the reference
+ * coefficients the next unit is coded against (decode.rs:1039-1040). */
+typedef struct vaco_av1pf_lr_tile_state {
+    uint16_t use_wiener[3];
+    uint16_t use_sgrproj[3];
+    uint16_t restoration_type[4];
+    int16_t ref_sgr_xqd[3][2];
+    int16_t ref_lr_wiener[3][2][3];
+} vaco_av1pf_lr_tile_state;
+
+/* The state at the start of a tile. */
+void vaco_av1pf_lr_tile_state_init(vaco_av1pf_lr_tile_state *st);
+
+/* read_restoration_units (decode.rs:1097-1164, with read_restoration_unit,
+ * read_wiener_unit, read_sgrproj_unit and the sub-exponential codes,
+ * 1166-1386) for the superblock at Mi position (mi_row, mi_col) of block
+ * size b_size (BLOCK_64X64 = 12, BLOCK_128X128 = 15): every unit whose
+ * top-left corner the superblock covers is read into units[plane][row *
+ * unit_cols + col] for each plane whose desc->lr_type is not NONE.
+ * desc supplies the geometry (upscaled_width, coded_height), subsampling,
+ * monochrome, lr_type, lr_unit_size and, as each units[] length,
+ * lr_unit_count. Like the reference it assumes one tile and no superres.
+ * Returns VACO_AV1PF_INVALID_DATA with the reference's message in *msg (if
+ * msg is not NULL) where the reference returns InvalidData. */
+vaco_av1pf_status vaco_av1pf_read_lr_units(vaco_av1pf_symbol_decoder *sd,
+                                           vaco_av1pf_lr_tile_state *st,
+                                           const vaco_av1pf_frame_desc *desc, uint32_t mi_row,
+                                           uint32_t mi_col, uint8_t b_size,
+                                           vaco_av1pf_lr_unit *units[3], const char **msg);
+
+/* read_cdef (decode.rs:2087-2145) for a block of size mi_size at Mi
+ * position (mi_row, mi_col): when CDEF is enabled, the block is not skip and
+ * its 64x64 unit's cdef_idx is still -1, reads cdef_bits bits and stores
+ * them in every 64x64 unit the block covers. cdef_idx has cdef_rows rows of
+ * cdef_stride entries (-1 = not read yet); entries outside it are ignored,
+ * as the reference's checked accesses ignore them. */
+void vaco_av1pf_read_cdef(vaco_av1pf_symbol_decoder *sd, int cdef_enabled, uint32_t cdef_bits,
+                          int8_t *cdef_idx, uint32_t cdef_stride, uint32_t cdef_rows,
+                          uint32_t mi_row, uint32_t mi_col, uint8_t mi_size, int skip);
+
 /* ------------------------------------------------------------ graphs --- */
 
 /* No reference counterpart. With graphs enabled, the kernel chain of a

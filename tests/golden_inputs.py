@@ -10,6 +10,8 @@ The inputs are rebuilt exactly as the reference's own tests build them:
 
 The fixture bytes are dav1d (rev aa09a630ef57ee7d9482ffb7ef355a903dbb5302)
 outputs checked into the reference; their SHA-256 digests are pinned below.
+``av1_libaom_symbols.bin`` is the reference's symbol-decoder trace from
+libaom 3.14.1 (crates/signal/vaco-codec-msac/tests/av1_oracle.rs:1-32).
 """
 from __future__ import annotations
 
@@ -29,6 +31,7 @@ SHA256 = {
     "restoration-dav1d-high.u16le": "1f46e3cc1038ee590638381de786f51a63949d5d9d5c762be34f9980c76c6780",
     "restoration-constant-8.u16le": "222e3f56d30ec897fa902cc63b2d21e63b2d7555443b97c85e1af82241f0f56e",
     "restoration-constant-high.u16le": "2861f658c08a5231da898e1a0944524816a66aa95e7011d3037bca0234af8bd2",
+    "av1_libaom_symbols.bin": "c10a382c4619d2838a339bee424ef60505a978424f2cbfa84ccff758bec425f8",
 }
 
 

@@ -138,6 +138,26 @@ class BitReader(C.Structure):
     ]
 
 
+class SymbolDecoder(C.Structure):
+    _fields_ = [
+        ("bits", BitReader),
+        ("value", C.c_uint32),
+        ("range", C.c_uint32),
+        ("max_bits", C.c_int64),
+        ("disable_cdf_update", C.c_uint32),
+    ]
+
+
+class LrTileState(C.Structure):
+    _fields_ = [
+        ("use_wiener", C.c_uint16 * 3),
+        ("use_sgrproj", C.c_uint16 * 3),
+        ("restoration_type", C.c_uint16 * 4),
+        ("ref_sgr_xqd", (C.c_int16 * 2) * 3),
+        ("ref_lr_wiener", ((C.c_int16 * 3) * 2) * 3),
+    ]
+
+
 class SeqInfo(C.Structure):
     _fields_ = [
         ("enable_cdef", C.c_uint8),

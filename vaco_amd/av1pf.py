@@ -130,6 +130,27 @@ def lib():
         L.vaco_av1pf_parse_superres_params.argtypes = [P(abi.BitReader), P(abi.SeqInfo), C.c_uint32,
                                                        P(C.c_uint8)]
         L.vaco_av1pf_parse_superres_params.restype = C.c_uint32
+        L.vaco_av1pf_sd_init.argtypes = [P(abi.SymbolDecoder), C.c_void_p, C.c_size_t, C.c_int]
+        L.vaco_av1pf_sd_init.restype = None
+        L.vaco_av1pf_sd_read_symbol.argtypes = [P(abi.SymbolDecoder), P(C.c_uint16), C.c_uint32]
+        L.vaco_av1pf_sd_read_symbol.restype = C.c_uint32
+        L.vaco_av1pf_sd_read_bool.argtypes = [P(abi.SymbolDecoder)]
+        L.vaco_av1pf_sd_read_bool.restype = C.c_uint32
+        L.vaco_av1pf_sd_read_literal.argtypes = [P(abi.SymbolDecoder), C.c_uint32]
+        L.vaco_av1pf_sd_read_literal.restype = C.c_uint32
+        L.vaco_av1pf_sd_exit.argtypes = [P(abi.SymbolDecoder)]
+        L.vaco_av1pf_sd_exit.restype = None
+        L.vaco_av1pf_sd_overrun.argtypes = [P(abi.SymbolDecoder)]
+        L.vaco_av1pf_sd_overrun.restype = C.c_int
+        L.vaco_av1pf_lr_tile_state_init.argtypes = [P(abi.LrTileState)]
+        L.vaco_av1pf_lr_tile_state_init.restype = None
+        L.vaco_av1pf_read_lr_units.argtypes = [P(abi.SymbolDecoder), P(abi.LrTileState), P(abi.FrameDesc),
+                                               C.c_uint32, C.c_uint32, C.c_uint8, P(P(abi.LrUnit)),
+                                               P(C.c_char_p)]
+        L.vaco_av1pf_read_lr_units.restype = C.c_int
+        L.vaco_av1pf_read_cdef.argtypes = [P(abi.SymbolDecoder), C.c_int, C.c_uint32, P(C.c_int8), C.c_uint32,
+                                           C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint8, C.c_int]
+        L.vaco_av1pf_read_cdef.restype = None
         L.vaco_av1pf_get_tuning.argtypes = [ctxp, P(abi.Tuning)]
         L.vaco_av1pf_set_tuning.argtypes = [ctxp, P(abi.Tuning)]
         L.vaco_av1pf_autotune.argtypes = [ctxp, C.c_uint32, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture),
@@ -576,6 +597,71 @@ def parse_superres_params(bits: Bits, seq: abi.SeqInfo, upscaled_width: int):
     use = C.c_uint8()
     w = lib().vaco_av1pf_parse_superres_params(C.byref(bits.r), C.byref(seq), upscaled_width, C.byref(use))
     return w, bool(use.value)
+
+
+class SymbolDecoder:
+    """The AV1 symbol decoder over one tile's bytes (vaco_av1pf_symbol_decoder)."""
+
+    def __init__(self, data: bytes, disable_cdf_update: bool):
+        self._buf = (C.c_uint8 * max(1, len(data))).from_buffer_copy(bytes(data) or b"\0")
+        self.sd = abi.SymbolDecoder()
+        lib().vaco_av1pf_sd_init(C.byref(self.sd), C.cast(self._buf, C.c_void_p) if data else None, len(data),
+                                 int(disable_cdf_update))
+
+    def read_symbol(self, cdf) -> int:
+        """cdf: a ctypes uint16 array of n + 1 entries, adapted in place."""
+        return lib().vaco_av1pf_sd_read_symbol(C.byref(self.sd), cdf, len(cdf) - 1)
+
+    def read_bool(self) -> int:
+        return lib().vaco_av1pf_sd_read_bool(C.byref(self.sd))
+
+    def read_literal(self, n: int) -> int:
+        return lib().vaco_av1pf_sd_read_literal(C.byref(self.sd), n)
+
+    def exit(self) -> None:
+        lib().vaco_av1pf_sd_exit(C.byref(self.sd))
+
+    @property
+    def overrun(self) -> bool:
+        return bool(lib().vaco_av1pf_sd_overrun(C.byref(self.sd)))
+
+    @property
+    def max_bits(self) -> int:
+        return self.sd.max_bits
+
+
+def cdf_array(values) -> C.Array:
+    """A cdf for SymbolDecoder.read_symbol: the thresholds then the counter."""
+    return (C.c_uint16 * len(values))(*values)
+
+
+def lr_tile_state() -> abi.LrTileState:
+    st = abi.LrTileState()
+    lib().vaco_av1pf_lr_tile_state_init(C.byref(st))
+    return st
+
+
+def read_lr_units(sd: SymbolDecoder, st: abi.LrTileState, desc: abi.FrameDesc, mi_row: int, mi_col: int,
+                  b_size: int, units) -> None:
+    """vaco_av1pf_read_lr_units; units: three ctypes LrUnit arrays (or None),
+    filled in place. Raises Av1Error with the reference's message."""
+    ptrs = (C.POINTER(abi.LrUnit) * 3)(*[C.cast(u, C.POINTER(abi.LrUnit)) if u is not None else None
+                                        for u in units])
+    msg = C.c_char_p()
+    rc = lib().vaco_av1pf_read_lr_units(C.byref(sd.sd), C.byref(st), C.byref(desc), mi_row, mi_col, b_size,
+                                         ptrs, C.byref(msg))
+    if rc:
+        raise Av1Error(rc, (msg.value or b"").decode())
+
+
+def read_cdef(sd: SymbolDecoder, enabled: bool, cdef_bits: int, cdef_idx: np.ndarray, mi_row: int, mi_col: int,
+              mi_size: int, skip: bool) -> None:
+    """vaco_av1pf_read_cdef on an int8 (rows, stride) map, in place."""
+    assert cdef_idx.dtype == np.int8 and cdef_idx.flags.c_contiguous
+    rows, stride = cdef_idx.shape
+    lib().vaco_av1pf_read_cdef(C.byref(sd.sd), int(enabled), cdef_bits,
+                               cdef_idx.ctypes.data_as(C.POINTER(C.c_int8)), stride, rows, mi_row, mi_col,
+                               mi_size, int(skip))
 
 
 # --------------------------------------------------------- host helpers --
