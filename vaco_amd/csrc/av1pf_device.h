@@ -118,6 +118,7 @@ struct LrPlane {
     uint8_t bit_depth, ss_y;
     uint8_t vec_ok;                    // before/after 8-byte aligned, strides multiples of 4
     uint8_t out_u8;                    // out holds one byte per sample (8-bit frame output)
+    uint8_t out_vec;                   // out 8-byte aligned, out_stride a multiple of 4
 };
 
 }  // namespace av1pf

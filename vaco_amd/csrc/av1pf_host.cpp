@@ -827,6 +827,7 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             L.vec_ok = !((uintptr_t)L.before & 7) && !((uintptr_t)L.after & 7) &&
                        !(L.before_stride & 3) && !(L.after_stride & 3);
             L.out_u8 = out[p].u8;  // restored planes lie inside the frame planes
+            L.out_vec = !((uintptr_t)L.out & 7) && !(L.out_stride & 3);
             plan.lr_blocks += L.n_rows * L.n_cols;
             const uint32_t item = (uint32_t)plan.lr.size();
             for (uint32_t ri = 0; ri < L.n_rows; ri++)

@@ -157,6 +157,30 @@ __device__ __forceinline__ void lr_copy(const Tile &t)
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
     const TileOut o = tile_out(P, x0, y0);
     bool bad = false;
+    if (P.vec_ok && P.out_vec && tw == 64 && (x0 & 3) == 0) {
+        // Four samples per load and store, every group 8-byte aligned in
+        // both planes.
+#pragma unroll 2
+        for (int e = threadIdx.x; e < 64 * 16; e += kLrThreads) {
+            const int r = e >> 4, c = (e & 15) * 4;
+            if (r >= th)
+                break;
+            const uint64_t w = *reinterpret_cast<gptr<const uint64_t>>(
+                glb(P.after) + (size_t)(y0 + r) * P.after_stride + x0 + c);
+            const uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+            const uint32_t m = max(max(lo & 0xffffu, lo >> 16), max(hi & 0xffffu, hi >> 16));
+            bad |= (int)m > maxv;
+            const uint32_t i = __umul24((uint32_t)r, o.stride) + (uint32_t)c;
+            if (o.u8)  // samples <= 255 when none is out of range
+                *reinterpret_cast<gptr<uint32_t>>(o.base8 + i) =
+                    (lo & 0xffu) | ((lo >> 8) & 0xff00u) | ((hi & 0xffu) << 16) | ((hi >> 16) << 24);
+            else
+                *reinterpret_cast<gptr<uint64_t>>(o.base16 + i) = w;
+        }
+        if (__syncthreads_or(bad) && threadIdx.x == 0)
+            raise_status(P.status, kFlagSampleRange);
+        return;
+    }
 #pragma unroll 4
     for (int e = threadIdx.x; e < 64 * 64; e += kLrThreads) {
         const int r = e >> 6, c = e & 63;
@@ -198,25 +222,61 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
     const int v0 = u.vertical[0], v1 = u.vertical[1], v2 = u.vertical[2];
     const int v3 = 128 - 2 * (v0 + v1 + v2);
     const int srows = th + 6;
-#pragma unroll 4
-    for (int e = tid; e < srows * 64; e += kLrThreads) {
-        const int r = e >> 6, c = e & 63;
+    // Rows: four adjacent outputs per item from ten staged samples (two
+    // 8-byte and one 4-byte LDS read), stored as one 8-byte M write.
+    for (int e = tid; e < srows * 16; e += kLrThreads) {
+        const int r = e >> 4, c = (e & 15) * 4;
         const uint16_t *s = &S[r * kS + c];
-        int h = h0 * (s[0] + s[6]) + h1 * (s[1] + s[5]) + h2 * (s[2] + s[4]) + h3 * s[3];
-        h = round2i(h, round_h);
-        M[r * 64 + c] = (int16_t)min(max(h, -offset), limit - offset);
+        const uint2 a = *reinterpret_cast<const uint2 *>(s);
+        const uint2 b = *reinterpret_cast<const uint2 *>(s + 4);
+        const uint32_t z = *reinterpret_cast<const uint32_t *>(s + 8);
+        const int q[10] = {(int)(a.x & 0xffffu), (int)(a.x >> 16), (int)(a.y & 0xffffu), (int)(a.y >> 16),
+                           (int)(b.x & 0xffffu), (int)(b.x >> 16), (int)(b.y & 0xffffu), (int)(b.y >> 16),
+                           (int)(z & 0xffffu),   (int)(z >> 16)};
+        uint32_t m[2];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int h = h0 * (q[j] + q[j + 6]) + h1 * (q[j + 1] + q[j + 5]) + h2 * (q[j + 2] + q[j + 4]) +
+                    h3 * q[j + 3];
+            h = round2i(h, round_h);
+            const uint32_t v = (uint16_t)(int16_t)min(max(h, -offset), limit - offset);
+            if (j & 1)
+                m[j >> 1] |= v << 16;
+            else
+                m[j >> 1] = v;
+        }
+        *reinterpret_cast<uint2 *>(&M[r * 64 + c]) = make_uint2(m[0], m[1]);
     }
     __syncthreads();
-#pragma unroll 4
-    for (int e = tid; e < 64 * 64; e += kLrThreads) {
-        const int r = e >> 6, c = e & 63;
-        if (r >= th || c >= tw)
-            continue;
-        const int16_t *m = &M[r * 64 + c];
-        int s = v0 * (m[0] + m[6 * 64]) + v1 * (m[64] + m[5 * 64]) + v2 * (m[2 * 64] + m[4 * 64]) +
-                v3 * m[3 * 64];
-        s = round2i(s, 14 - round_h);
-        put(o, r, c, min(max(s, 0), maxv));
+    // Columns: a 4-row x 2-column block per thread (16 x 32 blocks cover the
+    // tile) from ten 4-byte M reads, each holding both columns.
+    {
+        const int cb = tid & 31, rb = tid >> 5;
+        const int c = 2 * cb, r0 = 4 * rb;
+        int lo[10], hi[10];
+#pragma unroll
+        for (int k = 0; k < 10; k++) {
+            const uint32_t w = *reinterpret_cast<const uint32_t *>(&M[(r0 + k) * 64 + c]);
+            lo[k] = (int16_t)(w & 0xffffu);
+            hi[k] = (int16_t)(w >> 16);
+        }
+        if (c < tw) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int r = r0 + j;
+                if (r >= th)
+                    break;
+                int s0 = v0 * (lo[j] + lo[j + 6]) + v1 * (lo[j + 1] + lo[j + 5]) +
+                         v2 * (lo[j + 2] + lo[j + 4]) + v3 * lo[j + 3];
+                int s1 = v0 * (hi[j] + hi[j + 6]) + v1 * (hi[j + 1] + hi[j + 5]) +
+                         v2 * (hi[j + 2] + hi[j + 4]) + v3 * hi[j + 3];
+                s0 = min(max(round2i(s0, 14 - round_h), 0), maxv);
+                s1 = min(max(round2i(s1, 14 - round_h), 0), maxv);
+                put(o, r, c, s0);
+                if (c + 1 < tw)
+                    put(o, r, c + 1, s1);
+            }
+        }
     }
 }
 
