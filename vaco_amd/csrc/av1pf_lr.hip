@@ -283,7 +283,6 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
 // --------------------------------------------------------- self-guided --
 
 constexpr int kSgrThreads = kLrThreads;
-constexpr int kSgrPx = 64 * 64 / kSgrThreads;                          // per thread
 
 // A/B of one box from its sums (restoration.rs:314-339), packed as
 // b2 << 9 | a2 (a2 <= 256; b2 < 2^21 for 8/10/12-bit samples), in exact
@@ -295,13 +294,13 @@ constexpr int kSgrPx = 64 * 64 / kSgrThreads;                          // per th
 //  - (256 - a2) * recip <= 255 * 455 < 2^24 and sum <= 25 * 4095 < 2^24, and
 //    their product plus 2^11 stays below 2^32 (largest case 255 * 164 *
 //    102375 + 2^11 = 4281324548).
-template <int N>
-__device__ __forceinline__ uint32_t ab_of(int sum, int sumsq, int bd, uint32_t scale, uint32_t thr,
+template <int N, int BD>
+__device__ __forceinline__ uint32_t ab_of(int sum, int sumsq, uint32_t scale, uint32_t thr,
                                           const uint16_t *xbx1)
 {
     constexpr uint32_t recip = ((1u << 12) + N / 2) / N;
-    const uint32_t a = (uint32_t)round2i(sumsq, 2 * (bd - 8));
-    const uint32_t d = (uint32_t)round2i(sum, bd - 8);
+    const uint32_t a = (uint32_t)round2i(sumsq, 2 * (BD - 8));
+    const uint32_t d = (uint32_t)round2i(sum, BD - 8);
     const int p = max((int)__umul24(a, (uint32_t)N) - (int)__umul24(d, d), 0);
     const uint32_t z = (min((uint32_t)p, thr) * scale + (1u << 19)) >> 20;
     const uint32_t a2 = z >= 255 ? 256u : xbx1[z];
@@ -313,9 +312,9 @@ __device__ __forceinline__ uint32_t ab_of(int sum, int sumsq, int bd, uint32_t s
 // x0-1+cc), cr < th+2, cc < tw+2, the packed A/B, written over VS/VQ as
 // grid AB (row stride kG). Pass 0 (R = 2) needs odd rows y only
 // (restoration.rs:347-356), so it walks only those centre rows.
-template <int R>
+template <int R, int BD>
 __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th, int tw, int y0,
-                                         int bd, int eps, const uint16_t *xbx1)
+                                         int eps, const uint16_t *xbx1)
 {
     constexpr int N = (2 * R + 1) * (2 * R + 1);
     constexpr int STEP = R == 2 ? 2 : 1;  // centre rows walked
@@ -359,7 +358,7 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
             s += VS[cr * kS + cc + 2 + dx];
             q += VQ[cr * kS + cc + 2 + dx];
         }
-        ab[i] = ab_of<N>(s, q, bd, scale, thr, xbx1);
+        ab[i] = ab_of<N, BD>(s, q, scale, thr, xbx1);
     }
     __syncthreads();
 #pragma unroll
@@ -374,6 +373,59 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
 
 __device__ __forceinline__ int ab_a(uint32_t v) { return (int)(v & 511u); }
 __device__ __forceinline__ int ab_b(uint32_t v) { return (int)(v >> 9); }
+
+// guided_pass (restoration.rs:341-371), pass 0 (radius 2), for the 8 rows
+// rs .. rs+7 of column c. Grid row j (tile row rs + j - 1) holds A/B only
+// for odd plane rows: where (PAR + j) is even, PAR the parity of y0 + rs.
+// An odd output row takes its own grid row with weights 5 6 5 and shift 8;
+// an even one the odd rows above and below, shift 9.
+template <int PAR>
+__device__ __forceinline__ void sgr_pass0_strip(const uint32_t *X, int c, int rs, const int *uu, int *f0)
+{
+    int ta[10], tb[10];
+#pragma unroll
+    for (int j = 0; j < 10; j++) {
+        if ((PAR + j) & 1)
+            continue;
+        const int k = (rs + j) * kG + c;
+        const uint32_t p = X[k], q = X[k + 1], w = X[k + 2];
+        ta[j] = 5 * (ab_a(p) + ab_a(w)) + 6 * ab_a(q);
+        tb[j] = 5 * (ab_b(p) + ab_b(w)) + 6 * ab_b(q);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        if ((PAR + i) & 1)
+            f0[i] = (ta[i + 1] * uu[i] + tb[i + 1] + (1 << 7)) >> 8;
+        else
+            f0[i] = ((ta[i] + ta[i + 2]) * uu[i] + tb[i] + tb[i + 2] + (1 << 8)) >> 9;
+    }
+}
+
+// Pass 1 (radius 1) for the same strip: weights 4 on the cross, 3 on the
+// diagonals, shift 9. Per grid row, T = 3 * (left + right) + 4 * centre
+// (the row above or below) and U = 4 * (left + centre + right) (the own
+// row); output row i sums T(i) + U(i + 1) + T(i + 2).
+__device__ __forceinline__ void sgr_pass1_strip(const uint32_t *X, int c, int rs, const int *uu, int *f1)
+{
+    int ta[3], tb[3], ua[3], ub[3];  // rows j - 2, j - 1, j, rotating
+#pragma unroll
+    for (int j = 0; j < 10; j++) {
+        const int k = (rs + j) * kG + c;
+        const uint32_t p = X[k], q = X[k + 1], w = X[k + 2];
+        const int sa = ab_a(p) + ab_a(w), sb = ab_b(p) + ab_b(w);
+        const int qa = ab_a(q), qb = ab_b(q);
+        ta[j % 3] = 3 * sa + 4 * qa;
+        tb[j % 3] = 3 * sb + 4 * qb;
+        ua[j % 3] = 4 * (sa + qa);
+        ub[j % 3] = 4 * (sb + qb);
+        if (j >= 2) {
+            const int i = j - 2;
+            const int a = ta[i % 3] + ua[(i + 1) % 3] + ta[j % 3];
+            const int b = tb[i % 3] + ub[(i + 1) % 3] + tb[j % 3];
+            f1[i] = (a * uu[i] + b + (1 << 8)) >> 9;
+        }
+    }
+}
 
 // LDS: S (staged source), X (VS u16 + VQ, then the A/B grid), xbx1.
 constexpr int kSgrLds = 70 * kS * 2 + 66 * kS * 3 / 2 * 4 + 256 * 2;
@@ -398,57 +450,49 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
     // self_guided, restoration.rs:373-384, with guided_pass 341-371.
     const int set = t.u.set & 15;
     const int r0 = kSgr[set][0], e0 = kSgr[set][1], r1 = kSgr[set][2], e1 = kSgr[set][3];
-    int f0[kSgrPx];
-    if (r0)
-        sgr_grid<2>(S, X, th, tw, y0, bd, e0, xbx1);
+    // Thread = column c and a strip of 8 output rows rs .. rs + 7 (its wave),
+    // so each A/B grid row the strip needs is read once for all 8 rows.
+    const int c = tid & 63, rs = (tid >> 6) * 8;
+    int uu[8], f0[8], f1[8];
 #pragma unroll
-    for (int i = 0; i < kSgrPx; i++) {
-        const int e = tid + i * kSgrThreads, r = e >> 6, c = e & 63;
-        const int uu = S[(r + 3) * kS + c + 3];
-        if (!r0 || r >= th || c >= tw) {
-            f0[i] = uu << 4;
-        } else if ((y0 + r) & 1) {  // odd row: its own row, weights 5 6 5, shift 4
-            const int k = (r + 1) * kG + c;
-            const uint32_t p = X[k], q = X[k + 1], w = X[k + 2];
-            const int a = 5 * (ab_a(p) + ab_a(w)) + 6 * ab_a(q);
-            const int b = 5 * (ab_b(p) + ab_b(w)) + 6 * ab_b(q);
-            f0[i] = (a * uu + b + (1 << 7)) >> 8;
-        } else {  // even row: the odd rows above and below, shift 5
-            const int k = r * kG + c, k2 = k + 2 * kG;
-            const uint32_t p = X[k], q = X[k + 1], w = X[k + 2];
-            const uint32_t p2 = X[k2], q2 = X[k2 + 1], w2 = X[k2 + 2];
-            const int a = 5 * (ab_a(p) + ab_a(w) + ab_a(p2) + ab_a(w2)) + 6 * (ab_a(q) + ab_a(q2));
-            const int b = 5 * (ab_b(p) + ab_b(w) + ab_b(p2) + ab_b(w2)) + 6 * (ab_b(q) + ab_b(q2));
-            f0[i] = (a * uu + b + (1 << 8)) >> 9;
-        }
+    for (int i = 0; i < 8; i++) {
+        uu[i] = S[(rs + i + 3) * kS + c + 3];
+        f0[i] = f1[i] = uu[i] << 4;
+    }
+    if (r0) {
+        if (bd == 8)
+            sgr_grid<2, 8>(S, X, th, tw, y0, e0, xbx1);
+        else if (bd == 10)
+            sgr_grid<2, 10>(S, X, th, tw, y0, e0, xbx1);
+        else
+            sgr_grid<2, 12>(S, X, th, tw, y0, e0, xbx1);
+        if ((y0 + rs) & 1)
+            sgr_pass0_strip<1>(X, c, rs, uu, f0);
+        else
+            sgr_pass0_strip<0>(X, c, rs, uu, f0);
     }
     if (r1) {
         if (r0)
             __syncthreads();  // the pass-0 grid is fully consumed
-        sgr_grid<1>(S, X, th, tw, y0, bd, e1, xbx1);
+        if (bd == 8)
+            sgr_grid<1, 8>(S, X, th, tw, y0, e1, xbx1);
+        else if (bd == 10)
+            sgr_grid<1, 10>(S, X, th, tw, y0, e1, xbx1);
+        else
+            sgr_grid<1, 12>(S, X, th, tw, y0, e1, xbx1);
+        sgr_pass1_strip(X, c, rs, uu, f1);
     }
     const int w0 = t.u.xqd[0], w1 = t.u.xqd[1];
+    if (c < tw) {
 #pragma unroll
-    for (int i = 0; i < kSgrPx; i++) {
-        const int e = tid + i * kSgrThreads, r = e >> 6, c = e & 63;
-        if (r >= th || c >= tw)
-            continue;
-        const int uu = S[(r + 3) * kS + c + 3];
-        int f1 = uu << 4;
-        if (r1) {  // weights 4 on the cross, 3 on the diagonals, shift 5
-            const int k = r * kG + c, k1 = k + kG, k2 = k + 2 * kG;
-            const uint32_t n0 = X[k], n1 = X[k + 1], n2 = X[k + 2];
-            const uint32_t m0 = X[k1], m1 = X[k1 + 1], m2 = X[k1 + 2];
-            const uint32_t s0 = X[k2], s1 = X[k2 + 1], s2 = X[k2 + 2];
-            const int a = 4 * (ab_a(n1) + ab_a(m0) + ab_a(m1) + ab_a(m2) + ab_a(s1)) +
-                          3 * (ab_a(n0) + ab_a(n2) + ab_a(s0) + ab_a(s2));
-            const int b = 4 * (ab_b(n1) + ab_b(m0) + ab_b(m1) + ab_b(m2) + ab_b(s1)) +
-                          3 * (ab_b(n0) + ab_b(n2) + ab_b(s0) + ab_b(s2));
-            f1 = (a * uu + b + (1 << 8)) >> 9;
+        for (int i = 0; i < 8; i++) {
+            const int r = rs + i;
+            if (r >= th)
+                break;
+            int v = w0 * f0[i] + w1 * (uu[i] << 4) + (128 - w0 - w1) * f1[i];
+            v = (v + (1 << 10)) >> 11;
+            put(o, r, c, min(max(v, 0), maxv));
         }
-        int v = w0 * f0[i] + w1 * (uu << 4) + (128 - w0 - w1) * f1;
-        v = (v + (1 << 10)) >> 11;
-        put(o, r, c, min(max(v, 0), maxv));
     }
 }
 
