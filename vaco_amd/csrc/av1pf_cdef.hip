@@ -393,15 +393,54 @@ __device__ __forceinline__ void store_pair(gptr<uint16_t> plane, size_t i, s16x2
 // touches a plane edge (unavailable taps; partial superblocks store only
 // samples inside the plane; plane widths are even, so a pair is all in or
 // all out).
+// The descriptor fields the phases after staging use, read at the start of
+// the workgroup so that their load latency hides under the staging loads.
+struct SbFields {
+    gptr<uint16_t> out[3];
+    uint32_t out_stride[3];
+    int crop_w[3], crop_h[3];
+    uint32_t str[8];  // y_pri, y_sec, uv_pri, uv_sec: 8 presets each, 4 per word
+    int damping, nplanes;
+    bool out_u8;
+
+    __device__ __forceinline__ void load(const CdefFrame &f)
+    {
+        static_assert(offsetof(CdefFrame, y_pri) % 4 == 0 && offsetof(CdefFrame, y_sec) ==
+                      offsetof(CdefFrame, y_pri) + 8 && offsetof(CdefFrame, uv_sec) ==
+                      offsetof(CdefFrame, y_pri) + 24, "strength tables are read as words");
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(f.y_pri);
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            str[i] = w[i];
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            out[p] = glb(f.out[p]);
+            out_stride[p] = f.out_stride[p];
+            crop_w[p] = (int)f.crop_w[p];
+            crop_h[p] = (int)f.crop_h[p];
+        }
+        damping = f.damping;
+        nplanes = f.nplanes;
+        out_u8 = f.out_u8 != 0;
+    }
+
+    // Preset ix of table t (0 y_pri, 1 y_sec, 2 uv_pri, 3 uv_sec).
+    __device__ __forceinline__ uint32_t strength(int t, int ix) const
+    {
+        const uint32_t w = (ix & 4) ? str[2 * t + 1] : str[2 * t];
+        return (w >> (8 * (ix & 3))) & 255u;
+    }
+};
+
 template <bool BORDER, int SX, int SY>
-__device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*lt)[kTile],
+__device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t (*lt)[kTile],
                                           const Blk (*blk)[64], const uint8_t *bact, bool any,
                                           int x0, int y0, int ix)
 {
     // Strengths are per superblock preset (the luma primary is only ever
     // lowered per block, never raised from zero).
-    const bool ypri = f.y_pri[ix] != 0, ysec = f.y_sec[ix] != 0;
-    const bool cpri = f.uv_pri[ix] != 0, csec = f.uv_sec[ix] != 0;
+    const bool ypri = f.strength(0, ix) != 0, ysec = f.strength(1, ix) != 0;
+    const bool cpri = f.strength(2, ix) != 0, csec = f.strength(3, ix) != 0;
     constexpr int BW = 8 >> SX, BH = 8 >> SY, PPP = BW * BH / 2, CPX = BW * BH / 16;
     const int tid = threadIdx.x;
     const int b = tid >> 4, q = tid & 15;
@@ -413,7 +452,7 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
         if (act)
             B = blk[0][b];
         const uint16_t *t = lt[0];
-        gptr<uint16_t> dst = glb(f.out[0]);
+        gptr<uint16_t> dst = f.out[0];
         const int ost = f.out_stride[0], W0 = f.crop_w[0], H0 = f.crop_h[0];
         const int ci0 = (2 + bi * 8 + i0) * kRow + 8 + bj * 8 + j;
         s16x2 r2[2];
@@ -451,7 +490,9 @@ __device__ __forceinline__ void filter_sb(const CdefFrame &f, const uint16_t (*l
             else
                 res = ld2(t, ci);
             if (!BORDER || (x < W1 && y < H1))
-                store_pair(glb(f.out[p]), (size_t)y * f.out_stride[p] + x, res, f.out_u8, x + 1 < W1);
+                store_pair(p == 1 ? f.out[1] : f.out[2],
+                           (size_t)y * (p == 1 ? f.out_stride[1] : f.out_stride[2]) + x, res, f.out_u8,
+                           x + 1 < W1);
         }
     }
 }
@@ -478,6 +519,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     const int x0 = sbx * 64, y0 = sby * 64;
     const int W0 = f.width[0], H0 = f.height[0];
     const int idx = glb(f.cdef_idx)[sby * f.idx_stride + sbx];
+    SbFields sf;
+    sf.load(f);
     const bool border = x0 == 0 || y0 == 0 || x0 + 64 >= W0 || y0 + 64 >= H0;
 
     if (tid < 64) {  // decode.rs:774-777, 892-896
@@ -544,17 +587,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
                 const uint32_t var = (bc - cost[(best + 4) & 7][b]) >> 10;
                 const int ix = idx & 7;
                 if (cls == 0) {  // decode.rs:806-818
-                    const uint32_t pri = (uint32_t)f.y_pri[ix] << shift;
+                    const uint32_t pri = sf.strength(0, ix) << shift;
                     uint32_t adj = 0;
                     if (var != 0) {  // adjust_strength, cdef.rs:208-220
                         const uint32_t sc = var >> 6;
                         const uint32_t lg = sc ? min(ilog2u(sc), 12) : 0;
                         adj = (pri * (4 + lg) + 8) >> 4;
                     }
-                    B = make_blk(pri ? best : 0, adj, (uint32_t)f.y_sec[ix] << shift,
-                                 (uint8_t)(f.damping + shift), shift);
+                    B = make_blk(pri ? best : 0, adj, sf.strength(1, ix) << shift,
+                                 (uint8_t)(sf.damping + shift), shift);
                 } else {
-                    const uint32_t pri = (uint32_t)f.uv_pri[ix] << shift;
+                    const uint32_t pri = sf.strength(2, ix) << shift;
                     // chroma_direction, cdef.rs:222-230: nibble `best` of
                     // {7,0,2,4,5,6,6,6} (4:2:2) or {1,2,2,2,3,4,6,0} (4:4:0).
                     int cdir = best;
@@ -562,8 +605,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
                         cdir = (0x66654207u >> (4 * best)) & 7;
                     else if (!SX && SY)
                         cdir = (0x06432221u >> (4 * best)) & 7;
-                    B = make_blk(pri ? cdir : 0, pri, (uint32_t)f.uv_sec[ix] << shift,
-                                 (uint8_t)(f.damping + shift - 1), shift);
+                    B = make_blk(pri ? cdir : 0, pri, sf.strength(3, ix) << shift,
+                                 (uint8_t)(sf.damping + shift - 1), shift);
                 }
             }
             blk[cls][b] = B;
@@ -573,9 +616,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
 
     const bool any = idx >= 0;
     if (border)
-        filter_sb<true, SX, SY>(f, lt, blk, bact, any, x0, y0, idx & 7);
+        filter_sb<true, SX, SY>(sf, lt, blk, bact, any, x0, y0, idx & 7);
     else
-        filter_sb<false, SX, SY>(f, lt, blk, bact, any, x0, y0, idx & 7);
+        filter_sb<false, SX, SY>(sf, lt, blk, bact, any, x0, y0, idx & 7);
 }
 
 // Kernel-level cases (cdef::find_direction + cdef::filter_block on one
