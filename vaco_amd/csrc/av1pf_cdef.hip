@@ -207,8 +207,6 @@ __device__ __forceinline__ int filter_px(const uint16_t *t, int ci, const Blk &B
 // block. Every quantity fits int16: samples <= 4095, differences within
 // +-4095, and the tap sum is at most 2*(4+2)*240 + 4*(2+1)*64 = 3648, so
 // each operation works on both samples at once (v_pk_*_i16).
-typedef short s16x2 __attribute__((ext_vector_type(2)));
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 // Samples i and i + 1 of a tile as one aligned 32-bit LDS read (i even).
 __device__ __forceinline__ s16x2 ld2(const uint16_t *t, int i)

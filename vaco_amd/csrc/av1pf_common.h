@@ -8,6 +8,10 @@
 
 namespace av1pf {
 
+// Two 16-bit lanes of one 32-bit register (packed v_pk_* / v_dot2 operands).
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ int ilog2u(uint32_t v) { return 31 - __clz(v); }
 
 // Pointers into device memory. A pointer read out of a work-item struct is

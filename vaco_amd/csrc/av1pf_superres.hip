@@ -59,8 +59,12 @@ constexpr int kSrWin = kSrSeg + 16;          // staged source samples per row
 template <int R>
 __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict__ jobs, int n_jobs)
 {
+    // win: each row's window; wsh: the same shifted by one sample (window
+    // index i at i + 1), so the eight taps from any position are four
+    // aligned sample pairs of one of the two copies.
     __shared__ __align__(16) int16_t tab[64 * 8];
-    __shared__ uint16_t win[R][kSrWin];
+    __shared__ __align__(16) uint16_t win[R][kSrWin];
+    __shared__ __align__(16) uint16_t wsh[R][kSrWin + 2];
     const SrJob &j = find_item(jobs, n_jobs, blockIdx.x);
     const uint32_t local = blockIdx.x - j.block_begin;
     const uint32_t rb = local / j.segs;
@@ -74,6 +78,12 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
     const int hi = ((-(1 << 14) + j.start + xn * j.step) >> 14) + 4;
     const int last = (int)j.in_width - 1;
     bool bad = false;
+    size_t orow[R];  // output offset of each row
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t y = j.rows ? glb(j.rows)[r0 + min(r, nr - 1)] : (uint32_t)(r0 + r);
+        orow[r] = (size_t)y * j.out_stride;
+    }
     for (int r = 0; r < nr; r++) {
         const uint32_t y = j.rows ? glb(j.rows)[r0 + r] : (uint32_t)(r0 + r);
         gptr<const uint16_t> row = glb(j.in) + (size_t)y * j.in_stride;
@@ -81,12 +91,17 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
             const int v = row[min(max(lo + i, 0), last)];
             bad |= v > j.max_sample;
             win[r][i] = (uint16_t)v;
+            wsh[r][i + 1] = (uint16_t)v;
         }
     }
     for (int i = tid; i < 64 * 8; i += kSrThreads)
         tab[i] = kUpscale[i >> 3][i & 7];
     if (__syncthreads_or(bad) && tid == 0)
         raise_status(j.status, kFlagSampleRange);
+    gptr<uint16_t> out16 = glb(j.out);
+    gptr<uint8_t> out8 = reinterpret_cast<gptr<uint8_t>>(out16);
+    const bool u8 = j.out_u8 != 0;
+    const int maxv = j.max_sample;
 #pragma unroll
     for (int k = 0; k < kSrPer; k++) {
         const int x = x0 + tid + k * kSrThreads;
@@ -94,23 +109,26 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
             break;
         const int src_x = -(1 << 14) + j.start + x * j.step;
         const int base = (src_x >> 14) - 3 - lo;
-        const int16_t *fp = &tab[((src_x & 0x3fff) >> 8) * 8];
-        int f[8];
+        // Taps as pairs, (0,1) (2,3) (4,5) (6,7), for v_dot2 over sample pairs.
+        const uint4 fq = *reinterpret_cast<const uint4 *>(&tab[((src_x & 0x3fff) >> 8) * 8]);
+        const s16x2 f0 = __builtin_bit_cast(s16x2, fq.x), f1 = __builtin_bit_cast(s16x2, fq.y);
+        const s16x2 f2 = __builtin_bit_cast(s16x2, fq.z), f3 = __builtin_bit_cast(s16x2, fq.w);
+        const int odd = base & 1;
 #pragma unroll
-        for (int t = 0; t < 8; t++)
-            f[t] = fp[t];
-        for (int r = 0; r < nr; r++) {
-            const uint32_t y = j.rows ? glb(j.rows)[r0 + r] : (uint32_t)(r0 + r);
-            int sum = 0;
-#pragma unroll
-            for (int t = 0; t < 8; t++)
-                sum += win[r][base + t] * f[t];
-            const int v = min(max((sum + 64) >> 7, 0), (int)j.max_sample);
-            const size_t o = (size_t)y * j.out_stride + x;
-            if (j.out_u8)
-                reinterpret_cast<gptr<uint8_t>>(glb(j.out))[o] = (uint8_t)v;
+        for (int r = 0; r < R; r++) {
+            if (r >= nr)
+                break;
+            const uint32_t *w = reinterpret_cast<const uint32_t *>(odd ? &wsh[r][base + 1] : &win[r][base]);
+            int sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[0]), f0, 0, false);
+            sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[1]), f1, sum, false);
+            sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[2]), f2, sum, false);
+            sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[3]), f3, sum, false);
+            const int v = min(max((sum + 64) >> 7, 0), maxv);
+            const size_t o = orow[r] + x;
+            if (u8)
+                out8[o] = (uint8_t)v;
             else
-                glb(j.out)[o] = (uint16_t)v;
+                out16[o] = (uint16_t)v;
         }
     }
 }
