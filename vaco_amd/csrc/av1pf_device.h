@@ -75,7 +75,7 @@ struct SrJob {
     uint32_t block_begin;
     uint16_t max_sample;
     uint8_t out_u8;                    // out holds one byte per sample (8-bit frame output)
-    uint8_t pad;
+    uint8_t vec_ok;                    // in 8-byte aligned, in_stride a multiple of 4
 };
 
 // -------------------------------------------------------- restoration --

@@ -754,6 +754,7 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
                 j.block_begin = plan.sr_blocks;
                 j.max_sample = (uint16_t)((1u << d.bit_depth) - 1);
                 j.out_u8 = dst.u8;
+                j.vec_ok = !((uintptr_t)j.in & 7) && !(j.in_stride & 3);
                 plan.sr_blocks += sr_job_blocks(j.n_rows, j.segs, plan.tune.superres_rows);
                 plan.bytes[VACO_AV1PF_KERNEL_SUPERRES] +=
                     (uint64_t)j.n_rows * (g.sr_vis[p] + g.sr_out_w[p]) * 2;

@@ -77,21 +77,66 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
     const int lo = ((-(1 << 14) + j.start + x0 * j.step) >> 14) - 3;
     const int hi = ((-(1 << 14) + j.start + xn * j.step) >> 14) + 4;
     const int last = (int)j.in_width - 1;
+    // Window origin: lo, or lo rounded down to a multiple of 4 where the
+    // window lies inside the row with room for whole 4-sample groups, which
+    // are then loaded 8 bytes at a time. Only samples lo .. hi are checked
+    // against the bit depth (the taps read no others).
+    const int groups = (hi - (lo & ~3)) / 4 + 1;
+    const bool vec = j.vec_ok && lo >= 0 && (lo & ~3) + 4 * groups - 1 <= last;
+    const int org = vec ? (lo & ~3) : lo;
+    const int maxv = j.max_sample;
     bool bad = false;
     size_t orow[R];  // output offset of each row
+    uint32_t yrow[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const uint32_t y = j.rows ? glb(j.rows)[r0 + min(r, nr - 1)] : (uint32_t)(r0 + r);
-        orow[r] = (size_t)y * j.out_stride;
+        yrow[r] = j.rows ? glb(j.rows)[r0 + min(r, nr - 1)] : (uint32_t)(r0 + r);
+        orow[r] = (size_t)yrow[r] * j.out_stride;
     }
-    for (int r = 0; r < nr; r++) {
-        const uint32_t y = j.rows ? glb(j.rows)[r0 + r] : (uint32_t)(r0 + r);
-        gptr<const uint16_t> row = glb(j.in) + (size_t)y * j.in_stride;
-        for (int i = tid; i <= hi - lo; i += kSrThreads) {
-            const int v = row[min(max(lo + i, 0), last)];
-            bad |= v > j.max_sample;
-            win[r][i] = (uint16_t)v;
-            wsh[r][i + 1] = (uint16_t)v;
+    if (vec) {
+        // At most two groups per thread and row; every load is issued before
+        // any is used.
+        uint64_t q[R][2];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            gptr<const uint64_t> row =
+                reinterpret_cast<gptr<const uint64_t>>(glb(j.in) + (size_t)yrow[r] * j.in_stride + org);
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int g = tid + k * kSrThreads;
+                q[r][k] = r < nr && g < groups ? row[g] : 0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int g = tid + k * kSrThreads;
+                if (r >= nr || g >= groups)
+                    continue;
+                const uint32_t lo32 = (uint32_t)q[r][k], hi32 = (uint32_t)(q[r][k] >> 32);
+                const int s0 = org + 4 * g;
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int v = (int)(((t < 2 ? lo32 : hi32) >> (16 * (t & 1))) & 0xffffu);
+                    bad |= s0 + t >= lo && s0 + t <= hi && v > maxv;
+                }
+                *reinterpret_cast<uint2 *>(&win[r][4 * g]) = make_uint2(lo32, hi32);
+                uint16_t *u = &wsh[r][4 * g + 1];  // window index i at i + 1
+                u[0] = (uint16_t)lo32;
+                *reinterpret_cast<uint32_t *>(u + 1) = (lo32 >> 16) | (hi32 << 16);
+                u[3] = (uint16_t)(hi32 >> 16);
+            }
+        }
+    } else {
+        for (int r = 0; r < nr; r++) {
+            gptr<const uint16_t> row = glb(j.in) + (size_t)yrow[r] * j.in_stride;
+            for (int i = tid; i <= hi - lo; i += kSrThreads) {
+                const int v = row[min(max(lo + i, 0), last)];
+                bad |= v > maxv;
+                win[r][i] = (uint16_t)v;
+                wsh[r][i + 1] = (uint16_t)v;
+            }
         }
     }
     for (int i = tid; i < 64 * 8; i += kSrThreads)
@@ -101,14 +146,13 @@ __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict
     gptr<uint16_t> out16 = glb(j.out);
     gptr<uint8_t> out8 = reinterpret_cast<gptr<uint8_t>>(out16);
     const bool u8 = j.out_u8 != 0;
-    const int maxv = j.max_sample;
 #pragma unroll
     for (int k = 0; k < kSrPer; k++) {
         const int x = x0 + tid + k * kSrThreads;
         if (x > xn)
             break;
         const int src_x = -(1 << 14) + j.start + x * j.step;
-        const int base = (src_x >> 14) - 3 - lo;
+        const int base = (src_x >> 14) - 3 - org;
         // Taps as pairs, (0,1) (2,3) (4,5) (6,7), for v_dot2 over sample pairs.
         const uint4 fq = *reinterpret_cast<const uint4 *>(&tab[((src_x & 0x3fff) >> 8) * 8]);
         const s16x2 f0 = __builtin_bit_cast(s16x2, fq.x), f1 = __builtin_bit_cast(s16x2, fq.y);
