@@ -318,7 +318,7 @@ __device__ __forceinline__ bool stage(uint16_t *dst, gptr<const uint16_t> src, i
                                       int xs, int ys, bool border, int maxv)
 {
     bool bad = false;
-#pragma unroll 2
+#pragma unroll 5
     for (int e = threadIdx.x; e < ROWS * COLS; e += kThreads) {
         const int r = e / COLS, c = e - r * COLS;
         const int x = xs + c, y = ys + r;

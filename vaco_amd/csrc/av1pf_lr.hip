@@ -100,7 +100,7 @@ __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int max
         // Columns x0-4 .. x0+67 all inside the plane: 8-byte loads of four
         // samples (the column left of the window is loaded and dropped).
         constexpr int V = 18;
-#pragma unroll 2
+#pragma unroll 3
         for (int e = threadIdx.x; e < 70 * V; e += THREADS) {
             const int r = e / V, v = e - r * V;
             if (r >= srows)
@@ -126,7 +126,7 @@ __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int max
         }
         return bad;
     }
-#pragma unroll 4
+#pragma unroll 10
     for (int e = threadIdx.x; e < 70 * kS; e += THREADS) {
         const int r = e / kS, c = e - r * kS;
         if (r >= srows || c >= scols)
@@ -181,7 +181,7 @@ __device__ __forceinline__ void lr_copy(const Tile &t)
             raise_status(P.status, kFlagSampleRange);
         return;
     }
-#pragma unroll 4
+#pragma unroll 8
     for (int e = threadIdx.x; e < 64 * 64; e += kLrThreads) {
         const int r = e >> 6, c = e & 63;
         if (r < th && c < tw) {
