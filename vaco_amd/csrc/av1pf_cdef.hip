@@ -237,8 +237,9 @@ __device__ __forceinline__ s16x2 ldb(const char *a)
 // select per tap.
 // Taps k0 .. k1-1 (both signs) of filter_pairs. FULL = false: the tap
 // direction's strength is zero, so its terms vanish (constrain() returns 0,
-// cdef.rs:127-129) but its samples still bound the clamp range.
-template <bool BORDER, bool FULL, int N, int STRIDE, int K0, int K1>
+// cdef.rs:127-129) but its samples still bound the clamp range. CLAMP =
+// false: no clamp range is kept (see filter_pairs).
+template <bool BORDER, bool FULL, bool CLAMP, int N, int STRIDE, int K0, int K1>
 __device__ __forceinline__ void apply_taps(const char *a, const Blk &B, const u16x2 *x, u16x2 S,
                                            u16x2 sh, s16x2 w0, s16x2 w1, s16x2 *sum, u16x2 *mn,
                                            u16x2 *mx)
@@ -267,8 +268,10 @@ __device__ __forceinline__ void apply_taps(const char *a, const Blk &B, const u1
                                     __builtin_bit_cast(s16x2, __builtin_elementwise_min(dn, T));
                     sum[n] += w * c;
                 }
-                mn[n] = __builtin_elementwise_min(mn[n], p);
-                mx[n] = __builtin_elementwise_max(mx[n], p);
+                if (CLAMP) {
+                    mn[n] = __builtin_elementwise_min(mn[n], p);
+                    mx[n] = __builtin_elementwise_max(mx[n], p);
+                }
             }
         }
     }
@@ -276,6 +279,14 @@ __device__ __forceinline__ void apply_taps(const char *a, const Blk &B, const u1
 
 // pri_on / sec_on: the superblock's preset has a nonzero primary / secondary
 // strength (uniform over the workgroup, so the branches are scalar).
+//
+// With only one of them on, the clamp of cdef.rs:200-202 cannot bind, so it
+// and the other direction's taps are skipped: the tap weights of one
+// direction sum to 12 (2 * (4 + 2), 2 * (3 + 3) or 4 * (2 + 1)), each term has
+// the sign of p - x and at most its size, so with D = the largest p - x over
+// the taps (0 if none is above x), 8 + sum <= 8 + 12 D < 16 (D + 1) and y <=
+// x + D, the largest tap; likewise y >= the smallest. (An unavailable tap
+// contributes nothing either way.)
 template <bool BORDER, int N, int STRIDE>
 __device__ __forceinline__ void filter_pairs(const uint16_t *t, int ci, const Blk &B, s16x2 *res,
                                              bool pri_on, bool sec_on)
@@ -293,20 +304,22 @@ __device__ __forceinline__ void filter_pairs(const uint16_t *t, int ci, const Bl
     const u16x2 ps = (u16x2)(unsigned short)(B.shifts & 15), qs = (u16x2)(unsigned short)(B.shifts >> 4);
     const short t0 = (B.flags & 2) ? 3 : 4;
     const s16x2 w0 = (s16x2)t0, w1 = (s16x2)(short)(6 - t0);
-    if (pri_on)
-        apply_taps<BORDER, true, N, STRIDE, 0, 2>(a, B, x, P, ps, w0, w1, sum, mn, mx);
-    else
-        apply_taps<BORDER, false, N, STRIDE, 0, 2>(a, B, x, P, ps, w0, w1, sum, mn, mx);
-    if (sec_on)
-        apply_taps<BORDER, true, N, STRIDE, 2, 6>(a, B, x, Q, qs, (s16x2)2, (s16x2)1, sum, mn, mx);
-    else
-        apply_taps<BORDER, false, N, STRIDE, 2, 6>(a, B, x, Q, qs, (s16x2)2, (s16x2)1, sum, mn, mx);
+    const bool clamp = pri_on && sec_on;
+    if (clamp) {
+        apply_taps<BORDER, true, true, N, STRIDE, 0, 2>(a, B, x, P, ps, w0, w1, sum, mn, mx);
+        apply_taps<BORDER, true, true, N, STRIDE, 2, 6>(a, B, x, Q, qs, (s16x2)2, (s16x2)1, sum, mn, mx);
+    } else if (pri_on) {
+        apply_taps<BORDER, true, false, N, STRIDE, 0, 2>(a, B, x, P, ps, w0, w1, sum, mn, mx);
+    } else if (sec_on) {
+        apply_taps<BORDER, true, false, N, STRIDE, 2, 6>(a, B, x, Q, qs, (s16x2)2, (s16x2)1, sum, mn, mx);
+    }
 #pragma unroll
     for (int n = 0; n < N; n++) {
         const s16x2 xs = __builtin_bit_cast(s16x2, x[n]);
         const s16x2 y = xs + ((sum[n] + (s16x2)8 + (sum[n] >> (s16x2)15)) >> (s16x2)4);  // cdef.rs:200-201
-        res[n] = __builtin_elementwise_min(__builtin_elementwise_max(y, __builtin_bit_cast(s16x2, mn[n])),
-                                           __builtin_bit_cast(s16x2, mx[n]));
+        res[n] = clamp ? __builtin_elementwise_min(__builtin_elementwise_max(y, __builtin_bit_cast(s16x2, mn[n])),
+                                                   __builtin_bit_cast(s16x2, mx[n]))
+                       : y;
     }
 }
 
