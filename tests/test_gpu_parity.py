@@ -111,6 +111,17 @@ RANDOM_CASES = [
     dict(seed=17, coded_w=80, coded_h=72, zero_strengths=True, upscaled_w=128),
     dict(seed=18, coded_w=260, coded_h=200, unit_kinds=(2,), unit=64),
     dict(seed=19, coded_w=260, coded_h=200, unit_kinds=(1,), unit=256),
+    # self-guided only at 10 and 12 bits (the box arithmetic is specialised per depth)
+    dict(seed=20, coded_w=200, coded_h=140, bd=10, unit_kinds=(2,)),
+    dict(seed=21, coded_w=200, coded_h=140, bd=12, unit_kinds=(2,), unit=32),
+    # the smallest frames, and extreme aspect ratios
+    dict(seed=22, coded_w=8, coded_h=8),
+    dict(seed=23, coded_w=16, coded_h=10, bd=12, upscaled_w=30),
+    dict(seed=24, coded_w=24, coded_h=300, bd=10),
+    dict(seed=25, coded_w=600, coded_h=16, unit=32),
+    # superres at 12 bits in 4:4:4, and 4:2:2 superres with 32-sample units
+    dict(seed=26, coded_w=120, coded_h=64, bd=12, ssx=0, ssy=0, upscaled_w=200),
+    dict(seed=27, coded_w=97, coded_h=70, bd=10, ssx=1, ssy=0, upscaled_w=150, unit=32),
 ]
 
 
