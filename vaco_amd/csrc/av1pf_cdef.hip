@@ -48,28 +48,41 @@ constexpr int bin_count(int d, int b)
     return c;
 }
 
-constexpr int kRow = 80;         // LDS tile row: columns px0-8 .. px0+71
+// LDS tiles. A plane's tile holds its rows y0-2 .. with column x0-8 at index
+// 0 (luma: 68 rows of kRow samples, columns x0-8 .. x0+71). Each tile is
+// followed by its shifted copy (index i at i + 1), SHIFT samples on. The
+// luma row is 44 dwords: the 8 rows of a block then start in 8 distinct
+// groups of 4 banks, so a half-wave filtering one block along one direction
+// reads its taps without bank conflicts. Chroma tiles are 80 samples wide
+// (48 when subsampled horizontally) and 68 rows high (36 when subsampled
+// vertically), so that two workgroups' tiles fit a CU's LDS in every layout.
+constexpr int kRow = 88;
+constexpr int kTile = 68 * kRow;
+template <int SX>
+constexpr int chroma_row() { return SX ? 48 : 80; }
+template <int SX, int SY>
+constexpr int chroma_tile() { return (SY ? 36 : 68) * chroma_row<SX>(); }
 
-// LDS offsets (dy * kRow + dx) of DIRECTIONS[d][k] for directions 4h .. 4h+3,
+// LDS offsets (dy * ROW + dx) of DIRECTIONS[d][k] for directions 4h .. 4h+3,
 // 16 bits each, so a lane selects its direction's offset with register
 // operations instead of a memory load.
+template <int ROW>
 constexpr uint64_t dir_offsets(int k, int h)
 {
     uint64_t v = 0;
     for (int i = 0; i < 4; i++)
-        v |= (uint64_t)(uint16_t)(int16_t)(kDirDy[4 * h + i][k] * kRow + kDirDx[4 * h + i][k]) << (16 * i);
+        v |= (uint64_t)(uint16_t)(int16_t)(kDirDy[4 * h + i][k] * ROW + kDirDx[4 * h + i][k]) << (16 * i);
     return v;
 }
 
+template <int ROW>
 __device__ __forceinline__ int tap_offset(int dir, int k)
 {
-    constexpr uint64_t t00 = dir_offsets(0, 0), t01 = dir_offsets(0, 1);
-    constexpr uint64_t t10 = dir_offsets(1, 0), t11 = dir_offsets(1, 1);
+    constexpr uint64_t t00 = dir_offsets<ROW>(0, 0), t01 = dir_offsets<ROW>(0, 1);
+    constexpr uint64_t t10 = dir_offsets<ROW>(1, 0), t11 = dir_offsets<ROW>(1, 1);
     const uint64_t t = k ? (dir < 4 ? t10 : t11) : (dir < 4 ? t00 : t01);
     return (int16_t)(uint16_t)(t >> (16 * (dir & 3)));
 }
-constexpr int kTile = 68 * kRow; // one plane's tile
-constexpr int kShift = 3 * kTile; // tile p + kShift: plane p shifted by one sample
 constexpr int kThreads = 1024;
 
 // Cost of direction D (cdef.rs:104-113) of the 8x8 block whose first row
@@ -135,6 +148,8 @@ struct __align__(16) Blk {
     uint8_t flags;       // 1: filter this block, 2: primary taps are (3, 3)
 };
 
+// ROW, SHIFT: the row stride and shifted-copy distance of the block's tile.
+template <int ROW, int SHIFT>
 __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int damping,
                                         int shift)
 {
@@ -148,15 +163,15 @@ __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int
     const int d2 = (dir + 2) & 7, d6 = (dir + 6) & 7;
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-        B.off[k] = (int16_t)tap_offset(dir, k);
-        B.off[2 + k] = (int16_t)tap_offset(d2, k);
-        B.off[4 + k] = (int16_t)tap_offset(d6, k);
+        B.off[k] = (int16_t)tap_offset<ROW>(dir, k);
+        B.off[2 + k] = (int16_t)tap_offset<ROW>(d2, k);
+        B.off[4 + k] = (int16_t)tap_offset<ROW>(d6, k);
     }
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         const int o = B.off[k];
-        B.offp[k] = 2 * ((o & 1) ? kShift + 1 + o : o);
-        B.offm[k] = 2 * ((o & 1) ? kShift + 1 - o : -o);
+        B.offp[k] = 2 * ((o & 1) ? SHIFT + 1 + o : o);
+        B.offm[k] = 2 * ((o & 1) ? SHIFT + 1 - o : -o);
     }
     return B;
 }
@@ -324,9 +339,9 @@ __device__ __forceinline__ void filter_pairs(const uint16_t *t, int ci, const Bl
 }
 
 // Stage rows ys .. ys+ROWS-1, columns xs .. xs+COLS-1 of a plane into an LDS
-// tile at column offset 6, and into the shifted copy at dst + kShift;
-// positions outside the plane hold kUnavailable.
-template <int ROWS, int COLS>
+// tile of row stride ROW at column offset 6, and into the shifted copy at
+// dst + SHIFT; positions outside the plane hold kUnavailable.
+template <int ROWS, int COLS, int ROW, int SHIFT>
 __device__ __forceinline__ bool stage(uint16_t *dst, gptr<const uint16_t> src, int W, int H, int st,
                                       int xs, int ys, bool border, int maxv)
 {
@@ -340,8 +355,8 @@ __device__ __forceinline__ bool stage(uint16_t *dst, gptr<const uint16_t> src, i
             v = src[(size_t)y * st + x];
             bad |= v > maxv;
         }
-        dst[r * kRow + c + 6] = v;
-        dst[kShift + r * kRow + c + 7] = v;
+        dst[r * ROW + c + 6] = v;
+        dst[SHIFT + r * ROW + c + 7] = v;
     }
     return bad;
 }
@@ -352,7 +367,7 @@ __device__ __forceinline__ bool stage(uint16_t *dst, gptr<const uint16_t> src, i
 // columns are multiples of 4 and plane widths are multiples of 4, so a group
 // lies wholly inside or wholly outside the plane; outside (BORDER only) it
 // holds kUnavailable.
-template <int ROWS, int COLS, bool BORDER>
+template <int ROWS, int COLS, int ROW, int SHIFT, bool BORDER>
 __device__ __forceinline__ bool stage_vec(uint16_t *dst, gptr<const uint16_t> src, int W, int H, int st,
                                           int xs, int ys, int maxv)
 {
@@ -369,9 +384,9 @@ __device__ __forceinline__ bool stage_vec(uint16_t *dst, gptr<const uint16_t> sr
             const uint32_t m = max(max(q.x & 0xffffu, q.x >> 16), max(q.y & 0xffffu, q.y >> 16));
             bad |= (int)m > maxv;
         }
-        uint16_t *t = static_cast<uint16_t *>(__builtin_assume_aligned(dst + r * kRow + 4 + 4 * v, 8));
+        uint16_t *t = static_cast<uint16_t *>(__builtin_assume_aligned(dst + r * ROW + 4 + 4 * v, 8));
         *reinterpret_cast<uint2 *>(t) = q;
-        uint16_t *u = t + kShift + 1;  // sample i of the tile at i + 1
+        uint16_t *u = t + SHIFT + 1;  // sample i of the tile at i + 1
         u[0] = (uint16_t)q.x;
         *reinterpret_cast<uint32_t *>(u + 1) = (q.x >> 16) | (q.y << 16);
         u[3] = (uint16_t)(q.y >> 16);
@@ -397,13 +412,6 @@ __device__ __forceinline__ void store_pair(gptr<uint16_t> plane, size_t i, s16x2
     }
 }
 
-// Filter (or pass through) every sample of a staged superblock and store
-// it, two horizontally adjacent samples per lane (filter_pairs). Luma: 16
-// lanes per 8x8 block, pairs in rows i0 and i0+4 at columns 2*j2, 2*j2+1.
-// Chroma: 16 lanes per block, pairs of U then V. BORDER: the superblock
-// touches a plane edge (unavailable taps; partial superblocks store only
-// samples inside the plane; plane widths are even, so a pair is all in or
-// all out).
 // The descriptor fields the phases after staging use, read at the start of
 // the workgroup so that their load latency hides under the staging loads.
 struct SbFields {
@@ -443,46 +451,52 @@ struct SbFields {
     }
 };
 
+// Filter (or pass through) every sample of a staged superblock and store it.
+// Luma: each half-wave takes one 8x8 block (32 lanes = 8 rows x 4 pairs of
+// columns), blocks hw and hw + 32 for half-wave hw; all its lanes then read
+// their taps at one direction's offsets, which the luma row stride keeps in
+// distinct banks. Chroma: 16 lanes per block, pairs of U then V. BORDER: the
+// superblock touches a plane edge (unavailable taps; partial superblocks
+// store only samples inside the crop; widths are even, so a pair is all in
+// or all out, except at an odd crop width where the second sample is
+// dropped). L, U, V: the planes' tiles.
 template <bool BORDER, int SX, int SY>
-__device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t (*lt)[kTile],
-                                          const Blk (*blk)[64], const uint8_t *bact, bool any,
-                                          int x0, int y0, int ix)
+__device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, const uint16_t *U,
+                                          const uint16_t *V, const Blk (*blk)[64], const uint8_t *bact,
+                                          bool any, int x0, int y0, int ix)
 {
+    constexpr int CROW = chroma_row<SX>();
     // Strengths are per superblock preset (the luma primary is only ever
     // lowered per block, never raised from zero).
     const bool ypri = f.strength(0, ix) != 0, ysec = f.strength(1, ix) != 0;
     const bool cpri = f.strength(2, ix) != 0, csec = f.strength(3, ix) != 0;
     constexpr int BW = 8 >> SX, BH = 8 >> SY, PPP = BW * BH / 2, CPX = BW * BH / 16;
     const int tid = threadIdx.x;
-    const int b = tid >> 4, q = tid & 15;
-    const int bi = b >> 3, bj = b & 7;
-    const bool act = any && bact[b];
     {
-        const int j = (q & 3) * 2, i0 = q >> 2;
-        Blk B = {};
-        if (act)
-            B = blk[0][b];
-        const uint16_t *t = lt[0];
+        const int hw = tid >> 5, l = tid & 31, i = l >> 2, j = (l & 3) * 2;
         gptr<uint16_t> dst = f.out[0];
         const int ost = f.out_stride[0], W0 = f.crop_w[0], H0 = f.crop_h[0];
-        const int ci0 = (2 + bi * 8 + i0) * kRow + 8 + bj * 8 + j;
-        s16x2 r2[2];
-        if (act) {
-            filter_pairs<BORDER, 2, 4 * kRow>(t, ci0, B, r2, ypri, ysec);
-        } else {
-            r2[0] = ld2(t, ci0);
-            r2[1] = ld2(t, ci0 + 4 * kRow);
-        }
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
-            const int i = i0 + 4 * k;
+        for (int pass = 0; pass < 2; pass++) {
+            const int b = hw + 32 * pass, bi = b >> 3, bj = b & 7;
+            const bool act = any && bact[b];
+            const int ci = (2 + bi * 8 + i) * kRow + 8 + bj * 8 + j;
+            s16x2 res;
+            if (act) {
+                const Blk B = blk[0][b];
+                filter_pairs<BORDER, 1, 0>(L, ci, B, &res, ypri, ysec);
+            } else {
+                res = ld2(L, ci);
+            }
             const int x = x0 + bj * 8 + j, y = y0 + bi * 8 + i;
-            const s16x2 res = r2[k];
             if (!BORDER || (x < W0 && y < H0))
                 store_pair(dst, (size_t)y * ost + x, res, f.out_u8, x + 1 < W0);
         }
     }
     if (f.nplanes > 1) {
+        const int b = tid >> 4, q = tid & 15;
+        const int bi = b >> 3, bj = b & 7;
+        const bool act = any && bact[b];
         Blk B = {};
         if (act)
             B = blk[1][b];
@@ -493,8 +507,8 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t (*lt
             const int p = pi < PPP ? 1 : 2, r = pi % PPP;
             const int i = r / (BW / 2), j = (r % (BW / 2)) * 2;
             const int x = cx0 + bj * BW + j, y = cy0 + bi * BH + i;
-            const int ci = (2 + bi * BH + i) * kRow + 8 + bj * BW + j;
-            const uint16_t *t = lt[p];
+            const int ci = (2 + bi * BH + i) * CROW + 8 + bj * BW + j;
+            const uint16_t *t = p == 1 ? U : V;
             s16x2 res;
             if (act)
                 filter_pairs<BORDER, 1, 0>(t, ci, B, &res, cpri, csec);
@@ -515,7 +529,13 @@ template <int SX, int SY, bool XCD>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cdef(const CdefFrame *__restrict__ frames, int n_frames)
 {
     constexpr int CW = 64 >> SX, CH = 64 >> SY;
-    __shared__ __align__(16) uint16_t lt[6][kTile];  // planes, then their shifted copies
+    constexpr int CROW = chroma_row<SX>(), CTILE = chroma_tile<SX, SY>();
+    // Luma tile and its shifted copy, then U and V, each with its copy. V
+    // starts 8 samples further on: U and V pairs of one half-wave then fall
+    // in different banks (a model of the tap addresses: 2.3 accesses per bank
+    // per read without the gap, 1.3 with it).
+    __shared__ __align__(16) uint16_t lt[2 * kTile + 4 * CTILE + 8];
+    uint16_t *const L = lt, *const U = lt + 2 * kTile, *const V = U + 2 * CTILE + 8;
     __shared__ Blk blk[2][64];
     __shared__ uint32_t cost[8][64];
     __shared__ uint8_t bact[64];
@@ -542,12 +562,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     bool bad = false;
     const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
     if (!f.vec_ok) {
-        bad = stage<68, 68>(lt[0], glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border, maxv);
+        bad = stage<68, 68, kRow, kTile>(L, glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border,
+                                         maxv);
         if (nplanes > 1) {
-            bad |= stage<CH + 4, CW + 4>(lt[1], glb(f.in[1]), f.width[1], f.height[1], f.in_stride[1],
-                                         cx, cy, border, maxv);
-            bad |= stage<CH + 4, CW + 4>(lt[2], glb(f.in[2]), f.width[2], f.height[2], f.in_stride[2],
-                                         cx, cy, border, maxv);
+            bad |= stage<CH + 4, CW + 4, CROW, CTILE>(U, glb(f.in[1]), f.width[1], f.height[1],
+                                                      f.in_stride[1], cx, cy, border, maxv);
+            bad |= stage<CH + 4, CW + 4, CROW, CTILE>(V, glb(f.in[2]), f.width[2], f.height[2],
+                                                      f.in_stride[2], cx, cy, border, maxv);
         }
     } else {
         // Interior superblocks: rows y0-2 .. y0+65 and columns x0-4 .. x0+67
@@ -556,16 +577,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
         // for chroma), so no bounds are checked.
         const int W1 = nplanes > 1 ? (int)f.width[1] : 0, H1 = nplanes > 1 ? (int)f.height[1] : 0;
         if (border) {
-            bad = stage_vec<68, 68, true>(lt[0], glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2, y0 - 2, maxv);
+            bad = stage_vec<68, 68, kRow, kTile, true>(L, glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2,
+                                                       y0 - 2, maxv);
             if (nplanes > 1) {
-                bad |= stage_vec<CH + 4, CW + 4, true>(lt[1], glb(f.in[1]), W1, H1, f.in_stride[1], cx, cy, maxv);
-                bad |= stage_vec<CH + 4, CW + 4, true>(lt[2], glb(f.in[2]), W1, H1, f.in_stride[2], cx, cy, maxv);
+                bad |= stage_vec<CH + 4, CW + 4, CROW, CTILE, true>(U, glb(f.in[1]), W1, H1, f.in_stride[1],
+                                                                   cx, cy, maxv);
+                bad |= stage_vec<CH + 4, CW + 4, CROW, CTILE, true>(V, glb(f.in[2]), W1, H1, f.in_stride[2],
+                                                                   cx, cy, maxv);
             }
         } else {
-            bad = stage_vec<68, 68, false>(lt[0], glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2, y0 - 2, maxv);
+            bad = stage_vec<68, 68, kRow, kTile, false>(L, glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2,
+                                                        y0 - 2, maxv);
             if (nplanes > 1) {
-                bad |= stage_vec<CH + 4, CW + 4, false>(lt[1], glb(f.in[1]), W1, H1, f.in_stride[1], cx, cy, maxv);
-                bad |= stage_vec<CH + 4, CW + 4, false>(lt[2], glb(f.in[2]), W1, H1, f.in_stride[2], cx, cy, maxv);
+                bad |= stage_vec<CH + 4, CW + 4, CROW, CTILE, false>(U, glb(f.in[1]), W1, H1,
+                                                                    f.in_stride[1], cx, cy, maxv);
+                bad |= stage_vec<CH + 4, CW + 4, CROW, CTILE, false>(V, glb(f.in[2]), W1, H1,
+                                                                    f.in_stride[2], cx, cy, maxv);
             }
         }
     }
@@ -578,7 +605,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
             const int b = tid & 63;
             if (bact[b])
                 cost[tid >> 6][b] =
-                    dir_cost_any(tid >> 6, &lt[0][(2 + (b >> 3) * 8) * kRow + 8 + (b & 7) * 8], shift);
+                    dir_cost_any(tid >> 6, &L[(2 + (b >> 3) * 8) * kRow + 8 + (b & 7) * 8], shift);
         }
         __syncthreads();
         if (tid < 128) {
@@ -605,8 +632,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
                         const uint32_t lg = sc ? min(ilog2u(sc), 12) : 0;
                         adj = (pri * (4 + lg) + 8) >> 4;
                     }
-                    B = make_blk(pri ? best : 0, adj, sf.strength(1, ix) << shift,
-                                 (uint8_t)(sf.damping + shift), shift);
+                    B = make_blk<kRow, kTile>(pri ? best : 0, adj, sf.strength(1, ix) << shift,
+                                              (uint8_t)(sf.damping + shift), shift);
                 } else {
                     const uint32_t pri = sf.strength(2, ix) << shift;
                     // chroma_direction, cdef.rs:222-230: nibble `best` of
@@ -616,8 +643,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
                         cdir = (0x66654207u >> (4 * best)) & 7;
                     else if (!SX && SY)
                         cdir = (0x06432221u >> (4 * best)) & 7;
-                    B = make_blk(pri ? cdir : 0, pri, sf.strength(3, ix) << shift,
-                                 (uint8_t)(sf.damping + shift - 1), shift);
+                    B = make_blk<CROW, CTILE>(pri ? cdir : 0, pri, sf.strength(3, ix) << shift,
+                                              (uint8_t)(sf.damping + shift - 1), shift);
                 }
             }
             blk[cls][b] = B;
@@ -627,9 +654,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
 
     const bool any = idx >= 0;
     if (border)
-        filter_sb<true, SX, SY>(sf, lt, blk, bact, any, x0, y0, idx & 7);
+        filter_sb<true, SX, SY>(sf, L, U, V, blk, bact, any, x0, y0, idx & 7);
     else
-        filter_sb<false, SX, SY>(sf, lt, blk, bact, any, x0, y0, idx & 7);
+        filter_sb<false, SX, SY>(sf, L, U, V, blk, bact, any, x0, y0, idx & 7);
 }
 
 // Kernel-level cases (cdef::find_direction + cdef::filter_block on one
@@ -682,7 +709,7 @@ __global__ __launch_bounds__(64) void k_cdef_cases(const uint16_t *__restrict__ 
         dirs[cs] = (uint8_t)best;
         vars[cs] = (bc - cost[(best + 4) & 7]) >> 10;
     }
-    const Blk B = make_blk(P.direction, P.primary, P.secondary, P.damping, shift);
+    const Blk B = make_blk<kRow, kTile>(P.direction, P.primary, P.secondary, P.damping, shift);
     const int r = lane >> 3, c = lane & 7;
     uint16_t res = 0;
     if (r < (int)P.height && c < (int)P.width)
