@@ -225,14 +225,26 @@ template <class F>
 static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, uint64_t bytes, hipStream_t s,
                                F launch);
 
-// Copy the slot's staged bytes to its device mirror on the upload stream,
-// which is otherwise idle, so the copy runs while earlier frames still
-// compute; stream s waits for it before the call's kernels. The slot's
-// previous frame has finished (take_slot), so its mirror is free.
+// Copy the slot's staged bytes to its device mirror. Up to kInlineUpload
+// bytes (one frame's side information) the copy runs on stream s itself,
+// ahead of the call's kernels: a few microseconds of GPU time, against the
+// host cost of an event record and a cross-stream wait per call. Larger
+// uploads (batches) run on the upload stream, which is otherwise idle, so
+// they overlap the previous call's kernels; stream s waits for them. The
+// slot's previous frame has finished (take_slot), so its mirror is free.
+constexpr size_t kInlineUpload = 512 * 1024;
+
 static vaco_av1pf_status upload_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t s)
 {
     if (a->used == 0)
         return VACO_AV1PF_OK;
+    if (a->used <= kInlineUpload) {
+        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_UPLOAD, (uint64_t)a->used * 2, s,
+                              [&](hipEvent_t e0, hipEvent_t e1) {
+                                  return launch_upload(a->d, a->hd, a->used, s, e0, e1);
+                              }));
+        return VACO_AV1PF_OK;
+    }
     hipStream_t u = ctx->up_stream;
     HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_UPLOAD, (uint64_t)a->used * 2, u,
                           [&](hipEvent_t e0, hipEvent_t e1) {

@@ -12,14 +12,29 @@
 
 namespace av1pf {
 
+// At most 64 workgroups: the copy overlaps the previous call's kernels, and
+// waves waiting on the host link would otherwise hold wave slots on every
+// CU. Eight loads in flight per thread keep the link busy with few waves.
 constexpr int kUpThreads = 256;
-constexpr uint32_t kUpMaxBlocks = 1024;
+constexpr uint32_t kUpMaxBlocks = 64;
+constexpr uint32_t kUpUnroll = 8;
 
 // n16: 16-byte words to copy (both buffers hold at least n16 * 16 bytes).
 __global__ __launch_bounds__(kUpThreads) void k_upload(uint4 *__restrict__ dst,
                                                        const uint4 *__restrict__ src, uint32_t n16)
 {
-    for (uint32_t i = blockIdx.x * kUpThreads + threadIdx.x; i < n16; i += gridDim.x * kUpThreads)
+    const uint32_t stride = gridDim.x * kUpThreads;
+    uint32_t i = blockIdx.x * kUpThreads + threadIdx.x;
+    for (; i + (kUpUnroll - 1) * stride < n16; i += kUpUnroll * stride) {
+        uint4 v[kUpUnroll];
+#pragma unroll
+        for (uint32_t k = 0; k < kUpUnroll; k++)
+            v[k] = src[i + k * stride];
+#pragma unroll
+        for (uint32_t k = 0; k < kUpUnroll; k++)
+            dst[i + k * stride] = v[k];
+    }
+    for (; i < n16; i += stride)
         dst[i] = src[i];
 }
 
