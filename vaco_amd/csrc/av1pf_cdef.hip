@@ -250,11 +250,9 @@ __device__ __forceinline__ s16x2 ldb(const char *a)
 // sample in its place has exactly that effect (constrain(0) = 0, and x is
 // already in [mn, mx]), so BORDER superblocks swap it in with one bit
 // select per tap.
-// Taps k0 .. k1-1 (both signs) of filter_pairs. FULL = false: the tap
-// direction's strength is zero, so its terms vanish (constrain() returns 0,
-// cdef.rs:127-129) but its samples still bound the clamp range. CLAMP =
-// false: no clamp range is kept (see filter_pairs).
-template <bool BORDER, bool FULL, bool CLAMP, int N, int STRIDE, int K0, int K1>
+// Taps k0 .. k1-1 (both signs) of filter_pairs. CLAMP = false: no clamp
+// range is kept (see filter_pairs).
+template <bool BORDER, bool CLAMP, int N, int STRIDE, int K0, int K1>
 __device__ __forceinline__ void apply_taps(const char *a, const Blk &B, const u16x2 *x, u16x2 S,
                                            u16x2 sh, s16x2 w0, s16x2 w1, s16x2 *sum, u16x2 *mn,
                                            u16x2 *mx)
@@ -275,14 +273,12 @@ __device__ __forceinline__ void apply_taps(const char *a, const Blk &B, const u1
                                        (~m & __builtin_bit_cast(uint32_t, p));
                     p = __builtin_bit_cast(u16x2, v);
                 }
-                if (FULL) {
-                    const u16x2 up = __builtin_elementwise_sub_sat(p, x[n]);
-                    const u16x2 dn = __builtin_elementwise_sub_sat(x[n], p);
-                    const u16x2 T = __builtin_elementwise_sub_sat(S, (up | dn) >> sh);
-                    const s16x2 c = __builtin_bit_cast(s16x2, __builtin_elementwise_min(up, T)) -
-                                    __builtin_bit_cast(s16x2, __builtin_elementwise_min(dn, T));
-                    sum[n] += w * c;
-                }
+                const u16x2 up = __builtin_elementwise_sub_sat(p, x[n]);
+                const u16x2 dn = __builtin_elementwise_sub_sat(x[n], p);
+                const u16x2 T = __builtin_elementwise_sub_sat(S, (up | dn) >> sh);
+                const s16x2 c = __builtin_bit_cast(s16x2, __builtin_elementwise_min(up, T)) -
+                                __builtin_bit_cast(s16x2, __builtin_elementwise_min(dn, T));
+                sum[n] += w * c;
                 if (CLAMP) {
                     mn[n] = __builtin_elementwise_min(mn[n], p);
                     mx[n] = __builtin_elementwise_max(mx[n], p);
@@ -321,12 +317,12 @@ __device__ __forceinline__ void filter_pairs(const uint16_t *t, int ci, const Bl
     const s16x2 w0 = (s16x2)t0, w1 = (s16x2)(short)(6 - t0);
     const bool clamp = pri_on && sec_on;
     if (clamp) {
-        apply_taps<BORDER, true, true, N, STRIDE, 0, 2>(a, B, x, P, ps, w0, w1, sum, mn, mx);
-        apply_taps<BORDER, true, true, N, STRIDE, 2, 6>(a, B, x, Q, qs, (s16x2)2, (s16x2)1, sum, mn, mx);
+        apply_taps<BORDER, true, N, STRIDE, 0, 2>(a, B, x, P, ps, w0, w1, sum, mn, mx);
+        apply_taps<BORDER, true, N, STRIDE, 2, 6>(a, B, x, Q, qs, (s16x2)2, (s16x2)1, sum, mn, mx);
     } else if (pri_on) {
-        apply_taps<BORDER, true, false, N, STRIDE, 0, 2>(a, B, x, P, ps, w0, w1, sum, mn, mx);
+        apply_taps<BORDER, false, N, STRIDE, 0, 2>(a, B, x, P, ps, w0, w1, sum, mn, mx);
     } else if (sec_on) {
-        apply_taps<BORDER, true, false, N, STRIDE, 2, 6>(a, B, x, Q, qs, (s16x2)2, (s16x2)1, sum, mn, mx);
+        apply_taps<BORDER, false, N, STRIDE, 2, 6>(a, B, x, Q, qs, (s16x2)2, (s16x2)1, sum, mn, mx);
     }
 #pragma unroll
     for (int n = 0; n < N; n++) {
