@@ -406,8 +406,8 @@ def main():
                          "copy: 256 MiB device copies (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only: every box has one MI355X and the "
-                                   "stage shards by frame with no collectives (see DESIGN.md); run --gpus 1"}))
+        print(json.dumps({"error": "one GPU only: frames are independent, so the stage shards by frame "
+                                   "across GPUs with no collectives; run --gpus 1"}))
         sys.exit(2)
 
     from vaco_amd import av1pf, hip, workloads
@@ -510,7 +510,7 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_launch": d["bytes_per_launch"],
             "avg_launch_us": d["us_per_launch"],
-            "note": "integer stage, VALU-bound: see DESIGN.md section 7",
+            "note": "integer stage, VALU-bound: its instruction count takes longer than its bytes at 8 TB/s",
         },
         "uhd_10bit": uhd,
         "batched": batched,
