@@ -357,38 +357,75 @@ __device__ __forceinline__ bool stage(uint16_t *dst, gptr<const uint16_t> src, i
     return bad;
 }
 
-// stage() for planes with 8-byte aligned rows: the window widened to start
-// at column xs - 2 and loaded four samples at a time (the two extra columns
-// on each side are never read by a tap). Also fills the shifted copy. Group
-// columns are multiples of 4 and plane widths are multiples of 4, so a group
-// lies wholly inside or wholly outside the plane; outside (BORDER only) it
-// holds kUnavailable.
-template <int ROWS, int COLS, int ROW, int SHIFT, bool BORDER>
-__device__ __forceinline__ bool stage_vec(uint16_t *dst, gptr<const uint16_t> src, int W, int H, int st,
-                                          int xs, int ys, int maxv)
-{
-    constexpr int V = (COLS + 4) / 4;  // 4-sample groups per row
-    bool bad = false;
-#pragma unroll 2
-    for (int e = threadIdx.x; e < ROWS * V; e += kThreads) {
-        const int r = e / V, v = e - r * V;
-        const int y = ys + r, x = xs - 2 + 4 * v;
-        uint2 q = {0xffffffffu, 0xffffffffu};
-        if (!BORDER || (y >= 0 && y < H && x >= 0 && x < W)) {
+// A window of a plane with 8-byte aligned rows, staged as stage() stages it
+// but four samples at a time: the window widened to start at column xs - 2
+// (the two extra columns on each side are never read by a tap), loaded in
+// two phases so that a thread's loads for every plane are in flight
+// together. load() issues the thread's global loads into registers; store()
+// writes them, and the shifted copy, to the LDS tile once they arrive.
+// Group columns are multiples of 4 and plane widths are multiples of 4, so a
+// group lies wholly inside or wholly outside the plane; outside (BORDER only)
+// it holds kUnavailable. An outside group's load reads the nearest group
+// inside the plane instead, so every load is in bounds and unconditional.
+template <int ROWS, int COLS, bool BORDER>
+struct VecStage {
+    static constexpr int V = (COLS + 4) / 4;  // 4-sample groups per row
+    static constexpr int N = ROWS * V;
+    static constexpr int K = (N + kThreads - 1) / kThreads;
+    static_assert(K <= 32, "one validity bit per group");
+    uint2 q[K];
+    uint32_t inside;  // bit k: group k lies in the plane
+
+    __device__ __forceinline__ void load(gptr<const uint16_t> src, int W, int H, int st, int xs, int ys)
+    {
+        inside = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int e = min((int)threadIdx.x + k * kThreads, N - 1);
+            const int r = e / V, v = e - r * V;
+            int y = ys + r, x = xs - 2 + 4 * v;
+            if (BORDER) {
+                if (y >= 0 && y < H && x >= 0 && x < W)
+                    inside |= 1u << k;
+                y = min(max(y, 0), H - 1);
+                x = min(max(x, 0), W - 4);
+            } else {
+                inside |= 1u << k;
+            }
             const uint64_t w = *reinterpret_cast<gptr<const uint64_t>>(src + (size_t)y * st + x);
-            q = {(uint32_t)w, (uint32_t)(w >> 32)};
-            const uint32_t m = max(max(q.x & 0xffffu, q.x >> 16), max(q.y & 0xffffu, q.y >> 16));
-            bad |= (int)m > maxv;
+            q[k] = {(uint32_t)w, (uint32_t)(w >> 32)};
         }
-        uint16_t *t = static_cast<uint16_t *>(__builtin_assume_aligned(dst + r * ROW + 4 + 4 * v, 8));
-        *reinterpret_cast<uint2 *>(t) = q;
-        uint16_t *u = t + SHIFT + 1;  // sample i of the tile at i + 1
-        u[0] = (uint16_t)q.x;
-        *reinterpret_cast<uint32_t *>(u + 1) = (q.x >> 16) | (q.y << 16);
-        u[3] = (uint16_t)(q.y >> 16);
     }
-    return bad;
-}
+
+    // ROW, SHIFT: the tile's row stride and shifted-copy distance. Returns
+    // whether a sample above maxv was read.
+    template <int ROW, int SHIFT>
+    __device__ __forceinline__ bool store(uint16_t *dst, int maxv) const
+    {
+        bool bad = false;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int e = (int)threadIdx.x + k * kThreads;
+            if (N % kThreads != 0 && e >= N)
+                break;
+            const int r = e / V, v = e - r * V;
+            uint2 w = q[k];
+            if ((inside >> k) & 1) {
+                const uint32_t m = max(max(w.x & 0xffffu, w.x >> 16), max(w.y & 0xffffu, w.y >> 16));
+                bad |= (int)m > maxv;
+            } else {
+                w = {0xffffffffu, 0xffffffffu};
+            }
+            uint16_t *t = static_cast<uint16_t *>(__builtin_assume_aligned(dst + r * ROW + 4 + 4 * v, 8));
+            *reinterpret_cast<uint2 *>(t) = w;
+            uint16_t *u = t + SHIFT + 1;  // sample i of the tile at i + 1
+            u[0] = (uint16_t)w.x;
+            *reinterpret_cast<uint32_t *>(u + 1) = (w.x >> 16) | (w.y << 16);
+            u[3] = (uint16_t)(w.y >> 16);
+        }
+        return bad;
+    }
+};
 
 // Store a pair of output samples at element index i of a plane of u16
 // samples, or of bytes when u8 (8-bit frame output); the second sample only
@@ -408,6 +445,30 @@ __device__ __forceinline__ void store_pair(gptr<uint16_t> plane, size_t i, s16x2
     }
 }
 
+// The byte fields of a CdefFrame, read as two words (scalar loads; byte
+// loads would be vector loads, each waited on before its first use).
+struct FrameBytes {
+    uint32_t w0, w1;
+    __device__ __forceinline__ explicit FrameBytes(const CdefFrame &f)
+    {
+        static_assert(offsetof(CdefFrame, bit_depth) % 4 == 0 &&
+                      offsetof(CdefFrame, ss_x) == offsetof(CdefFrame, bit_depth) + 1 &&
+                      offsetof(CdefFrame, ss_y) == offsetof(CdefFrame, bit_depth) + 2 &&
+                      offsetof(CdefFrame, nplanes) == offsetof(CdefFrame, bit_depth) + 3 &&
+                      offsetof(CdefFrame, damping) == offsetof(CdefFrame, bit_depth) + 4 &&
+                      offsetof(CdefFrame, vec_ok) == offsetof(CdefFrame, bit_depth) + 5 &&
+                      offsetof(CdefFrame, out_u8) == offsetof(CdefFrame, bit_depth) + 6,
+                      "CdefFrame byte fields are read as two words");
+        __builtin_memcpy(&w0, &f.bit_depth, 4);
+        __builtin_memcpy(&w1, &f.damping, 4);
+    }
+    __device__ __forceinline__ int bit_depth() const { return (int)(w0 & 255u); }
+    __device__ __forceinline__ int nplanes() const { return (int)(w0 >> 24); }
+    __device__ __forceinline__ int damping() const { return (int)(w1 & 255u); }
+    __device__ __forceinline__ bool vec_ok() const { return ((w1 >> 8) & 255u) != 0; }
+    __device__ __forceinline__ bool out_u8() const { return ((w1 >> 16) & 255u) != 0; }
+};
+
 // The descriptor fields the phases after staging use, read at the start of
 // the workgroup so that their load latency hides under the staging loads.
 struct SbFields {
@@ -418,7 +479,7 @@ struct SbFields {
     int damping, nplanes;
     bool out_u8;
 
-    __device__ __forceinline__ void load(const CdefFrame &f)
+    __device__ __forceinline__ void load(const CdefFrame &f, const FrameBytes &fb)
     {
         static_assert(offsetof(CdefFrame, y_pri) % 4 == 0 && offsetof(CdefFrame, y_sec) ==
                       offsetof(CdefFrame, y_pri) + 8 && offsetof(CdefFrame, uv_sec) ==
@@ -434,9 +495,9 @@ struct SbFields {
             crop_w[p] = (int)f.crop_w[p];
             crop_h[p] = (int)f.crop_h[p];
         }
-        damping = f.damping;
-        nplanes = f.nplanes;
-        out_u8 = f.out_u8 != 0;
+        damping = fb.damping();
+        nplanes = fb.nplanes();
+        out_u8 = fb.out_u8();
     }
 
     // Preset ix of table t (0 y_pri, 1 y_sec, 2 uv_pri, 3 uv_sec).
@@ -446,6 +507,31 @@ struct SbFields {
         return (w >> (8 * (ix & 3))) & 255u;
     }
 };
+
+// Stage the superblock's windows of every plane (luma at x0-2, y0-2, 68x68;
+// chroma likewise subsampled) from 8-byte aligned planes: all loads first,
+// then all LDS stores. Returns whether a sample above maxv was read.
+template <int SX, int SY, bool BORDER>
+__device__ __forceinline__ bool stage_planes(const CdefFrame &f, uint16_t *L, uint16_t *U, uint16_t *V,
+                                             int nplanes, int x0, int y0, int maxv)
+{
+    constexpr int CW = 64 >> SX, CH = 64 >> SY;
+    constexpr int CROW = chroma_row<SX>(), CTILE = chroma_tile<SX, SY>();
+    const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
+    VecStage<68, 68, BORDER> l;
+    VecStage<CH + 4, CW + 4, BORDER> u, v;
+    l.load(glb(f.in[0]), (int)f.width[0], (int)f.height[0], (int)f.in_stride[0], x0 - 2, y0 - 2);
+    if (nplanes > 1) {
+        u.load(glb(f.in[1]), (int)f.width[1], (int)f.height[1], (int)f.in_stride[1], cx, cy);
+        v.load(glb(f.in[2]), (int)f.width[2], (int)f.height[2], (int)f.in_stride[2], cx, cy);
+    }
+    bool bad = l.template store<kRow, kTile>(L, maxv);
+    if (nplanes > 1) {
+        bad |= u.template store<CROW, CTILE>(U, maxv);
+        bad |= v.template store<CROW, CTILE>(V, maxv);
+    }
+    return bad;
+}
 
 // Filter (or pass through) every sample of a staged superblock and store it.
 // Luma: each half-wave takes one 8x8 block (32 lanes = 8 rows x 4 pairs of
@@ -535,29 +621,40 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     __shared__ Blk blk[2][64];
     __shared__ uint32_t cost[8][64];
     __shared__ uint8_t bact[64];
+    __shared__ int sidx;
 
     const uint32_t wg = XCD ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const CdefFrame &f = find_item(frames, n_frames, wg);
     const uint32_t sb = wg - f.block_begin;
     const int sby = (int)(sb / f.sb_cols), sbx = (int)(sb - (uint32_t)sby * f.sb_cols);
     const int tid = threadIdx.x;
-    const int bd = f.bit_depth, shift = bd - 8, maxv = (1 << bd) - 1;
-    const int nplanes = f.nplanes;
+    const FrameBytes fb(f);
+    const int bd = fb.bit_depth(), shift = bd - 8, maxv = (1 << bd) - 1;
+    const int nplanes = fb.nplanes();
     const int x0 = sbx * 64, y0 = sby * 64;
     const int W0 = f.width[0], H0 = f.height[0];
-    const int idx = glb(f.cdef_idx)[sby * f.idx_stride + sbx];
     SbFields sf;
-    sf.load(f);
+    sf.load(f, fb);
     const bool border = x0 == 0 || y0 == 0 || x0 + 64 >= W0 || y0 + 64 >= H0;
 
-    if (tid < 64) {  // decode.rs:774-777, 892-896
-        const int bx = x0 + (tid & 7) * 8, by = y0 + (tid >> 3) * 8;
-        bact[tid] = idx >= 0 && bx < W0 && by < H0 &&
-                    glb(f.skip8)[(size_t)(by >> 3) * f.skip_stride + (bx >> 3)] == 0;
+    // The superblock's cdef_idx and the skip flags of its 64 blocks
+    // (decode.rs:774-777, 892-896), loaded with the samples and stored to LDS
+    // with them; kept per lane until then, so that no wait for them comes
+    // before the sample loads are issued.
+    // Every wave loads them (lane l: block l & 63), so that no branch
+    // carries the result. Blocks outside the plane read the last block's
+    // flag (padded sizes are multiples of 8) and count as skipped.
+    uint32_t skipv;  // nonzero: skipped or outside the plane
+    int vidx;
+    {
+        const int bx = x0 + (tid & 7) * 8, by = y0 + ((tid >> 3) & 7) * 8;
+        const int bxc = min(bx, W0 - 8), byc = min(by, H0 - 8);
+        vidx = glb(f.cdef_idx)[sby * f.idx_stride + sbx];
+        skipv = glb(f.skip8)[(size_t)(byc >> 3) * f.skip_stride + (bxc >> 3)] | (bx < W0 && by < H0 ? 0u : 1u);
     }
     bool bad = false;
     const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
-    if (!f.vec_ok) {
+    if (!fb.vec_ok()) {
         bad = stage<68, 68, kRow, kTile>(L, glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border,
                                          maxv);
         if (nplanes > 1) {
@@ -571,29 +668,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
         // lie inside the plane, because padded sizes are multiples of 8 and
         // the superblock does not touch the last 8 rows or columns (likewise
         // for chroma), so no bounds are checked.
-        const int W1 = nplanes > 1 ? (int)f.width[1] : 0, H1 = nplanes > 1 ? (int)f.height[1] : 0;
-        if (border) {
-            bad = stage_vec<68, 68, kRow, kTile, true>(L, glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2,
-                                                       y0 - 2, maxv);
-            if (nplanes > 1) {
-                bad |= stage_vec<CH + 4, CW + 4, CROW, CTILE, true>(U, glb(f.in[1]), W1, H1, f.in_stride[1],
-                                                                   cx, cy, maxv);
-                bad |= stage_vec<CH + 4, CW + 4, CROW, CTILE, true>(V, glb(f.in[2]), W1, H1, f.in_stride[2],
-                                                                   cx, cy, maxv);
-            }
-        } else {
-            bad = stage_vec<68, 68, kRow, kTile, false>(L, glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2,
-                                                        y0 - 2, maxv);
-            if (nplanes > 1) {
-                bad |= stage_vec<CH + 4, CW + 4, CROW, CTILE, false>(U, glb(f.in[1]), W1, H1,
-                                                                    f.in_stride[1], cx, cy, maxv);
-                bad |= stage_vec<CH + 4, CW + 4, CROW, CTILE, false>(V, glb(f.in[2]), W1, H1,
-                                                                    f.in_stride[2], cx, cy, maxv);
-            }
-        }
+        if (border)
+            bad = stage_planes<SX, SY, true>(f, L, U, V, nplanes, x0, y0, maxv);
+        else
+            bad = stage_planes<SX, SY, false>(f, L, U, V, nplanes, x0, y0, maxv);
+    }
+    if (tid < 64) {
+        bact[tid] = vidx >= 0 && skipv == 0;
+        if (tid == 0)
+            sidx = vidx;
     }
     if (__syncthreads_or(bad) && tid == 0)
         raise_status(f.status, kFlagSampleRange);
+    const int idx = sidx;
 
     if (idx >= 0) {
         // Direction search: wave w < 8 computes direction w, lane = block.
