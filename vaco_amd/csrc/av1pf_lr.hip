@@ -41,13 +41,29 @@ struct Tile {
     LrRowSeg rs;
     LrColSeg cs;
     LrUnit u;
+    // The plane's byte fields, read as two words (scalar loads; byte loads
+    // would be vector loads, each waited on before its first use).
+    uint32_t b0, b1;
+    __device__ __forceinline__ int bit_depth() const { return (int)(b0 & 255u); }
+    __device__ __forceinline__ bool vec_ok() const { return ((b0 >> 16) & 255u) != 0; }
+    __device__ __forceinline__ bool out_u8() const { return (b0 >> 24) != 0; }
+    __device__ __forceinline__ bool out_vec() const { return (b1 & 255u) != 0; }
 };
+static_assert(offsetof(LrPlane, bit_depth) % 4 == 0 &&
+              offsetof(LrPlane, ss_y) == offsetof(LrPlane, bit_depth) + 1 &&
+              offsetof(LrPlane, vec_ok) == offsetof(LrPlane, bit_depth) + 2 &&
+              offsetof(LrPlane, out_u8) == offsetof(LrPlane, bit_depth) + 3 &&
+              offsetof(LrPlane, out_vec) == offsetof(LrPlane, bit_depth) + 4 &&
+              offsetof(LrPlane, out_vec) + 4 <= sizeof(LrPlane),
+              "LrPlane byte fields are read as two words");
 
 __device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, const uint32_t *list, uint32_t wg)
 {
     const uint32_t e = list[wg];
     Tile t;
     t.P = &planes[e >> 22];
+    __builtin_memcpy(&t.b0, &t.P->bit_depth, 4);
+    __builtin_memcpy(&t.b1, &t.P->out_vec, 4);
     const uint32_t local = e & 0x3fffffu;
     const uint32_t rsi = local / t.P->n_cols;
     t.rs = glb(t.P->rows)[rsi];
@@ -68,11 +84,12 @@ struct TileOut {
     bool u8;
 };
 
-__device__ __forceinline__ TileOut tile_out(const LrPlane &P, int x0, int y0)
+__device__ __forceinline__ TileOut tile_out(const Tile &t, int x0, int y0)
 {
+    const LrPlane &P = *t.P;
     const size_t o = (size_t)y0 * P.out_stride + x0;
     return TileOut{glb(P.out) + o, reinterpret_cast<gptr<uint8_t>>(glb(P.out)) + o, P.out_stride,
-                   P.out_u8 != 0};
+                   t.out_u8()};
 }
 
 // Store output sample v at row r, column c of the tile.
@@ -87,7 +104,12 @@ __device__ __forceinline__ void put(const TileOut &o, int r, int c, int v)
 
 // Stage Source::get (restoration.rs:233-244) over rows y0-3 .. y0+th+2 and
 // columns x0-3 .. x0+tw+2 into S (row stride kS). Returns whether a sample
-// above the bit depth was read.
+// above the bit depth was read. Source::get's row for a plane row y: y
+// clamped to the plane, then, outside the stripe, clamped to the stripe's
+// borders +-2 and read from the pre-CDEF image; inside it, from the CDEF
+// output. Both clamps together are one clamp to [start - 2, end + 2]. Every
+// thread issues all of its loads before it stores any of them; positions
+// past the window load a position inside it and are not stored.
 template <int THREADS>
 __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int maxv)
 {
@@ -95,53 +117,64 @@ __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int max
     const int W = P.width, H = P.height;
     const int x0 = t.cs.x0, y0 = t.rs.y0, ss = t.rs.stripe_start, se = t.rs.stripe_end;
     const int srows = t.rs.h + 6, scols = t.cs.w + 6;
+    const gptr<const uint16_t> before = glb(P.before), after = glb(P.after);
+    const uint32_t bst = P.before_stride, ast = P.after_stride;
+    auto row_of = [&](int r) {
+        const int yc = min(max(y0 - 3 + r, 0), H - 1);
+        const bool out = yc < ss || yc > se;
+        const int yr = min(max(yc, ss - 2), se + 2);
+        return (out ? before : after) + (size_t)(uint32_t)yr * (out ? bst : ast);  // 0 <= yr < H
+    };
     bool bad = false;
-    if (P.vec_ok && t.cs.w == 64 && x0 >= 4 && x0 + 68 <= W) {
+    if (t.vec_ok() && t.cs.w == 64 && x0 >= 4 && x0 + 68 <= W) {
         // Columns x0-4 .. x0+67 all inside the plane: 8-byte loads of four
         // samples (the column left of the window is loaded and dropped).
-        constexpr int V = 18;
-#pragma unroll 3
-        for (int e = threadIdx.x; e < 70 * V; e += THREADS) {
+        constexpr int V = 18, N = 70 * V, K = (N + THREADS - 1) / THREADS;
+        uint2 q[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int e = min((int)threadIdx.x + k * THREADS, N - 1);
+            const int r = min(e / V, srows - 1), v = e - (e / V) * V;
+            const uint64_t w = *reinterpret_cast<gptr<const uint64_t>>(row_of(r) + x0 - 4 + 4 * v);
+            q[k] = {(uint32_t)w, (uint32_t)(w >> 32)};
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int e = (int)threadIdx.x + k * THREADS;
             const int r = e / V, v = e - r * V;
-            if (r >= srows)
-                continue;
-            const int yc = min(max(y0 - 3 + r, 0), H - 1);
-            gptr<const uint16_t> row;
-            if (yc < ss)
-                row = glb(P.before) + (size_t)max(yc, ss - 2) * P.before_stride;
-            else if (yc > se)
-                row = glb(P.before) + (size_t)min(yc, se + 2) * P.before_stride;
-            else
-                row = glb(P.after) + (size_t)yc * P.after_stride;
-            const uint64_t w = *reinterpret_cast<gptr<const uint64_t>>(row + x0 - 4 + 4 * v);
-            const uint2 q = {(uint32_t)w, (uint32_t)(w >> 32)};
-            const uint32_t m = max(max(q.x & 0xffffu, q.x >> 16), max(q.y & 0xffffu, q.y >> 16));
+            if (e >= N || r >= srows)
+                break;
+            const uint2 w = q[k];
+            const uint32_t m = max(max(w.x & 0xffffu, w.x >> 16), max(w.y & 0xffffu, w.y >> 16));
             bad |= (int)m > maxv;
             uint16_t *d = S + r * kS + 4 * v - 1;  // column c of S is x0 - 3 + c
             if (v)
-                d[0] = (uint16_t)q.x;
-            d[1] = (uint16_t)(q.x >> 16);
-            d[2] = (uint16_t)q.y;
-            d[3] = (uint16_t)(q.y >> 16);
+                d[0] = (uint16_t)w.x;
+            d[1] = (uint16_t)(w.x >> 16);
+            d[2] = (uint16_t)w.y;
+            d[3] = (uint16_t)(w.y >> 16);
         }
         return bad;
     }
-#pragma unroll 10
-    for (int e = threadIdx.x; e < 70 * kS; e += THREADS) {
-        const int r = e / kS, c = e - r * kS;
-        if (r >= srows || c >= scols)
-            continue;
+    constexpr int N = 70 * kS, K = (N + THREADS - 1) / THREADS;
+    uint16_t q[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int e = min((int)threadIdx.x + k * THREADS, N - 1);
+        const int r = min(e / kS, srows - 1), c = min(e - (e / kS) * kS, scols - 1);
         const int xc = min(max(x0 - 3 + c, 0), W - 1);
-        const int yc = min(max(y0 - 3 + r, 0), H - 1);
-        int v;
-        if (yc < ss)
-            v = glb(P.before)[(size_t)max(yc, ss - 2) * P.before_stride + xc];
-        else if (yc > se)
-            v = glb(P.before)[(size_t)min(yc, se + 2) * P.before_stride + xc];
-        else
-            v = glb(P.after)[(size_t)yc * P.after_stride + xc];
-        bad |= v > maxv;
-        S[r * kS + c] = (uint16_t)v;
+        q[k] = row_of(r)[xc];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int e = (int)threadIdx.x + k * THREADS;
+        const int r = e / kS, c = e - r * kS;
+        if (e >= N || r >= srows)
+            break;
+        if (c < scols) {
+            bad |= q[k] > maxv;
+            S[r * kS + c] = q[k];
+        }
     }
     return bad;
 }
@@ -153,11 +186,11 @@ constexpr int kLrThreads = 512;
 __device__ __forceinline__ void lr_copy(const Tile &t)
 {
     const LrPlane &P = *t.P;
-    const int maxv = (1 << P.bit_depth) - 1;
+    const int maxv = (1 << t.bit_depth()) - 1;
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
-    const TileOut o = tile_out(P, x0, y0);
+    const TileOut o = tile_out(t, x0, y0);
     bool bad = false;
-    if (P.vec_ok && P.out_vec && tw == 64 && (x0 & 3) == 0) {
+    if (t.vec_ok() && t.out_vec() && tw == 64 && (x0 & 3) == 0) {
         // Four samples per load and store, every group 8-byte aligned in
         // both planes.
 #pragma unroll 2
@@ -203,9 +236,9 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
     uint16_t *S = reinterpret_cast<uint16_t *>(lds);
     int16_t *M = reinterpret_cast<int16_t *>(lds + 70 * kS * 2);
     const LrPlane &P = *t.P;
-    const int bd = P.bit_depth, maxv = (1 << bd) - 1;
+    const int bd = t.bit_depth(), maxv = (1 << bd) - 1;
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
-    const TileOut o = tile_out(P, x0, y0);
+    const TileOut o = tile_out(t, x0, y0);
     const int tid = threadIdx.x;
     if (__syncthreads_or(stage_source<kLrThreads>(S, t, maxv)) && tid == 0)
         raise_status(P.status, kFlagSampleRange);
@@ -436,9 +469,9 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
     uint32_t *X = reinterpret_cast<uint32_t *>(lds + 70 * kS * 2);
     uint16_t *xbx1 = reinterpret_cast<uint16_t *>(lds + 70 * kS * 2 + 66 * kS * 3 / 2 * 4);
     const LrPlane &P = *t.P;
-    const int bd = P.bit_depth, maxv = (1 << bd) - 1;
+    const int bd = t.bit_depth(), maxv = (1 << bd) - 1;
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
-    const TileOut o = tile_out(P, x0, y0);
+    const TileOut o = tile_out(t, x0, y0);
     const int tid = threadIdx.x;
     if (tid < 256) {  // x / (x + 1) table, restoration.rs:330-336
         const int z = tid;
