@@ -48,6 +48,17 @@ constexpr int bin_count(int d, int b)
     return c;
 }
 
+// Samples of bin b in rows i0 .. i0+3.
+constexpr int bin_count_rows(int d, int b, int i0)
+{
+    int c = 0;
+    for (int i = i0; i < i0 + 4; i++)
+        for (int j = 0; j < 8; j++)
+            if (bin_of(d, i, j) == b)
+                c++;
+    return c;
+}
+
 // LDS tiles. A plane's tile holds its rows y0-2 .. with column x0-8 at index
 // 0 (luma: 68 rows of kRow samples, columns x0-8 .. x0+71). Each tile is
 // followed by its shifted copy (index i at i + 1), SHIFT samples on. The
@@ -85,26 +96,53 @@ __device__ __forceinline__ int tap_offset(int dir, int k)
 }
 constexpr int kThreads = 1024;
 
+// In every direction's partition, row i + 4 of a column falls in the bin of
+// row i plus a constant (+4, +2, 0 or -2 by direction).
+constexpr int row4_offset(int d) { return bin_of(d, 4, 0) - bin_of(d, 0, 0); }
+
+constexpr bool row4_offset_is_uniform(int d)
+{
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 8; j++)
+            if (bin_of(d, i + 4, j) - bin_of(d, i, j) != row4_offset(d))
+                return false;
+    return true;
+}
+static_assert(row4_offset_is_uniform(0) && row4_offset_is_uniform(1) && row4_offset_is_uniform(2) &&
+                  row4_offset_is_uniform(3) && row4_offset_is_uniform(4) && row4_offset_is_uniform(5) &&
+                  row4_offset_is_uniform(6) && row4_offset_is_uniform(7),
+              "row i + 4 is binned at a fixed offset from row i");
+
 // Cost of direction D (cdef.rs:104-113) of the 8x8 block whose first row
 // starts at `p` (16-byte aligned LDS rows of kRow samples), with samples
-// taken >> shift. Rows stream into the 15 line sums, which stay in
-// registers; the -128 of cdef.rs:98 is applied once per bin as -128 *
-// count. For in-range samples every cost is below 840*64*128^2 < 2^31.
-template <int D>
+// taken >> shift. Samples (i, j) and (i + 4, j) are paired in one 32-bit
+// word (one byte permute per pair) and summed two at a time: the low half of
+// acc[b] collects bin b of rows 0-3 and the high half bin b + row4_offset(D)
+// of rows 4-7. A half sums at most 8 samples of at most 255, so it cannot
+// carry into the other. The -128 of cdef.rs:98 is applied once per bin as
+// -128 * count. For in-range samples every cost is below 840*64*128^2 < 2^31.
+// SH: shift is nonzero (8-bit frames skip the shifts).
+template <int D, bool SH>
 __device__ __forceinline__ uint32_t dir_cost(const uint16_t *p, int shift)
 {
-    int part[15];
+    constexpr int OFF = row4_offset(D);
+    const u16x2 sh = (u16x2)(unsigned short)shift;
+    u16x2 acc[15];
 #pragma unroll
     for (int b = 0; b < 15; b++)
-        part[b] = 0;
+        acc[b] = (u16x2)0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const uint4 q = *reinterpret_cast<const uint4 *>(p + i * kRow);
-        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+    for (int i = 0; i < 4; i++) {
+        const uint4 qa = *reinterpret_cast<const uint4 *>(p + i * kRow);
+        const uint4 qc = *reinterpret_cast<const uint4 *>(p + (i + 4) * kRow);
+        const uint32_t wa[4] = {qa.x, qa.y, qa.z, qa.w}, wc[4] = {qc.x, qc.y, qc.z, qc.w};
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const uint32_t s = (j & 1) ? (w[j >> 1] >> 16) : (w[j >> 1] & 0xffffu);
-            part[bin_of(D, i, j)] += (int)(s >> shift);
+        for (int m = 0; m < 4; m++) {
+            // (x[i][2m], x[i+4][2m]) and (x[i][2m+1], x[i+4][2m+1])
+            const u16x2 ev = __builtin_bit_cast(u16x2, __builtin_amdgcn_perm(wc[m], wa[m], 0x05040100u));
+            const u16x2 od = __builtin_bit_cast(u16x2, __builtin_amdgcn_perm(wc[m], wa[m], 0x07060302u));
+            acc[bin_of(D, i, 2 * m)] += SH ? ev >> sh : ev;
+            acc[bin_of(D, i, 2 * m + 1)] += SH ? od >> sh : od;
         }
     }
     uint32_t cost = 0;
@@ -112,25 +150,36 @@ __device__ __forceinline__ uint32_t dir_cost(const uint16_t *p, int shift)
     for (int b = 0; b < 15; b++) {
         const int cnt = bin_count(D, b);
         if (cnt != 0) {
-            const int s = part[b] - 128 * cnt;
+            int part = 0;
+            if (bin_count_rows(D, b, 0) != 0)
+                part += acc[b].x;
+            if (b - OFF >= 0 && b - OFF < 15 && bin_count_rows(D, b, 4) != 0)
+                part += acc[b - OFF].y;
+            const int s = part - 128 * cnt;
             cost += (uint32_t)(s * s) * (uint32_t)(840 / cnt);
         }
     }
     return cost;
 }
 
-__device__ __forceinline__ uint32_t dir_cost_any(int d, const uint16_t *p, int shift)
+template <bool SH>
+__device__ __forceinline__ uint32_t dir_cost_sh(int d, const uint16_t *p, int shift)
 {
     switch (d) {
-    case 0: return dir_cost<0>(p, shift);
-    case 1: return dir_cost<1>(p, shift);
-    case 2: return dir_cost<2>(p, shift);
-    case 3: return dir_cost<3>(p, shift);
-    case 4: return dir_cost<4>(p, shift);
-    case 5: return dir_cost<5>(p, shift);
-    case 6: return dir_cost<6>(p, shift);
-    default: return dir_cost<7>(p, shift);
+    case 0: return dir_cost<0, SH>(p, shift);
+    case 1: return dir_cost<1, SH>(p, shift);
+    case 2: return dir_cost<2, SH>(p, shift);
+    case 3: return dir_cost<3, SH>(p, shift);
+    case 4: return dir_cost<4, SH>(p, shift);
+    case 5: return dir_cost<5, SH>(p, shift);
+    case 6: return dir_cost<6, SH>(p, shift);
+    default: return dir_cost<7, SH>(p, shift);
     }
+}
+
+__device__ __forceinline__ uint32_t dir_cost_any(int d, const uint16_t *p, int shift)
+{
+    return shift ? dir_cost_sh<true>(d, p, shift) : dir_cost_sh<false>(d, p, shift);
 }
 
 // Filter parameters of one 8x8 block (one plane class), cdef.rs:35-54.
