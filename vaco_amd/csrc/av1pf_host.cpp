@@ -31,7 +31,7 @@ hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode
                        hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, int rows, hipStream_t s,
                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-hipError_t launch_lr(const LrPlane *planes, const uint32_t *list, uint32_t n, bool xcd,
+hipError_t launch_lr(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd,
                      hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s,
                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
@@ -606,8 +606,8 @@ struct Plan {
     std::vector<LrPlane> lr;
     uint32_t lr_blocks = 0;
     // restoration tiles by kernel: 0 copy, 1 Wiener, 2 self-guided;
-    // entry = LrPlane index << 22 | tile index within the plane
-    std::vector<uint32_t> lr_tiles[3];
+    // entry = tile index within the plane << 32 | LrPlane index << 2 | kind
+    std::vector<uint64_t> lr_tiles[3];
     struct Copy {
         DevPlane src, dst;
         uint32_t w, h;
@@ -852,8 +852,8 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
                         kind = k == VACO_AV1PF_UNIT_WIENER ? 1 : k == VACO_AV1PF_UNIT_SELF_GUIDED ? 2 : 0;
                     }
                     // plane order (one list) or by kind (three lists)
-                    plan.lr_tiles[plan.tune.lr_order ? 0 : kind].push_back(item << 22 |
-                                                                           (ri * L.n_cols + ci));
+                    plan.lr_tiles[plan.tune.lr_order ? 0 : kind].push_back(
+                        (uint64_t)(ri * L.n_cols + ci) << 32 | (uint64_t)item << 2 | (uint64_t)kind);
                     plan.bytes[VACO_AV1PF_KERNEL_RESTORATION] +=
                         (uint64_t)(*rows)[ri].h * (*cols)[ci].w * 4;
                 }
@@ -882,7 +882,7 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
         dj = d;
     }
     const LrPlane *dl = nullptr;
-    const uint32_t *dlist = nullptr;
+    const uint64_t *dlist = nullptr;
     uint32_t nlist[3] = {};
     if (!plan.lr.empty()) {
         TAKE(h, d, LrPlane, plan.lr.size());
@@ -892,9 +892,9 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
         for (int k = 0; k < 3; k++)
             n += nlist[k] = (uint32_t)plan.lr_tiles[k].size();
         // self-guided tiles first, then Wiener, then copies (k_lr's order)
-        TAKE(hl, dlk, uint32_t, n);
+        TAKE(hl, dlk, uint64_t, n);
         for (int k = 2; k >= 0; k--) {
-            memcpy(hl, plan.lr_tiles[k].data(), nlist[k] * sizeof(uint32_t));
+            memcpy(hl, plan.lr_tiles[k].data(), nlist[k] * sizeof(uint64_t));
             hl += nlist[k];
         }
         dlist = dlk;
@@ -1345,7 +1345,7 @@ static size_t arena_need(const vaco_av1pf_frame_desc &d, const FrameGeo &g)
         const size_t rows = (size_t)g.lr_h[p] / 8 + 8, cols = (size_t)g.lr_w[p] / 32 + 4;
         n += (size_t)d.lr_unit_count[p] * sizeof(LrUnit) + 512;
         n += rows * sizeof(LrRowSeg) + cols * sizeof(LrColSeg) + 512;
-        n += rows * cols * sizeof(uint32_t) + 256;            // tile list
+        n += rows * cols * sizeof(uint64_t) + 256;            // tile list
         n += ((size_t)g.lr_h[p] / 4 + 16) * sizeof(uint32_t);  // superres border rows
     }
     return n;
@@ -1793,16 +1793,16 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     uint32_t ucols, urows;
     lr_unit_counts(cfg->width, cfg->height, cfg->unit_size, &ucols, &urows);
     // tile lists by kernel kind (0 copy, 1 Wiener, 2 self-guided)
-    std::vector<uint32_t> tiles[3];
+    std::vector<uint64_t> tiles[3];
     for (uint32_t ri = 0; ri < rows.size(); ri++)
         for (uint32_t ci = 0; ci < cols.size(); ci++) {
             const uint8_t k = units[rows[ri].unit_row * ucols + cols[ci].unit_col].kind;
             const int kind = k == VACO_AV1PF_UNIT_WIENER ? 1 : k == VACO_AV1PF_UNIT_SELF_GUIDED ? 2 : 0;
-            tiles[kind].push_back(ri * (uint32_t)cols.size() + ci);
+            tiles[kind].push_back((uint64_t)(ri * (uint32_t)cols.size() + ci) << 32 | (uint64_t)kind);
         }
     const size_t ntiles = rows.size() * cols.size();
     const size_t bytes_tail = rows.size() * sizeof(LrRowSeg) + cols.size() * sizeof(LrColSeg) +
-                              n_units * sizeof(LrUnit) + sizeof(LrPlane) + ntiles * 4 + 10 * 256;
+                              n_units * sizeof(LrUnit) + sizeof(LrPlane) + ntiles * 8 + 10 * 256;
     uint8_t *buf = nullptr;
     HIPCHECK(hipMalloc(&buf, (nb + na + no) * 2 + bytes_tail));
     uint16_t *db = (uint16_t *)buf, *da = db + nb, *dout = da + na;
@@ -1816,14 +1816,14 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     LrUnit *du = (LrUnit *)p8;
     p8 += (n_units * sizeof(LrUnit) + 255) & ~size_t(255);
     // one list for k_lr: self-guided tiles, then Wiener, then copies
-    uint32_t *dlist = (uint32_t *)p8;
+    uint64_t *dlist = (uint64_t *)p8;
     uint32_t nlist[3];
-    std::vector<uint32_t> list;
+    std::vector<uint64_t> list;
     for (int k = 0; k < 3; k++)
         nlist[k] = (uint32_t)tiles[k].size();
     for (int k = 2; k >= 0; k--)
         list.insert(list.end(), tiles[k].begin(), tiles[k].end());
-    p8 += (ntiles * 4 + 255) & ~size_t(255);
+    p8 += (ntiles * 8 + 255) & ~size_t(255);
     uint32_t *dst = (uint32_t *)p8;
     LrPlane L;
     memset(&L, 0, sizeof L);
@@ -1863,7 +1863,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     if (e == hipSuccess && n_units)
         e = hipMemcpyAsync(du, units, n_units * sizeof(LrUnit), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && !list.empty())
-        e = hipMemcpyAsync(dlist, list.data(), list.size() * 4, hipMemcpyHostToDevice, ctx->stream);
+        e = hipMemcpyAsync(dlist, list.data(), list.size() * 8, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)

@@ -35,12 +35,18 @@ __constant__ int kSgr[16][4] = {
     {0, 0, 1, 11},  {0, 0, 1, 14},  {2, 30, 0, 0},  {2, 75, 0, 0},
 };
 
-// One tile of a list entry: item (LrPlane index) << 22 | tile within item.
+// One tile of a list entry: tile index within its item << 32 | item
+// (LrPlane index) << 2 | unit type (kUnitNone, kUnitWiener, kUnitSelfGuided;
+// kUnitNone also for planes without restoration). The type in the entry
+// settles the workgroup's path before any descriptor is read; the unit's
+// coefficients are read by the Wiener and self-guided paths with the staging
+// loads (UnitStage).
 struct Tile {
     const LrPlane *P;
+    int kind;
     LrRowSeg rs;
     LrColSeg cs;
-    LrUnit u;
+    gptr<const LrUnit> unit;  // kind != kUnitNone only
     // The plane's byte fields, read as two words (scalar loads; byte loads
     // would be vector loads, each waited on before its first use).
     uint32_t b0, b1;
@@ -57,20 +63,19 @@ static_assert(offsetof(LrPlane, bit_depth) % 4 == 0 &&
               offsetof(LrPlane, out_vec) + 4 <= sizeof(LrPlane),
               "LrPlane byte fields are read as two words");
 
-__device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, const uint32_t *list, uint32_t wg)
+__device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, const uint64_t *list, uint32_t wg)
 {
-    const uint32_t e = list[wg];
+    const uint64_t e = list[wg];
     Tile t;
-    t.P = &planes[e >> 22];
+    t.kind = (int)(e & 3u);
+    t.P = &planes[(uint32_t)e >> 2];
     __builtin_memcpy(&t.b0, &t.P->bit_depth, 4);
     __builtin_memcpy(&t.b1, &t.P->out_vec, 4);
-    const uint32_t local = e & 0x3fffffu;
+    const uint32_t local = (uint32_t)(e >> 32);
     const uint32_t rsi = local / t.P->n_cols;
     t.rs = glb(t.P->rows)[rsi];
     t.cs = glb(t.P->cols)[local - rsi * t.P->n_cols];
-    t.u.kind = kUnitNone;
-    if (t.P->units)
-        t.u = glb(t.P->units)[t.rs.unit_row * t.P->unit_cols + t.cs.unit_col];
+    t.unit = glb(t.P->units) + (t.rs.unit_row * t.P->unit_cols + t.cs.unit_col);
     return t;
 }
 
@@ -179,6 +184,27 @@ __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int max
     return bad;
 }
 
+// The tile's RestorationUnit, fetched alongside the staging loads: lanes
+// 0-8 each load one 16-bit word of it (per-lane addresses, so nothing waits
+// for them before the staging loads are issued), store it to LDS after the
+// staged samples, and every thread reads the unit back after the staging
+// barrier.
+struct UnitStage {
+    static constexpr int kWords = sizeof(LrUnit) / 2;
+    static_assert(sizeof(LrUnit) % 2 == 0 && alignof(LrUnit) >= 2, "unit read as 16-bit words");
+    uint32_t w = 0;
+    __device__ __forceinline__ void load(const Tile &t)
+    {
+        if (threadIdx.x < kWords)
+            w = reinterpret_cast<gptr<const uint16_t>>(t.unit)[threadIdx.x];
+    }
+    __device__ __forceinline__ void store(uint16_t *su) const
+    {
+        if (threadIdx.x < kWords)
+            su[threadIdx.x] = (uint16_t)w;
+    }
+};
+
 // ---------------------------------------------------------------- copy --
 
 constexpr int kLrThreads = 512;
@@ -230,6 +256,10 @@ __device__ __forceinline__ void lr_copy(const Tile &t)
 // -------------------------------------------------------------- Wiener --
 
 constexpr int kWienerLds = 70 * kS * 2 + 70 * 64 * 2;
+// Self-guided LDS: S (staged source), X (VS u16 + VQ, then the A/B grid), xbx1.
+constexpr int kSgrLds = 70 * kS * 2 + 66 * kS * 3 / 2 * 4 + 256 * 2;
+// The staged unit (UnitStage) sits past both paths' LDS.
+constexpr int kUnitLds = ((kSgrLds > kWienerLds ? kSgrLds : kWienerLds) + 15) & ~15;
 
 __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
 {
@@ -240,8 +270,14 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
     const TileOut o = tile_out(t, x0, y0);
     const int tid = threadIdx.x;
-    if (__syncthreads_or(stage_source<kLrThreads>(S, t, maxv)) && tid == 0)
+    uint16_t *su = reinterpret_cast<uint16_t *>(lds + kUnitLds);
+    UnitStage us;
+    us.load(t);
+    const bool bad = stage_source<kLrThreads>(S, t, maxv);
+    us.store(su);
+    if (__syncthreads_or(bad) && tid == 0)
         raise_status(P.status, kFlagSampleRange);
+    const LrUnit u = *reinterpret_cast<const LrUnit *>(su);
 
     // wiener, restoration.rs:264-283: rows, then columns. The intermediate
     // is clamped to [-2^(bd+6-rh), 2^(bd+8-rh) - 1 - 2^(bd+6-rh)], which
@@ -249,7 +285,6 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
     const int round_h = bd == 12 ? 5 : 3;
     const int offset = 1 << (bd + 7 - round_h - 1);
     const int limit = (1 << (bd + 1 + 7 - round_h)) - 1;
-    const LrUnit &u = t.u;
     const int h0 = u.horizontal[0], h1 = u.horizontal[1], h2 = u.horizontal[2];
     const int h3 = 128 - 2 * (h0 + h1 + h2);
     const int v0 = u.vertical[0], v1 = u.vertical[1], v2 = u.vertical[2];
@@ -460,9 +495,6 @@ __device__ __forceinline__ void sgr_pass1_strip(const uint32_t *X, int c, int rs
     }
 }
 
-// LDS: S (staged source), X (VS u16 + VQ, then the A/B grid), xbx1.
-constexpr int kSgrLds = 70 * kS * 2 + 66 * kS * 3 / 2 * 4 + 256 * 2;
-
 __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
 {
     uint16_t *S = reinterpret_cast<uint16_t *>(lds);
@@ -477,11 +509,17 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
         const int z = tid;
         xbx1[z] = z == 0 ? 1 : (uint16_t)(((z << 8) + z / 2) / (z + 1));
     }
-    if (__syncthreads_or(stage_source<kSgrThreads>(S, t, maxv)) && tid == 0)
+    uint16_t *su = reinterpret_cast<uint16_t *>(lds + kUnitLds);
+    UnitStage us;
+    us.load(t);
+    const bool bad = stage_source<kSgrThreads>(S, t, maxv);
+    us.store(su);
+    if (__syncthreads_or(bad) && tid == 0)
         raise_status(P.status, kFlagSampleRange);
+    const LrUnit u = *reinterpret_cast<const LrUnit *>(su);
 
     // self_guided, restoration.rs:373-384, with guided_pass 341-371.
-    const int set = t.u.set & 15;
+    const int set = u.set & 15;
     const int r0 = kSgr[set][0], e0 = kSgr[set][1], r1 = kSgr[set][2], e1 = kSgr[set][3];
     // Thread = column c and a strip of 8 output rows rs .. rs + 7 (its wave),
     // so each A/B grid row the strip needs is read once for all 8 rows.
@@ -515,7 +553,7 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
             sgr_grid<1, 12>(S, X, th, tw, y0, e1, xbx1);
         sgr_pass1_strip(X, c, rs, uu, f1);
     }
-    const int w0 = t.u.xqd[0], w1 = t.u.xqd[1];
+    const int w0 = u.xqd[0], w1 = u.xqd[1];
     if (c < tw) {
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -533,14 +571,14 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
 // XCD: workgroup b takes entry xcd_remap(b) (for lists in plane order).
 template <bool XCD>
 __global__ __launch_bounds__(kLrThreads) void k_lr(const LrPlane *__restrict__ planes,
-                                                   const uint32_t *__restrict__ list)
+                                                   const uint64_t *__restrict__ list)
 {
-    __shared__ __align__(16) uint8_t lds[kSgrLds > kWienerLds ? kSgrLds : kWienerLds];
+    __shared__ __align__(16) uint8_t lds[kUnitLds + 32];
     const uint32_t wg = XCD ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const Tile t = fetch_tile(planes, list, wg);
-    if (t.u.kind == kUnitSelfGuided)
+    if (t.kind == kUnitSelfGuided)
         lr_sgr(t, lds);
-    else if (t.u.kind == kUnitWiener)
+    else if (t.kind == kUnitWiener)
         lr_wiener(t, lds);
     else
         lr_copy(t);
@@ -548,7 +586,7 @@ __global__ __launch_bounds__(kLrThreads) void k_lr(const LrPlane *__restrict__ p
 
 // n tiles; xcd: list in plane order, numbered XCD-contiguously. ev0/ev1
 // (may be null) receive the dispatch's start and end.
-hipError_t launch_lr(const LrPlane *planes, const uint32_t *list, uint32_t n, bool xcd,
+hipError_t launch_lr(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd,
                      hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (n == 0)
