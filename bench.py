@@ -214,14 +214,18 @@ def kernel_table(prof, steps):
     return out
 
 
-def read_traffic(kernel):
+def read_traffic(kernel, tuning):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (FETCH_SIZE / WRITE_SIZE, scaled by the copy calibration), or None."""
+    summary (FETCH_SIZE / WRITE_SIZE, scaled by the copy calibration), for the
+    launch variant `tuning` selects (work items numbered XCD-contiguously or
+    not), or None."""
     path = os.path.join(ROOT, "profiles", "pmc_hbm_traffic_1080p_per_frame.md")
+    xcd = {"cdef": tuning.get("cdef_xcd_order"), "restoration": tuning.get("lr_order")}.get(kernel)
     try:
         for line in open(path):
             if line.startswith("traffic-json:"):
-                return json.loads(line.split(":", 1)[1]).get(kernel)
+                per_variant = json.loads(line.split(":", 1)[1]).get(kernel) or {}
+                return per_variant.get("xcd" if xcd else "plain")
     except OSError:
         pass
     return None
@@ -473,7 +477,7 @@ def main():
     kern = per_frame["kernels"]
     dominant = max(kern, key=lambda k: kern[k]["ms_per_step"])
     d = kern[dominant]
-    traffic = read_traffic(dominant)
+    traffic = read_traffic(dominant, per_frame["tuning"])
     rec = {
         "metric": METRIC,
         "value": per_frame["mpix_s"],

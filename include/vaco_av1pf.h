@@ -401,7 +401,8 @@ vaco_av1pf_status vaco_av1pf_set_tuning(vaco_av1pf_ctx *ctx, const vaco_av1pf_tu
 /* Runs `n` frames (device planes, as vaco_av1pf_batch_device) `reps` times
  * with each variant of each kernel, times the kernels by their dispatch
  * events, keeps the fastest variant per kernel as the context's tuning and
- * returns it in *out. Synchronous; the outputs hold the frames' results. */
+ * returns it in *out. A field leaves its current value only for a variant
+ * at least 2% faster. Synchronous; the outputs hold the frames' results. */
 vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n,
                                       const vaco_av1pf_frame_desc *descs,
                                       const vaco_av1pf_picture *ins, vaco_av1pf_picture *outs,

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <new>
 #include <vector>
 
@@ -1595,57 +1596,34 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
         return r;
     };
     vaco_av1pf_tuning best = ctx->tune, t;
-    double tm, best_ms;
-    // superres rows per workgroup
-    best_ms = -1;
-    for (uint8_t rows : {4, 8, 16}) {
-        t = best;
-        t.superres_rows = rows;
-        if ((st = measure(t, VACO_AV1PF_KERNEL_SUPERRES, &tm)) != VACO_AV1PF_OK)
-            break;
-        if (tm > 0 && (best_ms < 0 || tm < best_ms)) {  // kernels absent from the batch time 0
-            best_ms = tm;
-            best.superres_rows = rows;
+    // One field at a time: the current value is timed first, and another
+    // replaces it only when at least 2% faster, so that timing noise between
+    // equivalent variants does not flip the choice. A kernel absent from the
+    // batch times 0 and keeps its value.
+    auto tune_field = [&](uint8_t vaco_av1pf_tuning::*field, std::initializer_list<uint8_t> values,
+                          int kernel) {
+        double keep_ms = 0, tm = 0;
+        if (st != VACO_AV1PF_OK || (st = measure(best, kernel, &keep_ms)) != VACO_AV1PF_OK || keep_ms <= 0)
+            return;
+        uint8_t keep = best.*field;
+        for (uint8_t v : values) {
+            if (v == best.*field)
+                continue;
+            t = best;
+            t.*field = v;
+            if ((st = measure(t, kernel, &tm)) != VACO_AV1PF_OK)
+                return;
+            if (tm > 0 && tm < keep_ms * 0.98) {
+                keep_ms = tm;
+                keep = v;
+            }
         }
-    }
-    // CDEF superblock order
-    best_ms = -1;
-    for (uint8_t v = 0; v < 2 && st == VACO_AV1PF_OK; v++) {
-        t = best;
-        t.cdef_xcd_order = v;
-        if ((st = measure(t, VACO_AV1PF_KERNEL_CDEF, &tm)) != VACO_AV1PF_OK)
-            break;
-        if (tm > 0 && (best_ms < 0 || tm < best_ms)) {  // kernels absent from the batch time 0
-            best_ms = tm;
-            best.cdef_xcd_order = v;
-        }
-    }
-    // restoration tile order
-    best_ms = -1;
-    for (uint8_t v = 0; v < 2 && st == VACO_AV1PF_OK; v++) {
-        t = best;
-        t.lr_order = v;
-        if ((st = measure(t, VACO_AV1PF_KERNEL_RESTORATION, &tm)) != VACO_AV1PF_OK)
-            break;
-        if (tm > 0 && (best_ms < 0 || tm < best_ms)) {  // kernels absent from the batch time 0
-            best_ms = tm;
-            best.lr_order = v;
-        }
-    }
-    // restoration tile height
-    best_ms = -1;
-    for (uint8_t rows : {64, 32, 16}) {
-        if (st != VACO_AV1PF_OK)
-            break;
-        t = best;
-        t.lr_tile_rows = rows;
-        if ((st = measure(t, VACO_AV1PF_KERNEL_RESTORATION, &tm)) != VACO_AV1PF_OK)
-            break;
-        if (tm > 0 && (best_ms < 0 || tm < best_ms)) {  // kernels absent from the batch time 0
-            best_ms = tm;
-            best.lr_tile_rows = rows;
-        }
-    }
+        best.*field = keep;
+    };
+    tune_field(&vaco_av1pf_tuning::superres_rows, {4, 8, 16}, VACO_AV1PF_KERNEL_SUPERRES);
+    tune_field(&vaco_av1pf_tuning::cdef_xcd_order, {0, 1}, VACO_AV1PF_KERNEL_CDEF);
+    tune_field(&vaco_av1pf_tuning::lr_order, {0, 1}, VACO_AV1PF_KERNEL_RESTORATION);
+    tune_field(&vaco_av1pf_tuning::lr_tile_rows, {64, 32, 16}, VACO_AV1PF_KERNEL_RESTORATION);
     ctx->tune = best;
     const vaco_av1pf_status ps = vaco_av1pf_profile(ctx, was_profiling ? 1 : 0);
     if (st != VACO_AV1PF_OK)
