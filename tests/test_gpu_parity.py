@@ -132,6 +132,62 @@ def test_gpu_frame_random_geometry_matches_oracle(ctx, case):
     assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), str(case))
 
 
+# Planes at a 2-byte offset with odd row strides: every kernel takes its
+# unaligned path (no 8-byte loads or stores), and nothing may be written
+# outside the output planes' rows.
+UNALIGNED_CASES = [
+    dict(seed=60, coded_w=131, coded_h=97),
+    dict(seed=61, coded_w=93, coded_h=75, bd=10, upscaled_w=160),
+    dict(seed=62, coded_w=72, coded_h=80, bd=12, ssx=0, ssy=0, lr=False),
+    dict(seed=63, coded_w=97, coded_h=70, bd=10, ssx=1, ssy=0, upscaled_w=150, unit=32),
+]
+_FILL = 0xABCD
+
+
+def _strided(shape, stride, src=None):
+    """A device buffer with a plane of `shape` from sample 1 on, rows `stride`
+    samples apart; every other sample holds _FILL."""
+    from vaco_amd import hip
+    h, w = shape
+    buf = np.full(1 + (h - 1) * stride + w + 7, _FILL, np.uint16)
+    if src is not None:
+        for y in range(h):
+            buf[1 + y * stride:1 + y * stride + w] = src[y]
+    return hip.DeviceArray.from_host(buf)
+
+
+def _unstride(buf, shape, stride):
+    h, w = shape
+    plane = np.stack([buf[1 + y * stride:1 + y * stride + w] for y in range(h)])
+    rest = buf.copy()
+    for y in range(h):
+        rest[1 + y * stride:1 + y * stride + w] = _FILL
+    return plane, bool(np.all(rest == _FILL))
+
+
+@pytest.mark.parametrize("case", UNALIGNED_CASES, ids=lambda c: "s%d" % c["seed"])
+def test_gpu_unaligned_strided_planes_match_oracle(ctx, case):
+    say(f"unaligned strided planes {case}")
+    f = F.random_frame(**case)
+    want = O.frame(f.desc, f.planes)
+    in_st = [p.shape[1] + 3 for p in f.planes]
+    geo = f.output_geometry()
+    out_st = [g[1] + 5 for g in geo]
+    ins = [_strided(p.shape, st, p) for p, st in zip(f.planes, in_st)]
+    outs = [_strided(g, st) for g, st in zip(geo, out_st)]
+    ctx.device_call(f, [d.ptr + 2 for d in ins], [d.ptr + 2 for d in outs],
+                    in_strides=in_st, out_strides=out_st)()
+    ctx.finish()
+    got = []
+    for d, g, st in zip(outs, geo, out_st):
+        plane, untouched = _unstride(d.download(), g, st)
+        assert untouched, f"{case}: a store landed outside the output plane"
+        got.append(plane)
+    assert_frame_equal(got, want, f"unaligned {case}")
+    for d, p, st in zip(ins, f.planes, in_st):
+        assert np.array_equal(_unstride(d.download(), p.shape, st)[0], p), "input modified"
+
+
 def test_gpu_frame_cfg2_4k_superres_matches_oracle(ctx):
     say("cfg2 frame 0 (4K 10-bit superres)")
     f = av1pf.generate(workloads.cfg2(), 0)

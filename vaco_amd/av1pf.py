@@ -285,16 +285,20 @@ class Context:
             return outs, f.value
         return outs
 
-    def device_call(self, frame: Frame, in_ptrs, out_ptrs, stream: int | None = None):
+    def device_call(self, frame: Frame, in_ptrs, out_ptrs, stream: int | None = None,
+                    in_strides=None, out_strides=None):
         """A zero-argument callable that enqueues the stage on these device
-        planes (integer device addresses, dense rows). The ctypes arguments
-        are built once, so repeated calls cost one foreign call each."""
+        planes (integer device addresses; dense rows unless row strides, in
+        samples, are given). The ctypes arguments are built once, so repeated
+        calls cost one foreign call each."""
         geo_in = [(p.shape[0], p.shape[1]) for p in frame.planes]
         geo_out = frame.output_geometry()
         pin, pout = abi.Picture(), abi.Picture()
         for i in range(frame.nplanes):
-            pin.plane[i] = abi.device_plane(in_ptrs[i], geo_in[i][1], geo_in[i][0])
-            pout.plane[i] = abi.device_plane(out_ptrs[i], geo_out[i][1], geo_out[i][0])
+            pin.plane[i] = abi.device_plane(in_ptrs[i], geo_in[i][1], geo_in[i][0],
+                                            in_strides[i] if in_strides else None)
+            pout.plane[i] = abi.device_plane(out_ptrs[i], geo_out[i][1], geo_out[i][0],
+                                             out_strides[i] if out_strides else None)
         fn, h = self._lib.vaco_av1pf_frame_device, self.handle
         d, pi, po, st = C.byref(frame.desc), C.byref(pin), C.byref(pout), C.c_void_p(stream or 0)
         check = self._check

@@ -75,7 +75,9 @@ __device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, const uint64_t
     const uint32_t rsi = local / t.P->n_cols;
     t.rs = glb(t.P->rows)[rsi];
     t.cs = glb(t.P->cols)[local - rsi * t.P->n_cols];
-    t.unit = glb(t.P->units) + (t.rs.unit_row * t.P->unit_cols + t.cs.unit_col);
+    t.unit = nullptr;
+    if (t.kind != kUnitNone)  // planes without restoration have no units
+        t.unit = glb(t.P->units) + (t.rs.unit_row * t.P->unit_cols + t.cs.unit_col);
     return t;
 }
 
