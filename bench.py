@@ -218,14 +218,16 @@ def read_traffic(kernel, tuning):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE / WRITE_SIZE, scaled by the copy calibration), for the
     launch variant `tuning` selects (work items numbered XCD-contiguously or
-    not), or None."""
+    not; for CDEF also the workgroup size), or None."""
     path = os.path.join(ROOT, "profiles", "pmc_hbm_traffic_1080p_per_frame.md")
     xcd = {"cdef": tuning.get("cdef_xcd_order"), "restoration": tuning.get("lr_order")}.get(kernel)
+    order = "xcd" if xcd else "plain"
+    key = f"{order}-{64 * tuning.get('cdef_waves', 16)}" if kernel == "cdef" else order
     try:
         for line in open(path):
             if line.startswith("traffic-json:"):
                 per_variant = json.loads(line.split(":", 1)[1]).get(kernel) or {}
-                return per_variant.get("xcd" if xcd else "plain")
+                return per_variant.get(key)
     except OSError:
         pass
     return None

@@ -392,6 +392,7 @@ typedef struct vaco_av1pf_tuning {
     uint8_t lr_order;        /* 0: restoration tiles by unit type, self-guided first;
                                 1: in plane order, numbered XCD-contiguously */
     uint8_t lr_tile_rows;    /* restoration tile height: 64, 32 or 16 rows */
+    uint8_t cdef_waves;      /* waves per CDEF workgroup (one superblock): 16 or 8 */
 } vaco_av1pf_tuning;
 
 vaco_av1pf_status vaco_av1pf_get_tuning(const vaco_av1pf_ctx *ctx, vaco_av1pf_tuning *out);

@@ -132,6 +132,36 @@ def test_gpu_frame_random_geometry_matches_oracle(ctx, case):
     assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), str(case))
 
 
+def test_gpu_eight_wave_cdef_workgroups_match_oracle(ctx):
+    """CDEF with 512-thread workgroups (tuning cdef_waves = 8): every random
+    geometry, 1080p, a mixed batch, and an out-of-range sample late in a
+    superblock, which must still raise the reference's error."""
+    from vaco_amd import hip
+    say("cdef_waves=8: random frames, cfg1 frame 0, batch, error")
+    ctx.tuning = {"cdef_waves": 8}
+    try:
+        for case in RANDOM_CASES:
+            f = F.random_frame(**case)
+            assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), f"cdef_waves=8 {case}")
+        f = av1pf.generate(workloads.cfg1(), 0)
+        assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), "cdef_waves=8 cfg1/0")
+        frames = [F.random_frame(**c) for c in RANDOM_CASES[:10]]
+        dev_in = [[hip.DeviceArray.from_host(p) for p in fr.planes] for fr in frames]
+        dev_out = [[hip.DeviceArray(g) for g in fr.output_geometry()] for fr in frames]
+        ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
+        ctx.batch_device(frames, ptr(dev_in), ptr(dev_out))
+        ctx.finish()
+        for k, fr in enumerate(frames):
+            assert_frame_equal([t.download() for t in dev_out[k]], O.frame(fr.desc, fr.planes), f"batch/{k}")
+        f = F.random_frame(46, 64, 64, skip_p=0.0, idx_neg_p=0.0)
+        f.planes[0][60, 50] = 999
+        with pytest.raises(av1pf.Av1Error) as e:
+            ctx.frame(f)
+        assert e.value.message == "CDEF sample exceeds bit depth"
+    finally:
+        ctx.tuning = {"cdef_waves": 16}
+
+
 # Planes at a 2-byte offset with odd row strides: every kernel takes its
 # unaligned path (no 8-byte loads or stores), and nothing may be written
 # outside the output planes' rows.

@@ -175,6 +175,7 @@ class Tuning(C.Structure):
         ("cdef_xcd_order", C.c_uint8),
         ("lr_order", C.c_uint8),
         ("lr_tile_rows", C.c_uint8),
+        ("cdef_waves", C.c_uint8),
     ]
 
 

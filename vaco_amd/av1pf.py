@@ -373,7 +373,7 @@ class Context:
 
     # ---- tuning
 
-    TUNING_FIELDS = ("superres_rows", "cdef_xcd_order", "lr_order", "lr_tile_rows")
+    TUNING_FIELDS = ("superres_rows", "cdef_xcd_order", "lr_order", "lr_tile_rows", "cdef_waves")
 
     @property
     def tuning(self) -> dict:

@@ -94,7 +94,6 @@ __device__ __forceinline__ int tap_offset(int dir, int k)
     const uint64_t t = k ? (dir < 4 ? t10 : t11) : (dir < 4 ? t00 : t01);
     return (int16_t)(uint16_t)(t >> (16 * (dir & 3)));
 }
-constexpr int kThreads = 1024;
 
 // In every direction's partition, row i + 4 of a column falls in the bin of
 // row i plus a constant (+4, +2, 0 or -2 by direction).
@@ -385,14 +384,15 @@ __device__ __forceinline__ void filter_pairs(const uint16_t *t, int ci, const Bl
 
 // Stage rows ys .. ys+ROWS-1, columns xs .. xs+COLS-1 of a plane into an LDS
 // tile of row stride ROW at column offset 6, and into the shifted copy at
-// dst + SHIFT; positions outside the plane hold kUnavailable.
-template <int ROWS, int COLS, int ROW, int SHIFT>
+// dst + SHIFT; positions outside the plane hold kUnavailable. T: threads
+// of the workgroup.
+template <int ROWS, int COLS, int ROW, int SHIFT, int T>
 __device__ __forceinline__ bool stage(uint16_t *dst, gptr<const uint16_t> src, int W, int H, int st,
                                       int xs, int ys, bool border, int maxv)
 {
     bool bad = false;
 #pragma unroll 5
-    for (int e = threadIdx.x; e < ROWS * COLS; e += kThreads) {
+    for (int e = threadIdx.x; e < ROWS * COLS; e += T) {
         const int r = e / COLS, c = e - r * COLS;
         const int x = xs + c, y = ys + r;
         uint16_t v = kUnavailable;
@@ -416,11 +416,11 @@ __device__ __forceinline__ bool stage(uint16_t *dst, gptr<const uint16_t> src, i
 // group lies wholly inside or wholly outside the plane; outside (BORDER only)
 // it holds kUnavailable. An outside group's load reads the nearest group
 // inside the plane instead, so every load is in bounds and unconditional.
-template <int ROWS, int COLS, bool BORDER>
+template <int ROWS, int COLS, bool BORDER, int T>
 struct VecStage {
     static constexpr int V = (COLS + 4) / 4;  // 4-sample groups per row
     static constexpr int N = ROWS * V;
-    static constexpr int K = (N + kThreads - 1) / kThreads;
+    static constexpr int K = (N + T - 1) / T;
     static_assert(K <= 32, "one validity bit per group");
     uint2 q[K];
     uint32_t inside;  // bit k: group k lies in the plane
@@ -430,7 +430,7 @@ struct VecStage {
         inside = 0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const int e = min((int)threadIdx.x + k * kThreads, N - 1);
+            const int e = min((int)threadIdx.x + k * T, N - 1);
             const int r = e / V, v = e - r * V;
             int y = ys + r, x = xs - 2 + 4 * v;
             if (BORDER) {
@@ -454,8 +454,8 @@ struct VecStage {
         bool bad = false;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const int e = (int)threadIdx.x + k * kThreads;
-            if (N % kThreads != 0 && e >= N)
+            const int e = (int)threadIdx.x + k * T;
+            if (N % T != 0 && e >= N)
                 break;
             const int r = e / V, v = e - r * V;
             uint2 w = q[k];
@@ -560,15 +560,15 @@ struct SbFields {
 // Stage the superblock's windows of every plane (luma at x0-2, y0-2, 68x68;
 // chroma likewise subsampled) from 8-byte aligned planes: all loads first,
 // then all LDS stores. Returns whether a sample above maxv was read.
-template <int SX, int SY, bool BORDER>
+template <int SX, int SY, bool BORDER, int T>
 __device__ __forceinline__ bool stage_planes(const CdefFrame &f, uint16_t *L, uint16_t *U, uint16_t *V,
                                              int nplanes, int x0, int y0, int maxv)
 {
     constexpr int CW = 64 >> SX, CH = 64 >> SY;
     constexpr int CROW = chroma_row<SX>(), CTILE = chroma_tile<SX, SY>();
     const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
-    VecStage<68, 68, BORDER> l;
-    VecStage<CH + 4, CW + 4, BORDER> u, v;
+    VecStage<68, 68, BORDER, T> l;
+    VecStage<CH + 4, CW + 4, BORDER, T> u, v;
     l.load(glb(f.in[0]), (int)f.width[0], (int)f.height[0], (int)f.in_stride[0], x0 - 2, y0 - 2);
     if (nplanes > 1) {
         u.load(glb(f.in[1]), (int)f.width[1], (int)f.height[1], (int)f.in_stride[1], cx, cy);
@@ -584,14 +584,14 @@ __device__ __forceinline__ bool stage_planes(const CdefFrame &f, uint16_t *L, ui
 
 // Filter (or pass through) every sample of a staged superblock and store it.
 // Luma: each half-wave takes one 8x8 block (32 lanes = 8 rows x 4 pairs of
-// columns), blocks hw and hw + 32 for half-wave hw; all its lanes then read
+// columns), blocks hw, hw + T / 32, ... for half-wave hw; all its lanes then read
 // their taps at one direction's offsets, which the luma row stride keeps in
 // distinct banks. Chroma: 16 lanes per block, pairs of U then V. BORDER: the
 // superblock touches a plane edge (unavailable taps; partial superblocks
 // store only samples inside the crop; widths are even, so a pair is all in
 // or all out, except at an odd crop width where the second sample is
 // dropped). L, U, V: the planes' tiles.
-template <bool BORDER, int SX, int SY>
+template <bool BORDER, int SX, int SY, int T>
 __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, const uint16_t *U,
                                           const uint16_t *V, const Blk (*blk)[64], const uint8_t *bact,
                                           bool any, int x0, int y0, int ix)
@@ -604,12 +604,13 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
     constexpr int BW = 8 >> SX, BH = 8 >> SY, PPP = BW * BH / 2, CPX = BW * BH / 16;
     const int tid = threadIdx.x;
     {
+        constexpr int HW = T / 32;  // half-waves: luma blocks per pass
         const int hw = tid >> 5, l = tid & 31, i = l >> 2, j = (l & 3) * 2;
         gptr<uint16_t> dst = f.out[0];
         const int ost = f.out_stride[0], W0 = f.crop_w[0], H0 = f.crop_h[0];
 #pragma unroll
-        for (int pass = 0; pass < 2; pass++) {
-            const int b = hw + 32 * pass, bi = b >> 3, bj = b & 7;
+        for (int pass = 0; pass < 64 / HW; pass++) {
+            const int b = hw + HW * pass, bi = b >> 3, bj = b & 7;
             const bool act = any && bact[b];
             const int ci = (2 + bi * 8 + i) * kRow + 8 + bj * 8 + j;
             s16x2 res;
@@ -624,8 +625,12 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
                 store_pair(dst, (size_t)y * ost + x, res, f.out_u8, x + 1 < W0);
         }
     }
-    if (f.nplanes > 1) {
-        const int b = tid >> 4, q = tid & 15;
+    constexpr int CB = T / 16;  // chroma blocks per pass
+#pragma unroll
+    for (int cp = 0; cp < 64 / CB; cp++) {
+        if (f.nplanes == 1)
+            break;
+        const int b = (tid >> 4) + CB * cp, q = tid & 15;
         const int bi = b >> 3, bj = b & 7;
         const bool act = any && bact[b];
         Blk B = {};
@@ -656,9 +661,10 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
 // One 64x64 luma superblock of one frame, all planes (decode.rs:772-856).
 // SX, SY: chroma subsampling (monochrome frames use <0, 0> with 1 plane).
 // XCD: superblocks numbered XCD-contiguously (xcd_remap).
-template <int SX, int SY, bool XCD>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cdef(const CdefFrame *__restrict__ frames, int n_frames)
+template <int SX, int SY, bool XCD, int T>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cdef(const CdefFrame *__restrict__ frames, int n_frames)
 {
+    static_assert(T == 512 || T == 1024, "the direction search takes 512 threads");
     constexpr int CW = 64 >> SX, CH = 64 >> SY;
     constexpr int CROW = chroma_row<SX>(), CTILE = chroma_tile<SX, SY>();
     // Luma tile and its shifted copy, then U and V, each with its copy. V
@@ -704,12 +710,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     bool bad = false;
     const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
     if (!fb.vec_ok()) {
-        bad = stage<68, 68, kRow, kTile>(L, glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border,
+        bad = stage<68, 68, kRow, kTile, T>(L, glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border,
                                          maxv);
         if (nplanes > 1) {
-            bad |= stage<CH + 4, CW + 4, CROW, CTILE>(U, glb(f.in[1]), f.width[1], f.height[1],
+            bad |= stage<CH + 4, CW + 4, CROW, CTILE, T>(U, glb(f.in[1]), f.width[1], f.height[1],
                                                       f.in_stride[1], cx, cy, border, maxv);
-            bad |= stage<CH + 4, CW + 4, CROW, CTILE>(V, glb(f.in[2]), f.width[2], f.height[2],
+            bad |= stage<CH + 4, CW + 4, CROW, CTILE, T>(V, glb(f.in[2]), f.width[2], f.height[2],
                                                       f.in_stride[2], cx, cy, border, maxv);
         }
     } else {
@@ -718,9 +724,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
         // the superblock does not touch the last 8 rows or columns (likewise
         // for chroma), so no bounds are checked.
         if (border)
-            bad = stage_planes<SX, SY, true>(f, L, U, V, nplanes, x0, y0, maxv);
+            bad = stage_planes<SX, SY, true, T>(f, L, U, V, nplanes, x0, y0, maxv);
         else
-            bad = stage_planes<SX, SY, false>(f, L, U, V, nplanes, x0, y0, maxv);
+            bad = stage_planes<SX, SY, false, T>(f, L, U, V, nplanes, x0, y0, maxv);
     }
     if (tid < 64) {
         bact[tid] = vidx >= 0 && skipv == 0;
@@ -786,9 +792,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
 
     const bool any = idx >= 0;
     if (border)
-        filter_sb<true, SX, SY>(sf, L, U, V, blk, bact, any, x0, y0, idx & 7);
+        filter_sb<true, SX, SY, T>(sf, L, U, V, blk, bact, any, x0, y0, idx & 7);
     else
-        filter_sb<false, SX, SY>(sf, L, U, V, blk, bact, any, x0, y0, idx & 7);
+        filter_sb<false, SX, SY, T>(sf, L, U, V, blk, bact, any, x0, y0, idx & 7);
 }
 
 // Kernel-level cases (cdef::find_direction + cdef::filter_block on one
@@ -849,31 +855,39 @@ __global__ __launch_bounds__(64) void k_cdef_cases(const uint16_t *__restrict__ 
     outs[(size_t)cs * 64 + lane] = res;
 }
 
-template <bool XCD>
+template <bool XCD, int T>
 static void launch_cdef_mode(const CdefFrame *frames, int n, uint32_t blocks, int mode,
                              hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
-    const dim3 g(blocks), b(kThreads);
+    const dim3 g(blocks), b(T);
     switch (mode) {
-    case 0: hipExtLaunchKernelGGL((k_cdef<1, 1, XCD>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
-    case 1: hipExtLaunchKernelGGL((k_cdef<0, 0, XCD>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
-    case 2: hipExtLaunchKernelGGL((k_cdef<1, 0, XCD>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
-    default: hipExtLaunchKernelGGL((k_cdef<0, 1, XCD>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    case 0: hipExtLaunchKernelGGL((k_cdef<1, 1, XCD, T>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    case 1: hipExtLaunchKernelGGL((k_cdef<0, 0, XCD, T>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    case 2: hipExtLaunchKernelGGL((k_cdef<1, 0, XCD, T>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    default: hipExtLaunchKernelGGL((k_cdef<0, 1, XCD, T>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
     }
 }
 
 // mode: 0 = 4:2:0, 1 = 4:4:4 or monochrome, 2 = 4:2:2, 3 = 4:4:0. xcd:
-// XCD-contiguous superblock order. ev0/ev1 (may be null) receive the
+// XCD-contiguous superblock order. waves: 16 or 8 waves (1024 or 512
+// threads) per superblock's workgroup. ev0/ev1 (may be null) receive the
 // dispatch's start and end.
-hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, bool xcd,
+hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, bool xcd, int waves,
                        hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (blocks == 0)
         return hipSuccess;
-    if (xcd)
-        launch_cdef_mode<true>(frames, n, blocks, mode, s, ev0, ev1);
-    else
-        launch_cdef_mode<false>(frames, n, blocks, mode, s, ev0, ev1);
+    if (waves == 8) {
+        if (xcd)
+            launch_cdef_mode<true, 512>(frames, n, blocks, mode, s, ev0, ev1);
+        else
+            launch_cdef_mode<false, 512>(frames, n, blocks, mode, s, ev0, ev1);
+    } else {
+        if (xcd)
+            launch_cdef_mode<true, 1024>(frames, n, blocks, mode, s, ev0, ev1);
+        else
+            launch_cdef_mode<false, 1024>(frames, n, blocks, mode, s, ev0, ev1);
+    }
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@
 #include "av1pf_internal.h"
 
 namespace av1pf {
-hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, bool xcd,
+hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, bool xcd, int waves,
                        hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, int rows, hipStream_t s,
                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
@@ -121,7 +121,7 @@ struct vaco_av1pf_ctx {
         uint32_t a, b;
         uint64_t bytes;
     };
-    vaco_av1pf_tuning tune = {8, 1, 0, 64};  // launch variants (vaco_av1pf_autotune)
+    vaco_av1pf_tuning tune = {8, 1, 0, 64, 16};  // launch variants (vaco_av1pf_autotune)
     bool use_graphs = false;                // replay kernel chains as hipGraphs
     bool prof = false;
     std::vector<hipEvent_t> prof_pool;
@@ -614,7 +614,7 @@ struct Plan {
         uint32_t w, h;
     };  // w x h samples; dst.u8: to bytes
     std::vector<Copy> copies;  // frames in which no stage runs
-    vaco_av1pf_tuning tune = {8, 1, 0, 64};  // launch variants this plan is built for
+    vaco_av1pf_tuning tune = {8, 1, 0, 64, 16};  // launch variants this plan is built for
     // algorithmic bytes per kernel id (samples read once + written once, 2 B)
     uint64_t bytes[VACO_AV1PF_KERNEL_COUNT] = {};
     uint64_t cdef_bytes[4] = {};  // CDEF by chroma layout
@@ -919,8 +919,8 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
                                       [&](hipEvent_t e0, hipEvent_t e1) {
                                           return launch_cdef(dc[m], (int)plan.cdef[m].size(),
                                                              plan.cdef_blocks[m], m,
-                                                             plan.tune.cdef_xcd_order != 0, s, e0,
-                                                             e1);
+                                                             plan.tune.cdef_xcd_order != 0,
+                                                             plan.tune.cdef_waves, s, e0, e1);
                                       }));
         if (dj)
             HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES,
@@ -950,7 +950,8 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
         key.insert(key.end(), {(uint64_t)(uintptr_t)dc[m], plan.cdef_blocks[m], plan.cdef[m].size()});
     key.insert(key.end(), {(uint64_t)(uintptr_t)dj, plan.sr_blocks, plan.sr.size(),
                            (uint64_t)(uintptr_t)dl, (uint64_t)(uintptr_t)dlist, n_tiles,
-                           plan.tune.superres_rows, plan.tune.cdef_xcd_order, plan.tune.lr_order});
+                           plan.tune.superres_rows, plan.tune.cdef_xcd_order, plan.tune.lr_order,
+                           plan.tune.cdef_waves});
     hipGraphExec_t exec = nullptr;
     for (auto &gr : a->graphs)
         if (gr.key == key)
@@ -1543,7 +1544,8 @@ static bool tuning_ok(const vaco_av1pf_tuning &t)
 {
     return (t.superres_rows == 4 || t.superres_rows == 8 || t.superres_rows == 16) &&
            t.cdef_xcd_order <= 1 && t.lr_order <= 1 &&
-           (t.lr_tile_rows == 64 || t.lr_tile_rows == 32 || t.lr_tile_rows == 16);
+           (t.lr_tile_rows == 64 || t.lr_tile_rows == 32 || t.lr_tile_rows == 16) &&
+           (t.cdef_waves == 16 || t.cdef_waves == 8);
 }
 
 extern "C" vaco_av1pf_status vaco_av1pf_get_tuning(const vaco_av1pf_ctx *ctx, vaco_av1pf_tuning *out)
@@ -1622,6 +1624,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
     };
     tune_field(&vaco_av1pf_tuning::superres_rows, {4, 8, 16}, VACO_AV1PF_KERNEL_SUPERRES);
     tune_field(&vaco_av1pf_tuning::cdef_xcd_order, {0, 1}, VACO_AV1PF_KERNEL_CDEF);
+    tune_field(&vaco_av1pf_tuning::cdef_waves, {16, 8}, VACO_AV1PF_KERNEL_CDEF);
     tune_field(&vaco_av1pf_tuning::lr_order, {0, 1}, VACO_AV1PF_KERNEL_RESTORATION);
     tune_field(&vaco_av1pf_tuning::lr_tile_rows, {64, 32, 16}, VACO_AV1PF_KERNEL_RESTORATION);
     ctx->tune = best;
