@@ -46,7 +46,7 @@ struct Tile {
     int kind;
     LrRowSeg rs;
     LrColSeg cs;
-    gptr<const LrUnit> unit;  // kind != kUnitNone only
+    uint32_t unit_ix;  // index of the tile's unit in P->units (kind != kUnitNone only)
     // The plane's byte fields, read as two words (scalar loads; byte loads
     // would be vector loads, each waited on before its first use).
     uint32_t b0, b1;
@@ -75,9 +75,7 @@ __device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, const uint64_t
     const uint32_t rsi = local / t.P->n_cols;
     t.rs = glb(t.P->rows)[rsi];
     t.cs = glb(t.P->cols)[local - rsi * t.P->n_cols];
-    t.unit = nullptr;
-    if (t.kind != kUnitNone)  // planes without restoration have no units
-        t.unit = glb(t.P->units) + (t.rs.unit_row * t.P->unit_cols + t.cs.unit_col);
+    t.unit_ix = t.rs.unit_row * t.P->unit_cols + t.cs.unit_col;
     return t;
 }
 
@@ -198,7 +196,7 @@ struct UnitStage {
     __device__ __forceinline__ void load(const Tile &t)
     {
         if (threadIdx.x < kWords)
-            w = reinterpret_cast<gptr<const uint16_t>>(t.unit)[threadIdx.x];
+            w = reinterpret_cast<gptr<const uint16_t>>(glb(t.P->units) + t.unit_ix)[threadIdx.x];
     }
     __device__ __forceinline__ void store(uint16_t *su) const
     {
