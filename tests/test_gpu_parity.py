@@ -162,6 +162,23 @@ def test_gpu_eight_wave_cdef_workgroups_match_oracle(ctx):
         ctx.tuning = {"cdef_waves": 16}
 
 
+@pytest.mark.parametrize("rows", [32, 16])
+def test_gpu_short_restoration_tiles_match_oracle(ctx, rows):
+    """Restoration cut into 32- or 16-row tiles (tuning lr_tile_rows), which
+    run on workgroups of 8 threads per row (256 or 128): every random
+    geometry and 1080p."""
+    say(f"lr_tile_rows={rows}: random frames, cfg1 frame 0")
+    ctx.tuning = {"lr_tile_rows": rows}
+    try:
+        for case in RANDOM_CASES:
+            f = F.random_frame(**case)
+            assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), f"lr_tile_rows={rows} {case}")
+        f = av1pf.generate(workloads.cfg1(), 0)
+        assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), f"lr_tile_rows={rows} cfg1/0")
+    finally:
+        ctx.tuning = {"lr_tile_rows": 64}
+
+
 # Planes at a 2-byte offset with odd row strides: every kernel takes its
 # unaligned path (no 8-byte loads or stores), and nothing may be written
 # outside the output planes' rows.

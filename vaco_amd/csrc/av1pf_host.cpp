@@ -32,7 +32,7 @@ hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode
                        hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, int rows, hipStream_t s,
                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-hipError_t launch_lr(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd,
+hipError_t launch_lr(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd, int rows,
                      hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s,
                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
@@ -933,8 +933,8 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
             HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION,
                                   plan.bytes[VACO_AV1PF_KERNEL_RESTORATION], s,
                                   [&](hipEvent_t e0, hipEvent_t e1) {
-                                      return launch_lr(dl, dlist, n_tiles, plan.tune.lr_order != 0, s,
-                                                       e0, e1);
+                                      return launch_lr(dl, dlist, n_tiles, plan.tune.lr_order != 0,
+                                                       plan.tune.lr_tile_rows, s, e0, e1);
                                   }));
         return VACO_AV1PF_OK;
     };
@@ -1848,7 +1848,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     if (e == hipSuccess)
         e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-        e = launch_lr(dl, dlist, nlist[0] + nlist[1] + nlist[2], false, ctx->stream);
+        e = launch_lr(dl, dlist, nlist[0] + nlist[1] + nlist[2], false, 64, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpy2DAsync(out->data, out->stride * 2, dout, cfg->width * 2, cfg->width * 2,
                              cfg->height, hipMemcpyDeviceToHost, ctx->stream);

@@ -7,15 +7,16 @@
 //
 // A tile is at most 64x64 output samples inside one restoration stripe and
 // one restoration unit, so its filter type is uniform. The host sorts tiles
-// by type and one launch of k_lr (512 threads) covers all of them; each
+// by type and one launch of k_lr (8 threads per row of the tuned tile
+// height: 512, 256 or 128) covers all of them; each
 // workgroup branches once on its tile's unit type. The host orders the list
 // (tuning): either self-guided tiles, then Wiener, then copies, so the
 // longest tiles start first, or plane order numbered XCD-contiguously, so
 // neighbouring tiles share halo lines in one L2. Each kind is a branch-free
 // path of its own:
 //   copy        RestorationUnit::None and planes without restoration
-//   Wiener      ~19 KB LDS: staged source, 16-bit row pass, column pass
-//   self-guided ~39 KB LDS: staged source, column sums of x and x^2, box
+//   Wiener      ~19 KB LDS at 64 rows: staged source, 16-bit row pass, column pass
+//   self-guided ~39 KB LDS at 64 rows: staged source, column sums of x and x^2, box
 //               sums per centre, A/B grids (built in registers and written
 //               over the column sums), 3x3 weighted combination
 // Each staged window follows Source::get: CDEF output inside the stripe,
@@ -115,7 +116,7 @@ __device__ __forceinline__ void put(const TileOut &o, int r, int c, int v)
 // output. Both clamps together are one clamp to [start - 2, end + 2]. Every
 // thread issues all of its loads before it stores any of them; positions
 // past the window load a position inside it and are not stored.
-template <int THREADS>
+template <int ROWS, int THREADS>
 __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int maxv)
 {
     const LrPlane &P = *t.P;
@@ -134,7 +135,7 @@ __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int max
     if (t.vec_ok() && t.cs.w == 64 && x0 >= 4 && x0 + 68 <= W) {
         // Columns x0-4 .. x0+67 all inside the plane: 8-byte loads of four
         // samples (the column left of the window is loaded and dropped).
-        constexpr int V = 18, N = 70 * V, K = (N + THREADS - 1) / THREADS;
+        constexpr int V = 18, N = (ROWS + 6) * V, K = (N + THREADS - 1) / THREADS;
         uint2 q[K];
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -161,7 +162,7 @@ __device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int max
         }
         return bad;
     }
-    constexpr int N = 70 * kS, K = (N + THREADS - 1) / THREADS;
+    constexpr int N = (ROWS + 6) * kS, K = (N + THREADS - 1) / THREADS;
     uint16_t q[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -207,8 +208,22 @@ struct UnitStage {
 
 // ---------------------------------------------------------------- copy --
 
-constexpr int kLrThreads = 512;
+// Workgroup size and LDS layout of k_lr for tiles of at most ROWS rows (64,
+// 32 or 16; the host cuts tiles to the tuned height): 8 threads per tile
+// row, so that the self-guided strips (64 columns x 8 rows) and the Wiener
+// column blocks (32 column pairs x 4 rows) cover the tile with every thread.
+template <int ROWS>
+struct LrShape {
+    static constexpr int kThreads = 8 * ROWS;
+    static constexpr int kSrcBytes = (ROWS + 6) * kS * 2;                // staged source
+    static constexpr int kWienerLds = kSrcBytes + (ROWS + 6) * 64 * 2;   // + row-filtered M
+    static constexpr int kXBytes = (ROWS + 2) * kS * 3 / 2 * 4;          // VS + VQ, then A/B
+    static constexpr int kSgrLds = kSrcBytes + kXBytes + 256 * 2;        // + x/(x+1) table
+    // The staged unit (UnitStage) sits past both paths' LDS.
+    static constexpr int kUnitLds = ((kSgrLds > kWienerLds ? kSgrLds : kWienerLds) + 15) & ~15;
+};
 
+template <int T>
 __device__ __forceinline__ void lr_copy(const Tile &t)
 {
     const LrPlane &P = *t.P;
@@ -220,7 +235,7 @@ __device__ __forceinline__ void lr_copy(const Tile &t)
         // Four samples per load and store, every group 8-byte aligned in
         // both planes.
 #pragma unroll 2
-        for (int e = threadIdx.x; e < 64 * 16; e += kLrThreads) {
+        for (int e = threadIdx.x; e < 64 * 16; e += T) {
             const int r = e >> 4, c = (e & 15) * 4;
             if (r >= th)
                 break;
@@ -241,7 +256,7 @@ __device__ __forceinline__ void lr_copy(const Tile &t)
         return;
     }
 #pragma unroll 8
-    for (int e = threadIdx.x; e < 64 * 64; e += kLrThreads) {
+    for (int e = threadIdx.x; e < 64 * 64; e += T) {
         const int r = e >> 6, c = e & 63;
         if (r < th && c < tw) {
             const int v = glb(P.after)[(size_t)(y0 + r) * P.after_stride + x0 + c];
@@ -255,25 +270,23 @@ __device__ __forceinline__ void lr_copy(const Tile &t)
 
 // -------------------------------------------------------------- Wiener --
 
-constexpr int kWienerLds = 70 * kS * 2 + 70 * 64 * 2;
-// Self-guided LDS: S (staged source), X (VS u16 + VQ, then the A/B grid), xbx1.
-constexpr int kSgrLds = 70 * kS * 2 + 66 * kS * 3 / 2 * 4 + 256 * 2;
-// The staged unit (UnitStage) sits past both paths' LDS.
-constexpr int kUnitLds = ((kSgrLds > kWienerLds ? kSgrLds : kWienerLds) + 15) & ~15;
 
+template <int ROWS>
 __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
 {
+    using Sh = LrShape<ROWS>;
+    constexpr int T = Sh::kThreads;
     uint16_t *S = reinterpret_cast<uint16_t *>(lds);
-    int16_t *M = reinterpret_cast<int16_t *>(lds + 70 * kS * 2);
+    int16_t *M = reinterpret_cast<int16_t *>(lds + Sh::kSrcBytes);
     const LrPlane &P = *t.P;
     const int bd = t.bit_depth(), maxv = (1 << bd) - 1;
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
     const TileOut o = tile_out(t, x0, y0);
     const int tid = threadIdx.x;
-    uint16_t *su = reinterpret_cast<uint16_t *>(lds + kUnitLds);
+    uint16_t *su = reinterpret_cast<uint16_t *>(lds + Sh::kUnitLds);
     UnitStage us;
     us.load(t);
-    const bool bad = stage_source<kLrThreads>(S, t, maxv);
+    const bool bad = stage_source<ROWS, T>(S, t, maxv);
     us.store(su);
     if (__syncthreads_or(bad) && tid == 0)
         raise_status(P.status, kFlagSampleRange);
@@ -292,7 +305,7 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
     const int srows = th + 6;
     // Rows: four adjacent outputs per item from ten staged samples (two
     // 8-byte and one 4-byte LDS read), stored as one 8-byte M write.
-    for (int e = tid; e < srows * 16; e += kLrThreads) {
+    for (int e = tid; e < srows * 16; e += T) {
         const int r = e >> 4, c = (e & 15) * 4;
         const uint16_t *s = &S[r * kS + c];
         const uint2 a = *reinterpret_cast<const uint2 *>(s);
@@ -316,8 +329,8 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
         *reinterpret_cast<uint2 *>(&M[r * 64 + c]) = make_uint2(m[0], m[1]);
     }
     __syncthreads();
-    // Columns: a 4-row x 2-column block per thread (16 x 32 blocks cover the
-    // tile) from ten 4-byte M reads, each holding both columns.
+    // Columns: a 4-row x 2-column block per thread (ROWS / 4 x 32 blocks
+    // cover the tile) from ten 4-byte M reads, each holding both columns.
     {
         const int cb = tid & 31, rb = tid >> 5;
         const int c = 2 * cb, r0 = 4 * rb;
@@ -350,7 +363,6 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
 
 // --------------------------------------------------------- self-guided --
 
-constexpr int kSgrThreads = kLrThreads;
 
 // A/B of one box from its sums (restoration.rs:314-339), packed as
 // b2 << 9 | a2 (a2 <= 256; b2 < 2^21 for 8/10/12-bit samples), in exact
@@ -380,13 +392,14 @@ __device__ __forceinline__ uint32_t ab_of(int sum, int sumsq, uint32_t scale, ui
 // x0-1+cc), cr < th+2, cc < tw+2, the packed A/B, written over VS/VQ as
 // grid AB (row stride kG). Pass 0 (R = 2) needs odd rows y only
 // (restoration.rs:347-356), so it walks only those centre rows.
-template <int R, int BD>
+template <int R, int BD, int ROWS>
 __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th, int tw, int y0,
                                          int eps, const uint16_t *xbx1)
 {
     constexpr int N = (2 * R + 1) * (2 * R + 1);
     constexpr int STEP = R == 2 ? 2 : 1;  // centre rows walked
-    constexpr int KC = ((R == 2 ? 34 : 66) * kS + kSgrThreads - 1) / kSgrThreads;
+    constexpr int T = LrShape<ROWS>::kThreads;
+    constexpr int KC = ((R == 2 ? ROWS / 2 + 2 : ROWS + 2) * kS + T - 1) / T;
     const uint32_t n2e = (uint32_t)(N * N * eps);
     const uint32_t scale = ((1u << 20) + n2e / 2) / n2e;
     const uint32_t thr = ((255u << 20) - (1u << 19) + scale - 1) / scale;
@@ -395,9 +408,9 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
     const int nr = (crows - cr0 + STEP - 1) / STEP;
     const int tid = threadIdx.x;
     uint16_t *VS = reinterpret_cast<uint16_t *>(X);
-    int *VQ = reinterpret_cast<int *>(X) + 66 * kS / 2;
+    int *VQ = reinterpret_cast<int *>(X) + (ROWS + 2) * kS / 2;
 #pragma unroll 2
-    for (int e = tid; e < nr * kS; e += kSgrThreads) {
+    for (int e = tid; e < nr * kS; e += T) {
         const int k = e / kS, c = e - k * kS, cr = cr0 + STEP * k;
         if (c >= scols)
             continue;
@@ -415,7 +428,7 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
     uint32_t ab[KC];
 #pragma unroll
     for (int i = 0; i < KC; i++) {
-        const int e = tid + i * kSgrThreads;
+        const int e = tid + i * T;
         const int k = e / kS, cc = e - k * kS, cr = cr0 + STEP * k;
         ab[i] = 0;
         if (k >= nr || cc >= ccols)
@@ -431,7 +444,7 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < KC; i++) {
-        const int e = tid + i * kSgrThreads;
+        const int e = tid + i * T;
         const int k = e / kS, cc = e - k * kS, cr = cr0 + STEP * k;
         if (k < nr && cc < ccols)
             X[cr * kG + cc] = ab[i];
@@ -495,24 +508,25 @@ __device__ __forceinline__ void sgr_pass1_strip(const uint32_t *X, int c, int rs
     }
 }
 
+template <int ROWS>
 __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
 {
+    using Sh = LrShape<ROWS>;
+    constexpr int T = Sh::kThreads;
     uint16_t *S = reinterpret_cast<uint16_t *>(lds);
-    uint32_t *X = reinterpret_cast<uint32_t *>(lds + 70 * kS * 2);
-    uint16_t *xbx1 = reinterpret_cast<uint16_t *>(lds + 70 * kS * 2 + 66 * kS * 3 / 2 * 4);
+    uint32_t *X = reinterpret_cast<uint32_t *>(lds + Sh::kSrcBytes);
+    uint16_t *xbx1 = reinterpret_cast<uint16_t *>(lds + Sh::kSrcBytes + Sh::kXBytes);
     const LrPlane &P = *t.P;
     const int bd = t.bit_depth(), maxv = (1 << bd) - 1;
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
     const TileOut o = tile_out(t, x0, y0);
     const int tid = threadIdx.x;
-    if (tid < 256) {  // x / (x + 1) table, restoration.rs:330-336
-        const int z = tid;
+    for (int z = tid; z < 256; z += T)  // x / (x + 1) table, restoration.rs:330-336
         xbx1[z] = z == 0 ? 1 : (uint16_t)(((z << 8) + z / 2) / (z + 1));
-    }
-    uint16_t *su = reinterpret_cast<uint16_t *>(lds + kUnitLds);
+    uint16_t *su = reinterpret_cast<uint16_t *>(lds + Sh::kUnitLds);
     UnitStage us;
     us.load(t);
-    const bool bad = stage_source<kSgrThreads>(S, t, maxv);
+    const bool bad = stage_source<ROWS, T>(S, t, maxv);
     us.store(su);
     if (__syncthreads_or(bad) && tid == 0)
         raise_status(P.status, kFlagSampleRange);
@@ -532,11 +546,11 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
     }
     if (r0) {
         if (bd == 8)
-            sgr_grid<2, 8>(S, X, th, tw, y0, e0, xbx1);
+            sgr_grid<2, 8, ROWS>(S, X, th, tw, y0, e0, xbx1);
         else if (bd == 10)
-            sgr_grid<2, 10>(S, X, th, tw, y0, e0, xbx1);
+            sgr_grid<2, 10, ROWS>(S, X, th, tw, y0, e0, xbx1);
         else
-            sgr_grid<2, 12>(S, X, th, tw, y0, e0, xbx1);
+            sgr_grid<2, 12, ROWS>(S, X, th, tw, y0, e0, xbx1);
         if ((y0 + rs) & 1)
             sgr_pass0_strip<1>(X, c, rs, uu, f0);
         else
@@ -546,11 +560,11 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
         if (r0)
             __syncthreads();  // the pass-0 grid is fully consumed
         if (bd == 8)
-            sgr_grid<1, 8>(S, X, th, tw, y0, e1, xbx1);
+            sgr_grid<1, 8, ROWS>(S, X, th, tw, y0, e1, xbx1);
         else if (bd == 10)
-            sgr_grid<1, 10>(S, X, th, tw, y0, e1, xbx1);
+            sgr_grid<1, 10, ROWS>(S, X, th, tw, y0, e1, xbx1);
         else
-            sgr_grid<1, 12>(S, X, th, tw, y0, e1, xbx1);
+            sgr_grid<1, 12, ROWS>(S, X, th, tw, y0, e1, xbx1);
         sgr_pass1_strip(X, c, rs, uu, f1);
     }
     const int w0 = u.xqd[0], w1 = u.xqd[1];
@@ -569,32 +583,49 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
 
 // One launch over every tile of the plan, one workgroup per list entry.
 // XCD: workgroup b takes entry xcd_remap(b) (for lists in plane order).
-template <bool XCD>
-__global__ __launch_bounds__(kLrThreads) void k_lr(const LrPlane *__restrict__ planes,
-                                                   const uint64_t *__restrict__ list)
+// ROWS: no Wiener or self-guided tile is taller (copy tiles, which use no
+// LDS, may be up to 64 rows).
+template <bool XCD, int ROWS>
+__global__ __launch_bounds__(LrShape<ROWS>::kThreads) void k_lr(const LrPlane *__restrict__ planes,
+                                                               const uint64_t *__restrict__ list)
 {
-    __shared__ __align__(16) uint8_t lds[kUnitLds + 32];
+    __shared__ __align__(16) uint8_t lds[LrShape<ROWS>::kUnitLds + 32];
     const uint32_t wg = XCD ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const Tile t = fetch_tile(planes, list, wg);
     if (t.kind == kUnitSelfGuided)
-        lr_sgr(t, lds);
+        lr_sgr<ROWS>(t, lds);
     else if (t.kind == kUnitWiener)
-        lr_wiener(t, lds);
+        lr_wiener<ROWS>(t, lds);
     else
-        lr_copy(t);
+        lr_copy<LrShape<ROWS>::kThreads>(t);
 }
 
-// n tiles; xcd: list in plane order, numbered XCD-contiguously. ev0/ev1
-// (may be null) receive the dispatch's start and end.
-hipError_t launch_lr(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd,
+template <int ROWS>
+static void launch_lr_rows(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd, hipStream_t s,
+                           hipEvent_t ev0, hipEvent_t ev1)
+{
+    const dim3 g(n), b(LrShape<ROWS>::kThreads);
+    if (xcd)
+        hipExtLaunchKernelGGL((k_lr<true, ROWS>), g, b, 0, s, ev0, ev1, 0, planes, list);
+    else
+        hipExtLaunchKernelGGL((k_lr<false, ROWS>), g, b, 0, s, ev0, ev1, 0, planes, list);
+}
+
+// n tiles; xcd: list in plane order, numbered XCD-contiguously. rows: the
+// tallest Wiener or self-guided tile (64, 32 or 16; the workgroup takes 8
+// threads per row). ev0/ev1 (may be null) receive the dispatch's start and
+// end.
+hipError_t launch_lr(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd, int rows,
                      hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (n == 0)
         return hipSuccess;
-    if (xcd)
-        hipExtLaunchKernelGGL(k_lr<true>, dim3(n), dim3(kLrThreads), 0, s, ev0, ev1, 0, planes, list);
+    if (rows <= 16)
+        launch_lr_rows<16>(planes, list, n, xcd, s, ev0, ev1);
+    else if (rows <= 32)
+        launch_lr_rows<32>(planes, list, n, xcd, s, ev0, ev1);
     else
-        hipExtLaunchKernelGGL(k_lr<false>, dim3(n), dim3(kLrThreads), 0, s, ev0, ev1, 0, planes, list);
+        launch_lr_rows<64>(planes, list, n, xcd, s, ev0, ev1);
     return hipGetLastError();
 }
 
