@@ -60,7 +60,7 @@ def measure_mixed_batch(av1pf, hip, workloads, reps: int, n: int = 64):
     ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
     per_frame = [ctx.device_call(f, ptr(dev_in)[k], ptr(out_pf)[k]) for k, f in enumerate(frames)]
     batch = ctx.batch_call(frames, ptr(dev_in), ptr(out_b))
-    tuning = ctx.autotune(frames, ptr(dev_in), ptr(out_b), reps=2)  # untimed
+    tuning = ctx.autotune(frames, ptr(dev_in), ptr(out_b), reps=4)  # untimed
     for c in per_frame:
         c()
     batch()
@@ -351,7 +351,7 @@ def measure_4k10(av1pf, hip, workloads, reps: int, n: int = 16):
     ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
     calls = [ctx.device_call(f, ptr(dev_in)[k], ptr(out_pf)[k]) for k, f in enumerate(frames)]
     batch = ctx.batch_call(frames, ptr(dev_in), ptr(out_b))
-    tuning = ctx.autotune(frames, ptr(dev_in), ptr(out_b), reps=1)  # untimed
+    tuning = ctx.autotune(frames, ptr(dev_in), ptr(out_b), reps=4)  # untimed
     for c in calls:
         c()
     batch()

@@ -400,10 +400,11 @@ vaco_av1pf_status vaco_av1pf_get_tuning(const vaco_av1pf_ctx *ctx, vaco_av1pf_tu
  * made after it returns. */
 vaco_av1pf_status vaco_av1pf_set_tuning(vaco_av1pf_ctx *ctx, const vaco_av1pf_tuning *tuning);
 /* Runs `n` frames (device planes, as vaco_av1pf_batch_device) `reps` times
- * with each variant of each kernel, times the kernels by their dispatch
- * events, keeps the fastest variant per kernel as the context's tuning and
- * returns it in *out. A field leaves its current value only for a variant
- * at least 2% faster. Synchronous; the outputs hold the frames' results. */
+ * with each variant of each kernel (after one untimed run), times the
+ * kernels by their dispatch events (the fastest of the `reps` runs counts),
+ * keeps the fastest variant per kernel as the context's tuning and returns
+ * it in *out. A field leaves its current value only for a variant at least
+ * 2% faster. Synchronous; the outputs hold the frames' results. */
 vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n,
                                       const vaco_av1pf_frame_desc *descs,
                                       const vaco_av1pf_picture *ins, vaco_av1pf_picture *outs,

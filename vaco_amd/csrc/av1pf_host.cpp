@@ -1581,20 +1581,25 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
     double ms[VACO_AV1PF_KERNEL_COUNT];
     uint64_t launches[VACO_AV1PF_KERNEL_COUNT];
     // Mean time of `kernel` per batch with tuning t (one untimed batch first).
+    // Fastest of `reps` batches (the minimum is robust to a batch slowed by
+    // something else on the device), after one untimed batch.
     auto measure = [&](const vaco_av1pf_tuning &t, int kernel, double *res) {
         ctx->tune = t;
+        *res = 0.0;
         vaco_av1pf_status r = av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr);
         if (r == VACO_AV1PF_OK)
             r = vaco_av1pf_finish(ctx);
         if (r == VACO_AV1PF_OK)
             r = vaco_av1pf_profile_read(ctx, nullptr, nullptr, nullptr);
-        for (uint32_t i = 0; i < reps && r == VACO_AV1PF_OK; i++)
+        for (uint32_t i = 0; i < reps && r == VACO_AV1PF_OK; i++) {
             r = av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr);
-        if (r == VACO_AV1PF_OK)
-            r = vaco_av1pf_finish(ctx);
-        if (r == VACO_AV1PF_OK)
-            r = vaco_av1pf_profile_read(ctx, ms, launches, nullptr);
-        *res = launches[kernel] ? ms[kernel] / reps : 0.0;
+            if (r == VACO_AV1PF_OK)
+                r = vaco_av1pf_finish(ctx);
+            if (r == VACO_AV1PF_OK)
+                r = vaco_av1pf_profile_read(ctx, ms, launches, nullptr);
+            if (r == VACO_AV1PF_OK && launches[kernel] && (i == 0 || ms[kernel] < *res))
+                *res = ms[kernel];
+        }
         return r;
     };
     vaco_av1pf_tuning best = ctx->tune, t;
