@@ -47,6 +47,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# Launch variants every leg uses instead of autotuning (--tuning), so that a
+# profiling run times exactly the variants of the run it documents.
+FIXED_TUNING = None
+
+
+def tune(ctx, frames, in_ptrs, out_ptrs, reps):
+    """The leg's launch variants: FIXED_TUNING when given, else the
+    autotuner's choice on these frames (untimed)."""
+    if FIXED_TUNING is not None:
+        ctx.tuning = FIXED_TUNING
+        return ctx.tuning
+    return ctx.autotune(frames, in_ptrs, out_ptrs, reps=reps)
+
+
 def measure_mixed_batch(av1pf, hip, workloads, reps: int, n: int = 64):
     """64 seeded frames of mixed geometry (360p/720p/1080p/2160p), bit depth
     (8/10) and superres (half the frames): the per-frame path (one call per
@@ -60,7 +74,7 @@ def measure_mixed_batch(av1pf, hip, workloads, reps: int, n: int = 64):
     ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
     per_frame = [ctx.device_call(f, ptr(dev_in)[k], ptr(out_pf)[k]) for k, f in enumerate(frames)]
     batch = ctx.batch_call(frames, ptr(dev_in), ptr(out_b))
-    tuning = ctx.autotune(frames, ptr(dev_in), ptr(out_b), reps=4)  # untimed
+    tuning = tune(ctx, frames, ptr(dev_in), ptr(out_b), reps=4)
     for c in per_frame:
         c()
     batch()
@@ -244,8 +258,7 @@ def measure_per_frame(av1pf, hip, cfg, n_frames, steps, warmup):
     hip.synchronize()
     calls = [ctx.device_call(f, [t.ptr for t in dev_in[k]], [t.ptr for t in dev_out[k]])
              for k, f in enumerate(frames)]
-    tuning = ctx.autotune(frames[:1], [[t.ptr for t in dev_in[0]]], [[t.ptr for t in dev_out[0]]],
-                          reps=3)  # untimed
+    tuning = tune(ctx, frames[:1], [[t.ptr for t in dev_in[0]]], [[t.ptr for t in dev_out[0]]], reps=3)
     for i in range(warmup):
         calls[i % len(calls)]()
     ctx.finish()
@@ -351,7 +364,7 @@ def measure_4k10(av1pf, hip, workloads, reps: int, n: int = 16):
     ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
     calls = [ctx.device_call(f, ptr(dev_in)[k], ptr(out_pf)[k]) for k, f in enumerate(frames)]
     batch = ctx.batch_call(frames, ptr(dev_in), ptr(out_b))
-    tuning = ctx.autotune(frames, ptr(dev_in), ptr(out_b), reps=4)  # untimed
+    tuning = tune(ctx, frames, ptr(dev_in), ptr(out_b), reps=4)
     for c in calls:
         c()
     batch()
@@ -410,7 +423,13 @@ def main():
     ap.add_argument("--mode", choices=["full", "perframe", "batched", "served", "copy"], default="full",
                     help="full: the JSON line; perframe / batched / served: one leg only (for profiling); "
                          "copy: 256 MiB device copies (HBM counter calibration)")
+    ap.add_argument("--tuning", default=None,
+                    help="launch variants to use instead of autotuning, as a JSON object of "
+                         "vaco_av1pf_tuning fields (reproducible profiling runs)")
     args = ap.parse_args()
+    global FIXED_TUNING
+    if args.tuning:
+        FIXED_TUNING = json.loads(args.tuning)
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU only: frames are independent, so the stage shards by frame "
                                    "across GPUs with no collectives; run --gpus 1"}))
