@@ -35,7 +35,10 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 19
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
-    assert av1pf.lib().vaco_av1pf_abi_version() == 1
+    header = open(os.path.join(ROOT, "include", "vaco_av1pf.h")).read()
+    declared = int(re.search(r"#define VACO_AV1PF_ABI_VERSION (\d+)", header).group(1))
+    assert abi.ABI_VERSION == declared
+    assert av1pf.lib().vaco_av1pf_abi_version() == declared
 
 
 def test_struct_sizes_match_the_header_layout():

@@ -16,6 +16,9 @@ import ctypes as C
 
 import numpy as np
 
+# VACO_AV1PF_ABI_VERSION of include/vaco_av1pf.h that these layouts follow.
+ABI_VERSION = 1
+
 OK = 0
 INVALID_DATA = 1
 UNSUPPORTED = 2

@@ -173,6 +173,9 @@ def lib():
                      "vaco_av1pf_frame_output_geometry", "vaco_av1pf_frame_input_geometry",
                      "vaco_av1pf_gen_get_layout", "vaco_av1pf_gen_frame"):
             getattr(L, name).restype = C.c_int
+        if L.vaco_av1pf_abi_version() != abi.ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH} has ABI version {L.vaco_av1pf_abi_version()}, "
+                               f"this binding expects {abi.ABI_VERSION}: rebuild it")
         _lib = L
     return _lib
 
