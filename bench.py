@@ -420,8 +420,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
-    ap.add_argument("--mode", choices=["full", "perframe", "batched", "served", "copy"], default="full",
-                    help="full: the JSON line; perframe / batched / served: one leg only (for profiling); "
+    ap.add_argument("--mode", choices=["full", "perframe", "uhd", "batched", "served", "copy"], default="full",
+                    help="full: the JSON line; perframe / uhd / batched / served: one leg only (for profiling); "
                          "copy: 256 MiB device copies (HBM counter calibration)")
     ap.add_argument("--tuning", default=None,
                     help="launch variants to use instead of autotuning, as a JSON object of "
@@ -451,6 +451,9 @@ def main():
     if args.mode == "served":
         print(json.dumps({"mode": "served", **measure_served(av1pf, hip, workloads, args.serve_seconds)}),
               flush=True)
+        return
+    if args.mode == "uhd":
+        print(json.dumps({"mode": "uhd", **measure_4k10(av1pf, hip, workloads, reps=3)}), flush=True)
         return
     if args.mode == "batched":
         print(json.dumps({"mode": "batched",
