@@ -173,51 +173,11 @@ def test_negative_index_and_skipped_blocks_are_untouched():
                                       f.planes[1][by * 4:by * 4 + 4, bx * 4:bx * 4 + 4])
 
 
-def _brute_cdef(f):
-    """Frame CDEF assembled from the pinned block functions (cdef.rs) on
-    hand-gathered 12x12 neighbourhoods (decode.rs:772-856 spelled out)."""
-    d = f.desc
-    shift = d.bit_depth - 8
-    mi_cols = 2 * ((d.coded_width + 7) >> 3)
-    mi_rows = 2 * ((d.coded_height + 7) >> 3)
-    idx = f._keep[0]
-    skip = f._keep[1]
-    out = [p.copy() for p in f.planes]
-    for r in range(0, mi_rows, 2):
-        for c in range(0, mi_cols, 2):
-            i = int(idx[(r >> 4) * d.cdef_stride + (c >> 4)])
-            if i < 0 or skip[(r // 2) * (mi_cols // 2) + c // 2]:
-                continue
-            dirn, var = O.find_direction(f.planes[0][r * 4:r * 4 + 8, c * 4:c * 4 + 8], d.bit_depth)
-            for p in range(len(f.planes)):
-                sx = d.subsampling_x if p else 0
-                sy = d.subsampling_y if p else 0
-                x, y, w, h = (c * 4) >> sx, (r * 4) >> sy, 8 >> sx, 8 >> sy
-                src = f.planes[p]
-                H, W = src.shape
-                st = d.cdef.uv[i] if p else d.cdef.y[i]
-                pri, sec = st.primary << shift, st.secondary << shift
-                fp = abi.FilterParams()
-                fp.width, fp.height, fp.bit_depth = w, h, d.bit_depth
-                fp.direction = 0 if pri == 0 else (O.chroma_direction(dirn, sx, sy) if p else dirn)
-                fp.primary = pri if p else O.adjust_strength(pri, var)
-                fp.secondary = sec
-                fp.damping = d.cdef.damping + shift - (1 if p else 0)
-                fp.edges = (x > 0) | ((x + w < W) << 1) | ((y > 0) << 2) | ((y + h < H) << 3)
-                nb = np.zeros((12, 12), np.uint16)
-                ys = np.clip(np.arange(y - 2, y + h + 2), 0, H - 1)
-                xs = np.clip(np.arange(x - 2, x + w + 2), 0, W - 1)
-                nb[:h + 4, :w + 4] = src[np.ix_(ys, xs)]
-                res = O.filter_block(nb, fp)
-                out[p][y:y + h, x:x + w] = res[:h, :w]
-    return out
-
-
 @pytest.mark.parametrize("ss", [(1, 1), (0, 0), (1, 0), (0, 1)])
 def test_frame_cdef_equals_block_functions_on_gathered_neighbourhoods(ss):
     f = F.random_frame(20 + ss[0] * 2 + ss[1], 72, 56, lr=False, ssx=ss[0], ssy=ss[1])
     out = O.frame(f.desc, f.planes)
-    want = _brute_cdef(f)
+    want = F.brute_cdef(f)
     for p in range(3):
         assert np.array_equal(out[p], want[p]), p
 

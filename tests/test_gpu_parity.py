@@ -66,6 +66,36 @@ def test_gpu_restoration_matches_dav1d_corpus(ctx):
     assert n == 300
 
 
+@pytest.mark.parametrize("rows,lr_rows,lr_order", [(4, 32, 0), (16, 16, 1), (8, 64, 1), (4, 16, 0), (16, 32, 1)])
+def test_gpu_plane_corpora_under_each_launch_variant(ctx, rows, lr_rows, lr_order):
+    """The superres and restoration corpora through every launch variant the
+    tuner can pick (rows per superres workgroup, restoration tile height and
+    tile order), not only the defaults."""
+    say(f"corpora with superres_rows={rows} lr_tile_rows={lr_rows} lr_order={lr_order}")
+    saved = ctx.tuning
+    ctx.tuning = {"superres_rows": rows, "lr_tile_rows": lr_rows, "lr_order": lr_order}
+    try:
+        for case_id, (bd, vis, padded, out_w, h, inp, exp) in enumerate(G.superres_cases()):
+            assert np.array_equal(ctx.upscale_plane(inp, vis, out_w, h, bd), exp), f"superres case {case_id}"
+        for (bd, w, h, ssy, unit, mode, before, after, units, cfg, exp) in G.restoration_cases():
+            got = ctx.restore_plane(before, after, cfg, units)
+            assert np.array_equal(got, exp), f"depth={bd} {w}x{h} mode={mode}"
+        for bd, s, plane, exp in G.constant_cases():
+            cfg = abi.LrPlaneConfig(7, 9, 64, bd, 0)
+            sgr = abi.make_unit(abi.UNIT_SELF_GUIDED, set=s, xqd=(-32, 31))
+            assert np.array_equal(ctx.restore_plane(plane, plane, cfg, [sgr]), exp), (bd, s)
+    finally:
+        ctx.tuning = saved
+
+
+def test_gpu_superres_plane_wider_than_65536_is_refused(ctx):
+    say("superres plane width bound")
+    inp = np.zeros((1, 40000), np.uint16)
+    with pytest.raises(av1pf.Av1Error) as e:
+        ctx.upscale_plane(inp, 40000, 70000, 1, 8)
+    assert e.value.status == abi.LIMIT_EXCEEDED
+
+
 def test_gpu_restoration_constant_planes_match_dav1d(ctx):
     say("restoration constant planes")
     for bd, s, plane, exp in G.constant_cases():
@@ -235,10 +265,17 @@ def test_gpu_unaligned_strided_planes_match_oracle(ctx, case):
         assert np.array_equal(_unstride(d.download(), p.shape, st)[0], p), "input modified"
 
 
-def test_gpu_frame_cfg2_4k_superres_matches_oracle(ctx):
-    say("cfg2 frame 0 (4K 10-bit superres)")
-    f = av1pf.generate(workloads.cfg2(), 0)
-    assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), "cfg2/0")
+# Frames of the 4K 10-bit superres workload that together take every
+# denominator 9..16: coded widths 2048, 3413, 1920, 3072, 2363, 2793, 2560, 2194.
+CFG2_FRAMES = {0: 15, 1: 9, 2: 16, 3: 10, 5: 13, 11: 11, 12: 12, 15: 14}
+
+
+@pytest.mark.parametrize("frame_index", sorted(CFG2_FRAMES), ids=lambda k: f"f{k}-d{CFG2_FRAMES[k]}")
+def test_gpu_frame_cfg2_4k_superres_matches_oracle(ctx, frame_index):
+    f = av1pf.generate(workloads.cfg2(), frame_index)
+    say(f"cfg2 frame {frame_index} (4K 10-bit, coded width {f.desc.coded_width})")
+    assert f.desc.use_superres and f.desc.coded_width == (3840 * 8 + CFG2_FRAMES[frame_index] // 2) // CFG2_FRAMES[frame_index]
+    assert_frame_equal(ctx.frame(f), O.frame(f.desc, f.planes), f"cfg2/{frame_index}")
 
 
 def test_gpu_frame_device_entry_matches_host_entry(ctx):

@@ -224,50 +224,8 @@ __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int
     return B;
 }
 
-// Both taps of one direction/distance, cdef.rs:171-198. constrain()
-// (cdef.rs:126-133) is sign(d) * min(|d|, T) with T = max(0, S - (|d| >>
-// shift)); since T >= 0 that is d clamped to [-T, T].
-template <bool BORDER>
-__device__ __forceinline__ void tap2(const uint16_t *t, int ci, int off, int x, int S, int sh,
-                                     int w, int &sum, int &mn, int &mx)
-{
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-        const int p = t[s ? ci + off : ci - off];
-        const int d = p - x;
-        const int T = max(S - (abs(d) >> sh), 0);
-        const int c = min(max(d, -T), T);
-        if (BORDER) {  // unavailable taps are skipped (cdef.rs:189-191)
-            const bool valid = p != kUnavailable;
-            sum += valid ? w * c : 0;
-            mx = max(mx, valid ? p : x);
-        } else {
-            sum += w * c;
-            mx = max(mx, p);
-        }
-        mn = min(mn, p);
-    }
-}
-
-template <bool BORDER>
-__device__ __forceinline__ int filter_px(const uint16_t *t, int ci, const Blk &B)
-{
-    const int x = t[ci];
-    int sum = 0, mn = x, mx = x;
-    const int ps = B.shifts & 15, ss = B.shifts >> 4;
-    const int t0 = (B.flags & 2) ? 3 : 4;
-    tap2<BORDER>(t, ci, B.off[0], x, B.pri, ps, t0, sum, mn, mx);
-    tap2<BORDER>(t, ci, B.off[1], x, B.pri, ps, 6 - t0, sum, mn, mx);
-    tap2<BORDER>(t, ci, B.off[2], x, B.sec, ss, 2, sum, mn, mx);
-    tap2<BORDER>(t, ci, B.off[3], x, B.sec, ss, 1, sum, mn, mx);
-    tap2<BORDER>(t, ci, B.off[4], x, B.sec, ss, 2, sum, mn, mx);
-    tap2<BORDER>(t, ci, B.off[5], x, B.sec, ss, 1, sum, mn, mx);
-    const int y = x + ((8 + sum - (sum < 0)) >> 4);  // cdef.rs:200-201
-    return min(max(y, mn), mx);
-}
-
-// Packed form of filter_px for two horizontally adjacent samples of one
-// block. Every quantity fits int16: samples <= 4095, differences within
+// filter_block (cdef.rs:142-205), computed for two horizontally adjacent
+// samples of one block at once. Every quantity fits int16: samples <= 4095, differences within
 // +-4095, and the tap sum is at most 2*(4+2)*240 + 4*(2+1)*64 = 3648, so
 // each operation works on both samples at once (v_pk_*_i16).
 
@@ -798,12 +756,20 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(8, 8))) void 
 }
 
 // Kernel-level cases (cdef::find_direction + cdef::filter_block on one
-// 12x12 neighbourhood each, av1_cdef.rs:85-103): one wave per case.
+// 12x12 neighbourhood each, av1_cdef.rs:85-103): one wave per case, through
+// the frame kernel's own device code: dir_cost for the search, and the
+// packed filter_pairs with its tile layout (shifted copy, pre-resolved tap
+// offsets), its unavailable-tap select and its clamp skip for filtering.
 struct CdefCase {
     uint32_t width, height;
     uint8_t edges, bit_depth, direction, damping;
     uint16_t primary, secondary;
 };
+
+// The case tile: 12 rows of kRow samples (column c of the neighbourhood at
+// index c + 6, so the block starts at the even index 8), then its copy
+// shifted by one sample.
+constexpr int kCaseTile = 12 * kRow;
 
 __global__ __launch_bounds__(64) void k_cdef_cases(const uint16_t *__restrict__ inputs,
                                                    const CdefCase *__restrict__ params,
@@ -811,7 +777,7 @@ __global__ __launch_bounds__(64) void k_cdef_cases(const uint16_t *__restrict__ 
                                                    uint32_t *__restrict__ vars,
                                                    uint16_t *__restrict__ outs, uint32_t n)
 {
-    __shared__ __align__(16) uint16_t t[12 * kRow];
+    __shared__ __align__(16) uint16_t t[2 * kCaseTile + 8];
     __shared__ __align__(16) uint16_t raw[8 * kRow];
     __shared__ uint32_t cost[8];
     const uint32_t cs = blockIdx.x;
@@ -828,7 +794,9 @@ __global__ __launch_bounds__(64) void k_cdef_cases(const uint16_t *__restrict__ 
         const int ny = r - 2, nx = c - 2;
         const bool ok = !((nx < 0 && !(P.edges & 1)) || (nx >= (int)P.width && !(P.edges & 2)) ||
                           (ny < 0 && !(P.edges & 4)) || (ny >= (int)P.height && !(P.edges & 8)));
-        t[r * kRow + c + 6] = ok ? in[e] : kUnavailable;
+        const uint16_t v = ok ? in[e] : kUnavailable;
+        t[r * kRow + c + 6] = v;
+        t[kCaseTile + r * kRow + c + 7] = v;
     }
     // The direction search reads the raw 8x8 at (2,2) (av1_cdef.rs:90-95).
     raw[(lane >> 3) * kRow + (lane & 7)] = in[((lane >> 3) + 2) * 12 + (lane & 7) + 2];
@@ -847,12 +815,24 @@ __global__ __launch_bounds__(64) void k_cdef_cases(const uint16_t *__restrict__ 
         dirs[cs] = (uint8_t)best;
         vars[cs] = (bc - cost[(best + 4) & 7]) >> 10;
     }
-    const Blk B = make_blk<kRow, kTile>(P.direction, P.primary, P.secondary, P.damping, shift);
-    const int r = lane >> 3, c = lane & 7;
-    uint16_t res = 0;
-    if (r < (int)P.height && c < (int)P.width)
-        res = (uint16_t)filter_px<true>(t, (r + 2) * kRow + c + 8, B);
-    outs[(size_t)cs * 64 + lane] = res;
+    // Lanes 0-31 each filter one pair of samples (row lane / 4, columns
+    // 2 * (lane % 4) and the next). With every edge available no tap is
+    // unavailable, so those cases run the interior superblocks' variant.
+    if (lane < 32) {
+        const Blk B = make_blk<kRow, kCaseTile>(P.direction, P.primary, P.secondary, P.damping, shift);
+        const int r = lane >> 2, c = (lane & 3) * 2;
+        const int ci = (r + 2) * kRow + c + 8;
+        const bool pri_on = P.primary != 0, sec_on = P.secondary != 0;
+        s16x2 res;
+        if (P.edges == 15)
+            filter_pairs<false, 1, 0>(t, ci, B, &res, pri_on, sec_on);
+        else
+            filter_pairs<true, 1, 0>(t, ci, B, &res, pri_on, sec_on);
+        const bool in_block = r < (int)P.height && c < (int)P.width;
+        uint16_t *o = outs + (size_t)cs * 64 + r * 8 + c;
+        o[0] = in_block ? (uint16_t)res.x : 0;
+        o[1] = in_block ? (uint16_t)res.y : 0;  // widths are 4 or 8: both samples or neither
+    }
 }
 
 template <bool XCD, int T>

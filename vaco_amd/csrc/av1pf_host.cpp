@@ -50,6 +50,7 @@ namespace {
 
 constexpr int kSlots = 8;
 constexpr uint32_t kMaxBatch = 256;  // frames per batched call
+constexpr uint32_t kMaxSuperresWidth = 65536;  // k_superres's 32-bit source positions
 
 // One in-flight frame: its staging area (pinned host + device mirror), its
 // status word (coherent pinned host memory, raised by the kernels), and
@@ -536,6 +537,20 @@ void lr_col_segments(uint32_t width, uint32_t unit, std::vector<LrColSeg> &out)
     }
 }
 
+// Row segments cut into tiles of at most `rows` rows; each piece keeps its
+// segment's stripe and unit row.
+void lr_cut_rows(const std::vector<LrRowSeg> &segs, int rows, std::vector<LrRowSeg> &out)
+{
+    out.clear();
+    for (const LrRowSeg &r : segs)
+        for (int y = 0; y < r.h; y += rows) {
+            LrRowSeg c = r;
+            c.y0 = r.y0 + y;
+            c.h = std::min(r.h - y, rows);
+            out.push_back(c);
+        }
+}
+
 // Pre-CDEF rows that Source::get can read at stripe borders
 // (restoration.rs:236-241), for the upscaled pre-CDEF image.
 void lr_border_rows(const std::vector<LrRowSeg> &rows, uint32_t height, std::vector<uint32_t> &out)
@@ -791,13 +806,7 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             // Tiles: row segments cut to the tuned height (the stripe and unit
             // of each piece are its segment's).
             std::vector<LrRowSeg> rows_cut;
-            for (const LrRowSeg &r : lr_rows[p])
-                for (int y = 0; y < r.h; y += plan.tune.lr_tile_rows) {
-                    LrRowSeg c = r;
-                    c.y0 = r.y0 + y;
-                    c.h = std::min(r.h - y, (int)plan.tune.lr_tile_rows);
-                    rows_cut.push_back(c);
-                }
+            lr_cut_rows(lr_rows[p], plan.tune.lr_tile_rows, rows_cut);
             const std::vector<LrRowSeg> *rows = &rows_cut;
             const std::vector<LrColSeg> *cols = &lr_cols[p];
             if (none) {
@@ -1701,7 +1710,11 @@ extern "C" vaco_av1pf_status vaco_av1pf_upscale_plane(vaco_av1pf_ctx *ctx, const
     if (out->width != cfg->output_width || out->height != cfg->height || out->stride < out->width ||
         in->stride < in->width)
         return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "plane geometry");
+    // k_superres keeps source positions in 32 bits: x * step < 2^16 * 2^14.
+    if (cfg->output_width > kMaxSuperresWidth)
+        return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "superres output wider than 65536 samples");
     HIPCHECK(hipSetDevice(ctx->device));
+    const int rows_per_block = ctx->tune.superres_rows;
     const size_t nin = (size_t)in->width * cfg->height, nout = (size_t)cfg->output_width * cfg->height;
     uint8_t *buf = nullptr;
     HIPCHECK(hipMalloc(&buf, (nin + nout) * 2 + 512));
@@ -1735,7 +1748,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_upscale_plane(vaco_av1pf_ctx *ctx, const
     if (e == hipSuccess)
         e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-        e = launch_superres(dj, 1, sr_job_blocks(j.n_rows, j.segs, 8), 8, ctx->stream);
+        e = launch_superres(dj, 1, sr_job_blocks(j.n_rows, j.segs, rows_per_block), rows_per_block,
+                            ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpy2DAsync(out->data, out->stride * 2, dout, cfg->output_width * 2,
                              cfg->output_width * 2, cfg->height, hipMemcpyDeviceToHost, ctx->stream);
@@ -1770,10 +1784,14 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     if (out->width != cfg->width || out->height != cfg->height || out->stride < out->width)
         return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "plane geometry");
     HIPCHECK(hipSetDevice(ctx->device));
-    std::vector<LrRowSeg> rows;
+    std::vector<LrRowSeg> segs;
     std::vector<LrColSeg> cols;
-    lr_row_segments(cfg->height, cfg->unit_size, cfg->subsampling_y != 0, rows);
+    lr_row_segments(cfg->height, cfg->unit_size, cfg->subsampling_y != 0, segs);
     lr_col_segments(cfg->width, cfg->unit_size, cols);
+    // tiles cut to the tuned height, as the frame path cuts them
+    const vaco_av1pf_tuning tune = ctx->tune;
+    std::vector<LrRowSeg> rows;
+    lr_cut_rows(segs, tune.lr_tile_rows, rows);
     const size_t nb = (size_t)before->height * before->width, na = (size_t)after->height * after->width;
     const size_t no = (size_t)cfg->width * cfg->height;
     uint32_t ucols, urows;
@@ -1784,7 +1802,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
         for (uint32_t ci = 0; ci < cols.size(); ci++) {
             const uint8_t k = units[rows[ri].unit_row * ucols + cols[ci].unit_col].kind;
             const int kind = k == VACO_AV1PF_UNIT_WIENER ? 1 : k == VACO_AV1PF_UNIT_SELF_GUIDED ? 2 : 0;
-            tiles[kind].push_back((uint64_t)(ri * (uint32_t)cols.size() + ci) << 32 | (uint64_t)kind);
+            tiles[tune.lr_order ? 0 : kind].push_back((uint64_t)(ri * (uint32_t)cols.size() + ci) << 32 |
+                                                      (uint64_t)kind);
         }
     const size_t ntiles = rows.size() * cols.size();
     const size_t bytes_tail = rows.size() * sizeof(LrRowSeg) + cols.size() * sizeof(LrColSeg) +
@@ -1853,7 +1872,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     if (e == hipSuccess)
         e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-        e = launch_lr(dl, dlist, nlist[0] + nlist[1] + nlist[2], false, 64, ctx->stream);
+        e = launch_lr(dl, dlist, nlist[0] + nlist[1] + nlist[2], tune.lr_order != 0, tune.lr_tile_rows,
+                      ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpy2DAsync(out->data, out->stride * 2, dout, cfg->width * 2, cfg->width * 2,
                              cfg->height, hipMemcpyDeviceToHost, ctx->stream);
