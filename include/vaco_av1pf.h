@@ -241,6 +241,28 @@ vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32_t n,
                                           const vaco_av1pf_picture *ins,
                                           vaco_av1pf_picture *outs, void *stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): vaco_av1pf_frame_device without a context. It allocates
+ * no device or pinned memory: every device buffer it needs (status word,
+ * staged side information, intermediate images) lives in one file-scope
+ * device array of VACO_AV1PF_STATIC_SCRATCH_BYTES that each GPU holds from
+ * the library's first use on it until the process ends. It runs on the
+ * calling thread's current device and is synchronous: it returns once the
+ * output planes are written, with the status and, in *msg (may be NULL),
+ * the message vaco_av1pf_frame would report. Frames whose staging exceeds
+ * 2 MiB or whose intermediates exceed the rest of the array (a 4:2:0 frame
+ * with restoration up to 3840x2176 without superres, or 1920x1088 with it)
+ * return VACO_AV1PF_LIMIT_EXCEEDED before any device work. Every call on a
+ * GPU shares that GPU's array and leaves the last call's contents in it; the
+ * library serialises this process's calls on one GPU, and calls from other
+ * processes use their own copy. */
+#define VACO_AV1PF_STATIC_SCRATCH_BYTES (32u << 20)
+vaco_av1pf_status vaco_av1pf_frame_device_static(const vaco_av1pf_frame_desc *desc,
+                                                 const vaco_av1pf_picture *in,
+                                                 vaco_av1pf_picture *out, void *stream,
+                                                 const char **msg);
+
 /* ------------------------------------------------------ frame output --- */
 
 /* Output planes in the layout of the decoder's Frame (pic_to_frame,

@@ -90,6 +90,8 @@ def lib():
         L.vaco_av1pf_frame.argtypes = [ctxp, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture), P(C.c_uint64)]
         L.vaco_av1pf_frame_device.argtypes = [ctxp, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture), C.c_void_p]
         L.vaco_av1pf_finish.argtypes = [ctxp]
+        L.vaco_av1pf_frame_device_static.argtypes = [P(abi.FrameDesc), P(abi.Picture), P(abi.Picture), C.c_void_p,
+                                                     P(C.c_char_p)]
         L.vaco_av1pf_cdef_blocks.argtypes = [ctxp, C.c_uint32, P(C.c_uint16), P(abi.FilterParams),
                                              P(C.c_uint8), P(C.c_uint32), P(C.c_uint16)]
         L.vaco_av1pf_upscale_plane.argtypes = [ctxp, P(abi.Plane), P(abi.SrPlaneConfig), P(abi.Plane)]
@@ -171,6 +173,7 @@ def lib():
                      "vaco_av1pf_batch_device", "vaco_av1pf_profile", "vaco_av1pf_profile_read", "vaco_av1pf_create", "vaco_av1pf_frame", "vaco_av1pf_frame_device", "vaco_av1pf_finish",
                      "vaco_av1pf_cdef_blocks", "vaco_av1pf_upscale_plane", "vaco_av1pf_restore_plane",
                      "vaco_av1pf_frame_output_geometry", "vaco_av1pf_frame_input_geometry",
+                     "vaco_av1pf_frame_device_static",
                      "vaco_av1pf_gen_get_layout", "vaco_av1pf_gen_frame"):
             getattr(L, name).restype = C.c_int
         if L.vaco_av1pf_abi_version() != abi.ABI_VERSION:
@@ -545,6 +548,23 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def frame_device_static(frame: Frame, in_ptrs, out_ptrs, stream: int | None = None):
+    """vaco_av1pf_frame_device_static: the stage on device planes (integer
+    addresses, dense rows) without a context or any allocation; synchronous.
+    Raises Av1Error with the reference's message on failure."""
+    geo_out = frame.output_geometry()
+    pin, pout = abi.Picture(), abi.Picture()
+    for i in range(frame.nplanes):
+        h, w = frame.planes[i].shape
+        pin.plane[i] = abi.device_plane(in_ptrs[i], w, h)
+        pout.plane[i] = abi.device_plane(out_ptrs[i], geo_out[i][1], geo_out[i][0])
+    msg = C.c_char_p()
+    rc = lib().vaco_av1pf_frame_device_static(C.byref(frame.desc), C.byref(pin), C.byref(pout),
+                                              C.c_void_p(stream or 0), C.byref(msg))
+    if rc != abi.OK:
+        raise Av1Error(rc, (msg.value or b"").decode())
 
 
 def frame_bytes(frame: Frame) -> int:

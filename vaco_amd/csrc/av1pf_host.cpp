@@ -51,6 +51,11 @@ namespace {
 constexpr int kSlots = 8;
 constexpr uint32_t kMaxBatch = 256;  // frames per batched call
 constexpr uint32_t kMaxSuperresWidth = 65536;  // k_superres's 32-bit source positions
+// Launch variants of a new context (vaco_av1pf_autotune may change them).
+constexpr vaco_av1pf_tuning kDefaultTuning = {8, 1, 0, 64, 16};
+// Staging bytes of the static-scratch entry (the rest of its scratch holds
+// intermediate images).
+constexpr size_t kStaticStageBytes = 2u << 20;
 
 // One in-flight frame: its staging area (pinned host + device mirror), its
 // status word (coherent pinned host memory, raised by the kernels), and
@@ -70,6 +75,9 @@ struct Arena {
     hipStream_t stream = nullptr;  // the stream done_ev was recorded on
     hipEvent_t up_done = nullptr;  // its upload (on the context's upload stream) has finished
     bool pending = false;
+    // h is ordinary (pageable) host memory: the upload is a runtime copy, not
+    // k_upload's direct read of coherent pinned memory
+    bool pageable = false;
     uint32_t *status = nullptr;  // kMaxBatch words, one per frame
     // frames whose status must be checked before reuse
     std::vector<vaco_av1pf_frame_desc> descs;
@@ -122,7 +130,7 @@ struct vaco_av1pf_ctx {
         uint32_t a, b;
         uint64_t bytes;
     };
-    vaco_av1pf_tuning tune = {8, 1, 0, 64, 16};  // launch variants (vaco_av1pf_autotune)
+    vaco_av1pf_tuning tune = kDefaultTuning;  // launch variants (vaco_av1pf_autotune)
     bool use_graphs = false;                // replay kernel chains as hipGraphs
     bool prof = false;
     std::vector<hipEvent_t> prof_pool;
@@ -240,6 +248,10 @@ static vaco_av1pf_status upload_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t 
 {
     if (a->used == 0)
         return VACO_AV1PF_OK;
+    if (a->pageable) {
+        HIPCHECK(hipMemcpyAsync(a->d, a->h, a->used, hipMemcpyHostToDevice, s));
+        return VACO_AV1PF_OK;
+    }
     if (a->used <= kInlineUpload) {
         HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_UPLOAD, (uint64_t)a->used * 2, s,
                               [&](hipEvent_t e0, hipEvent_t e1) {
@@ -629,7 +641,7 @@ struct Plan {
         uint32_t w, h;
     };  // w x h samples; dst.u8: to bytes
     std::vector<Copy> copies;  // frames in which no stage runs
-    vaco_av1pf_tuning tune = {8, 1, 0, 64, 16};  // launch variants this plan is built for
+    vaco_av1pf_tuning tune = kDefaultTuning;  // launch variants this plan is built for
     // algorithmic bytes per kernel id (samples read once + written once, 2 B)
     uint64_t bytes[VACO_AV1PF_KERNEL_COUNT] = {};
     uint64_t cdef_bytes[4] = {};  // CDEF by chroma layout
@@ -947,7 +959,7 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
                                   }));
         return VACO_AV1PF_OK;
     };
-    if (!ctx->use_graphs || ctx->prof)
+    if (!ctx || !ctx->use_graphs || ctx->prof)
         return launch_all();
 
     // Graph replay: the chain's launch parameters (descriptor addresses in
@@ -1041,7 +1053,7 @@ static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, uint64_t bytes, 
                                F launch)
 {
     (void)s;
-    if (!ctx->prof)
+    if (!ctx || !ctx->prof)
         return launch(nullptr, nullptr);
     if (ctx->prof_next + 2 > ctx->prof_pool.size()) {
         hipError_t e = profile_collect(ctx);
@@ -1094,24 +1106,40 @@ extern "C" vaco_av1pf_status vaco_av1pf_profile_read(vaco_av1pf_ctx *ctx, double
 }
 
 // Validate the caller's planes against the geometry the reference implies.
+static vaco_av1pf_status check_geometry(const FrameGeo &g, uint32_t cdef_stride, const vaco_av1pf_picture *in,
+                                        const vaco_av1pf_picture *out, const char **msg)
+{
+    for (int p = 0; p < g.nplanes; p++) {
+        const vaco_av1pf_plane &i = in->plane[p], &o = out->plane[p];
+        *msg = "missing plane";
+        if (!i.data || !o.data)
+            return VACO_AV1PF_INVALID_ARGUMENT;
+        *msg = "input plane geometry differs from the Mi-padded frame";
+        if (i.width != g.in_w[p] || i.height != g.in_h[p] || i.stride < i.width)
+            return VACO_AV1PF_INVALID_ARGUMENT;
+        *msg = "output plane geometry differs from the stage output";
+        if (o.width != g.out_w[p] || o.height != g.out_h[p] || o.stride < o.width)
+            return VACO_AV1PF_INVALID_ARGUMENT;
+    }
+    *msg = "CDEF side information missing";
+    if (g.cdef && (!g.has_idx || !g.has_skip || cdef_stride < (g.mi_cols + 15) / 16))
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    *msg = "";
+    return VACO_AV1PF_OK;
+}
+
+// check_geometry plus the context's capacity.
 static vaco_av1pf_status check_planes(vaco_av1pf_ctx *ctx, const FrameGeo &g, uint32_t cdef_stride,
                                       const vaco_av1pf_picture *in, const vaco_av1pf_picture *out)
 {
-    auto d_stride_short = [&](const FrameGeo &gg) { return cdef_stride < (gg.mi_cols + 15) / 16; };
-    for (int p = 0; p < g.nplanes; p++) {
-        const vaco_av1pf_plane &i = in->plane[p], &o = out->plane[p];
-        if (!i.data || !o.data)
-            return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "missing plane");
-        if (i.width != g.in_w[p] || i.height != g.in_h[p] || i.stride < i.width)
-            return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "input plane geometry differs from the Mi-padded frame");
-        if (o.width != g.out_w[p] || o.height != g.out_h[p] || o.stride < o.width)
-            return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "output plane geometry differs from the stage output");
+    const char *msg = "";
+    const vaco_av1pf_status st = check_geometry(g, cdef_stride, in, out, &msg);
+    if (st != VACO_AV1PF_OK)
+        return fail(ctx, st, msg);
+    for (int p = 0; p < g.nplanes; p++)
         if ((size_t)g.in_w[p] > ctx->max_w || (size_t)g.in_h[p] > ctx->max_h ||
             (size_t)g.out_w[p] > ctx->max_w || (size_t)g.out_h[p] > ctx->max_h)
             return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "frame exceeds the context's capacity");
-    }
-    if (g.cdef && (!g.has_idx || !g.has_skip || d_stride_short(g)))
-        return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "CDEF side information missing");
     return VACO_AV1PF_OK;
 }
 
@@ -1647,6 +1675,80 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
         return st;
     *out = best;
     return ps;
+}
+
+// ------------------------------------------------ static-scratch entry --
+
+// The per-frame stage with every device buffer it needs inside `scratch`
+// (`bytes` of 256-byte aligned device memory): the status word and the
+// staged side information and work descriptors in the first
+// kStaticStageBytes, the intermediate images after them. Runs on stream s
+// and waits for it; allocates no device or pinned memory (the staged bytes
+// are built in a per-thread host buffer and copied up by the runtime).
+vaco_av1pf_status av1pf_frame_static(const vaco_av1pf_frame_desc *desc, const vaco_av1pf_picture *in,
+                                     vaco_av1pf_picture *out, hipStream_t s, uint8_t *scratch, size_t bytes,
+                                     const char **msg)
+{
+    *msg = "";
+    FrameGeo g;
+    frame_geometry(*desc, &g);
+    vaco_av1pf_status st = check_geometry(g, desc->cdef_stride, in, out, msg);
+    if (st != VACO_AV1PF_OK)
+        return st;
+    const size_t inter = scratch_need(g) * sizeof(uint16_t);
+    if (arena_need(*desc, g) > kStaticStageBytes || bytes < kStaticStageBytes ||
+        inter > bytes - kStaticStageBytes) {
+        *msg = "frame needs more device memory than the static scratch holds";
+        return VACO_AV1PF_LIMIT_EXCEEDED;
+    }
+    if (!params_clean(*desc, g)) {
+        // Parameter errors need the samples to order them exactly.
+        if (hipStreamSynchronize(s) != hipSuccess) {
+            *msg = "stream synchronisation failed";
+            return VACO_AV1PF_DEVICE_ERROR;
+        }
+        const vaco_av1pf_status r = av1pf_replay_device_frame(nullptr, *desc, *in, msg);
+        if (r != VACO_AV1PF_OK)
+            return r;
+    }
+    thread_local std::vector<uint8_t> host;
+    host.resize(kStaticStageBytes);
+    Arena a;
+    a.h = host.data();
+    a.d = scratch;
+    a.cap = kStaticStageBytes;
+    a.pageable = true;
+    uint32_t *dstatus = nullptr;
+    uint32_t *hstatus = a.take<uint32_t>(1, &dstatus);
+    *hstatus = 0;  // cleared by the upload itself
+    Scratch ws;
+    ws.base = reinterpret_cast<uint16_t *>(scratch + kStaticStageBytes);
+    ws.cap = (bytes - kStaticStageBytes) / sizeof(uint16_t);
+    Plan plan;
+    DevPlane din[3], dout[3];
+    for (int p = 0; p < g.nplanes; p++) {
+        din[p] = DevPlane{in->plane[p].data, in->plane[p].stride};
+        dout[p] = DevPlane{out->plane[p].data, out->plane[p].stride};
+    }
+    st = plan_frame(nullptr, *desc, g, din, dout, &a, dstatus, ws, plan);
+    if (st == VACO_AV1PF_OK)
+        st = launch_plan(nullptr, plan, &a, s);
+    if (st != VACO_AV1PF_OK) {
+        *msg = st == VACO_AV1PF_DEVICE_ERROR ? "kernel launch failed" : "static scratch exhausted";
+        (void)hipStreamSynchronize(s);
+        return st;
+    }
+    uint32_t flag = 0;
+    hipError_t e = hipMemcpyAsync(&flag, dstatus, sizeof flag, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        *msg = hipGetErrorString(e);
+        return VACO_AV1PF_DEVICE_ERROR;
+    }
+    if (flag & kFlagSampleRange)
+        return av1pf_replay_device_frame(nullptr, *desc, *in, msg);
+    return VACO_AV1PF_OK;
 }
 
 // ------------------------------------------------- kernel-level entries --

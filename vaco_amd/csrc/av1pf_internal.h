@@ -23,3 +23,10 @@ vaco_av1pf_status av1pf_replay_device_frame(vaco_av1pf_ctx *ctx, const vaco_av1p
 size_t av1pf_frame_device_bytes(const vaco_av1pf_frame_desc &d);
 
 int av1pf_ctx_device(const vaco_av1pf_ctx *ctx);
+
+// The per-frame stage inside caller-provided device memory `scratch`
+// (`bytes` long, 256-byte aligned); synchronous on stream s. *msg receives
+// the message of a failure.
+vaco_av1pf_status av1pf_frame_static(const vaco_av1pf_frame_desc *desc, const vaco_av1pf_picture *in,
+                                     vaco_av1pf_picture *out, hipStream_t s, uint8_t *scratch, size_t bytes,
+                                     const char **msg);
