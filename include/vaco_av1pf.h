@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define VACO_AV1PF_ABI_VERSION 1
+#define VACO_AV1PF_ABI_VERSION 2
 
 /* vaco_core::Error classes (crates/core/vaco-core/src/error.rs:14-59). */
 typedef enum vaco_av1pf_status {
@@ -221,7 +221,10 @@ vaco_av1pf_status vaco_av1pf_frame(vaco_av1pf_ctx *ctx,
  * the next vaco_av1pf_finish() on this context. The side-information arrays
  * `desc` points to and the input planes must stay unchanged until
  * vaco_av1pf_finish() returns: an error is replayed from them. Up to 8
- * frames are in flight per context; the 9th call waits for the 1st. */
+ * frames are in flight per context; the 9th call waits for the 1st. Calls
+ * share the context's intermediate images, so a call on another stream than
+ * the previous call's waits, on the device, for that call's work: calls on
+ * different streams are correct but do not overlap. */
 vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
                                           const vaco_av1pf_frame_desc *desc,
                                           const vaco_av1pf_picture *in,
@@ -233,9 +236,13 @@ vaco_av1pf_status vaco_av1pf_frame_device(vaco_av1pf_ctx *ctx,
  * device planes. All frames' work descriptors and side information go up
  * in one copy, and each stage kernel is launched once over the tiles of
  * every frame. Output is identical to n vaco_av1pf_frame_device calls.
- * Same lifetime rule and deferred errors as vaco_av1pf_frame_device (the
- * first failing frame's error is reported). No reference peer: the
- * reference filters one frame per decode_frame call. */
+ * Same lifetime rule, deferred errors (the first failing frame's error is
+ * reported) and cross-stream ordering as vaco_av1pf_frame_device. The
+ * intermediates of a batch live in a context workspace that grows (one
+ * device allocation, after the context's work has finished) when a batch
+ * needs more than any before it; the server below sizes its own at creation
+ * and never allocates afterwards. No reference peer: the reference filters
+ * one frame per decode_frame call. */
 vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32_t n,
                                           const vaco_av1pf_frame_desc *descs,
                                           const vaco_av1pf_picture *ins,
@@ -443,8 +450,20 @@ vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n,
  *    waiting request has waited window_us (continuous batching: requests
  *    arriving while batches run join the next one);
  *  - admission is FIFO; a request is admitted when the device bytes of the
- *    frames in flight plus its own stay within hbm_budget_bytes.
- * Results are identical to the same frame run alone. */
+ *    frames in flight plus its own stay within hbm_budget_bytes;
+ *  - each batch in flight runs on a stream of its own (max_inflight of
+ *    them) with an intermediate-image workspace of its own, so batches
+ *    overlap on the device; a batch completes as soon as its own work has.
+ *    The HIP runtime maps streams onto a pool of GPU_MAX_HW_QUEUES hardware
+ *    queues per process (4 by default) and streams sharing a queue run in
+ *    order, so overlap needs a queue per stream: set GPU_MAX_HW_QUEUES to
+ *    at least the process's stream count (each context has 2, the server
+ *    max_inflight more) before the runtime starts.
+ * Every device and pinned buffer the server uses is allocated by
+ * vaco_av1pf_server_create (one workspace per stream, up to 2 GiB each but
+ * at least the largest frame's; staging slots of up to 16 MiB); a batch is
+ * closed early rather than outgrow them, and nothing is allocated after
+ * creation. Results are identical to the same frame run alone. */
 typedef struct vaco_av1pf_server vaco_av1pf_server;
 
 typedef struct vaco_av1pf_server_config {
@@ -461,6 +480,12 @@ typedef struct vaco_av1pf_completion {
     uint64_t t_submit_ns; /* vaco_av1pf_now_ns() clock */
     uint64_t t_launch_ns;
     uint64_t t_done_ns;   /* when the server saw the batch complete */
+    /* Device clock, ns since the server was created: when the batch's stream
+     * reached its first command, and when its last command finished. Two
+     * batches whose [start, end] intervals intersect ran on the device at
+     * the same time. */
+    uint64_t t_gpu_start_ns;
+    uint64_t t_gpu_end_ns;
 } vaco_av1pf_completion;
 
 vaco_av1pf_status vaco_av1pf_server_create(vaco_av1pf_ctx *ctx,

@@ -49,7 +49,7 @@ def test_struct_sizes_match_the_header_layout():
 
 def test_server_config_and_frame_bytes_host_checks():
     L = av1pf.lib()
-    assert C.sizeof(abi.ServerConfig) == 24 and C.sizeof(abi.Completion) == 40
+    assert C.sizeof(abi.ServerConfig) == 24 and C.sizeof(abi.Completion) == 56
     h = C.c_void_p()
     bad = [abi.ServerConfig(1 << 30, 2000, 0, 2), abi.ServerConfig(1 << 30, 2000, 257, 2),
            abi.ServerConfig(1 << 30, 2000, 64, 0), abi.ServerConfig(0, 2000, 64, 2)]

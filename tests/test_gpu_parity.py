@@ -292,6 +292,36 @@ def test_gpu_frame_device_entry_matches_host_entry(ctx):
         assert_frame_equal(got, want, f"device/{k}")
 
 
+def test_gpu_calls_alternating_between_two_caller_streams_match_oracle(ctx):
+    """Per-frame and batched calls share the context's intermediate images;
+    calls alternating between two caller streams must still each equal the
+    oracle (a call on a new stream waits for the previous call's work)."""
+    from vaco_amd import hip
+    say("two caller streams: per-frame and batched calls alternating")
+    frames = [av1pf.generate(workloads.cfg1(), k) for k in range(2)]
+    frames += [F.random_frame(**c) for c in (dict(seed=12, coded_w=93, coded_h=75, upscaled_w=160),
+                                             dict(seed=13, coded_w=171, coded_h=130, upscaled_w=300, bd=10,
+                                                  unit=32))]
+    want = [O.frame(f.desc, f.planes) for f in frames]
+    streams = [hip.Stream(), hip.Stream()]
+    dev_in = [[hip.DeviceArray.from_host(p) for p in f.planes] for f in frames]
+    outs = [[[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames] for _ in range(3)]
+    ptr = lambda arrs: [t.ptr for t in arrs]
+    for rep in range(2):
+        for k, f in enumerate(frames):
+            ctx.frame_device(f, ptr(dev_in[k]), ptr(outs[rep][k]), stream=streams[(k + rep) % 2].handle)
+    ctx.batch_device(frames[:2], [ptr(d) for d in dev_in[:2]], [ptr(o) for o in outs[2][:2]],
+                     stream=streams[0].handle)
+    ctx.batch_device(frames[2:], [ptr(d) for d in dev_in[2:]], [ptr(o) for o in outs[2][2:]],
+                     stream=streams[1].handle)
+    ctx.finish()
+    for rep in range(3):
+        for k in range(len(frames)):
+            assert_frame_equal([t.download() for t in outs[rep][k]], want[k], f"stream call {rep}/{k}")
+    for st in streams:
+        st.destroy()
+
+
 def test_gpu_device_entry_defers_sample_range_errors_to_finish(ctx):
     from vaco_amd import hip
     say("device entry: deferred error")

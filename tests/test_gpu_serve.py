@@ -86,6 +86,48 @@ def test_gpu_served_frames_alone_and_in_a_mix_are_identical(ctx):
             assert np.array_equal(mix[p], alone[p]), (k, p, "mix vs alone")
 
 
+def test_gpu_frames_admitted_while_a_batch_runs_overlap_it_and_stay_exact(ctx):
+    """Continuous batching with work overlapping on the device: four 4K
+    frames then nine small frames submitted in one burst, no batching
+    window, at most 4 frames per batch and two batches in flight. Some batch
+    is launched while another is still on the GPU, and on the device clock
+    the two run at the same time (on one shared stream the later batch could
+    only start after the earlier one ended). Every frame's result equals its
+    result alone; the small frames equal the oracle."""
+    order = [av1pf.generate(workloads.cfg2(), k) for k in (0, 1, 2, 3)]
+    smalls = [F.random_frame(200 + k, 64, 64) for k in range(9)]
+    order += smalls
+    dev_in, out_alone = device_io(order)
+    _, out_mix = device_io(order)
+    say("serve: per-frame results alone")
+    for k, f in enumerate(order):
+        ctx.frame_device(f, ptrs(dev_in[k]), ptrs(out_alone[k]))
+    ctx.finish()
+    say("serve: four 4K frames then nine small ones in one burst, window 0, 2 batches in flight")
+    with av1pf.Server(ctx, window_us=0, max_batch=4, max_inflight=2) as srv:
+        calls = [srv.submit_call(f, ptrs(dev_in[k]), ptrs(out_mix[k]), tag=k) for k, f in enumerate(order)]
+        for c in calls:
+            c()
+        srv.drain()
+        done = srv.poll()
+    assert sorted(c[0] for c in done) == list(range(len(order)))
+    assert all(c[1] == abi.OK for c in done), done
+    # per batch: (host launch, host seen done, device start, device end)
+    batches = sorted({(c[4], c[5], c[6], c[7]) for c in done})
+    assert len(batches) >= 2 and all(0 < b[2] < b[3] for b in batches), batches
+    assert any(l2 > l1 and l2 < d1 and s2 < e1 and s1 < e2
+               for (l1, d1, s1, e1) in batches for (l2, d2, s2, e2) in batches), batches
+    for k, f in enumerate(order):
+        mix = [t.download() for t in out_mix[k]]
+        alone = [t.download() for t in out_alone[k]]
+        for p in range(len(alone)):
+            assert np.array_equal(mix[p], alone[p]), (k, p, "mix vs alone")
+    for f in smalls:
+        idx = order.index(f)
+        for p, w in enumerate(O.frame(f.desc, f.planes)):
+            assert np.array_equal(out_mix[idx][p].download(), w), (idx, p, "vs oracle")
+
+
 def test_gpu_served_errors_stay_with_their_frame(ctx):
     frames = [F.random_frame(60 + k, 64, 64, skip_p=0.0, idx_neg_p=0.0) for k in range(5)]
     frames[1].planes[0][9, 9] = 999          # flagged by the kernels, settled by the host replay

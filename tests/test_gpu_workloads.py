@@ -73,3 +73,41 @@ def test_gpu_mixed_batch_class_matches_oracle(mixed_run, cls):
     want = O.frame(f.desc, f.planes)
     for p, (g, w) in enumerate(zip(got_b[k], want)):
         assert g.shape == w.shape and np.array_equal(g, w), f"frame {k} plane {p}"
+
+
+# ---------------------------------------------------------------- serving ---
+# The serving workload at its real shapes: 32 streams of fixed 720p / 1080p /
+# 2160p resolution and 8 / 10-bit depth, Poisson arrivals at 3,000 frames/s,
+# for 0.5 s (1,500 requests), 32 GiB budget, 2 ms window.
+
+STREAM_CFGS = workloads.serve_streams(32)
+STREAM_CLASSES = {}
+for _s, _c in enumerate(STREAM_CFGS):
+    STREAM_CLASSES.setdefault((int(_c.width), int(_c.height), int(_c.bit_depth)), _s)
+
+
+@pytest.fixture(scope="module")
+def served_run():
+    from vaco_amd import streams
+    say("serving: 32 Poisson streams at 3000 frames/s for 0.5 s")
+    run = streams.serve_poisson(0.5, fps=3000.0)
+    say(f"serving: {run.stats}")
+    yield run
+
+
+def test_gpu_served_streams_equal_the_per_frame_path(served_run):
+    st = served_run.stats
+    assert st["frames"] == 1500 and st["spot_checks"] >= 15
+    for i, (s, k, planes) in served_run.checked.items():
+        for p, (a, b) in enumerate(zip(planes, served_run.reference[(s, k)])):
+            assert np.array_equal(a, b), f"request {i} (stream {s}, frame {k}) plane {p}"
+
+
+@pytest.mark.parametrize("cls", sorted(STREAM_CLASSES), ids=lambda c: f"{c[0]}x{c[1]}-bd{c[2]}")
+def test_gpu_served_stream_class_matches_oracle(served_run, cls):
+    s = STREAM_CLASSES[cls]
+    f = served_run.frames[s][0]
+    say(f"serving: stream {s} {cls} frame 0 vs oracle")
+    want = O.frame(f.desc, f.planes)
+    for p, (g, w) in enumerate(zip(served_run.reference[(s, 0)], want)):
+        assert g.shape == w.shape and np.array_equal(g, w), f"stream {s} plane {p}"

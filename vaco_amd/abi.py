@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 # VACO_AV1PF_ABI_VERSION of include/vaco_av1pf.h that these layouts follow.
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK = 0
 INVALID_DATA = 1
@@ -199,6 +199,8 @@ class Completion(C.Structure):
         ("t_submit_ns", C.c_uint64),
         ("t_launch_ns", C.c_uint64),
         ("t_done_ns", C.c_uint64),
+        ("t_gpu_start_ns", C.c_uint64),
+        ("t_gpu_end_ns", C.c_uint64),
     ]
 
 

@@ -482,7 +482,8 @@ class Server:
     """The serving loop (``vaco_av1pf_server``) over a Context it owns while open.
 
     submit() queues one frame on device planes; poll() returns completions
-    as ``(tag, status, batch_size, t_submit_ns, t_launch_ns, t_done_ns)``.
+    as ``(tag, status, batch_size, t_submit_ns, t_launch_ns, t_done_ns,
+    t_gpu_start_ns, t_gpu_end_ns)``.
     """
 
     def __init__(self, ctx: Context, hbm_budget_bytes: int = 8 << 30, window_us: int = 2000,
@@ -498,8 +499,9 @@ class Server:
         self._keep = {}
         self._buf = (abi.Completion * 256)()
 
-    def submit_call(self, frame: Frame, in_ptrs, out_ptrs, tag: int):
-        """A zero-argument callable submitting `frame` on these device planes."""
+    def submit_call(self, frame: Frame, in_ptrs, out_ptrs, tag: int = 0):
+        """A callable submitting `frame` on these device planes, with `tag` or
+        the tag it is called with. The ctypes arguments are built once."""
         geo_out = frame.output_geometry()
         pin, pout = abi.Picture(), abi.Picture()
         for i in range(frame.nplanes):
@@ -507,9 +509,9 @@ class Server:
             pin.plane[i] = abi.device_plane(in_ptrs[i], w, h)
             pout.plane[i] = abi.device_plane(out_ptrs[i], geo_out[i][1], geo_out[i][0])
         fn, h = self._lib.vaco_av1pf_server_submit, self.handle
-        d, pi, po, t = C.byref(frame.desc), C.byref(pin), C.byref(pout), C.c_uint64(tag)
+        d, pi, po = C.byref(frame.desc), C.byref(pin), C.byref(pout)
 
-        def call():
+        def call(t: int = tag):
             rc = fn(h, d, pi, po, t)
             if rc:
                 raise Av1Error(rc, "vaco_av1pf_server_submit failed")
@@ -529,7 +531,8 @@ class Server:
             for i in range(n):
                 c = self._buf[i]
                 self._keep.pop(c.tag, None)
-                out.append((c.tag, c.status, c.batch_size, c.t_submit_ns, c.t_launch_ns, c.t_done_ns))
+                out.append((c.tag, c.status, c.batch_size, c.t_submit_ns, c.t_launch_ns, c.t_done_ns,
+                            c.t_gpu_start_ns, c.t_gpu_end_ns))
             if n < 256:
                 return out
 

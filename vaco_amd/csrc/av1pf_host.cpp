@@ -124,6 +124,12 @@ struct vaco_av1pf_ctx {
     // device intermediates of batched calls (grown on demand)
     uint16_t *ws = nullptr;
     size_t ws_cap = 0;  // samples
+    // The last call that used the context's images or batch workspace: the
+    // stream it ran on and the event that ends it. A call on another stream
+    // waits for it first, so calls on different streams never overlap on
+    // those buffers.
+    hipStream_t last_stream = nullptr;
+    hipEvent_t last_done = nullptr;
     // kernel timing (vaco_av1pf_profile)
     struct ProfRec {
         int kernel;
@@ -271,14 +277,28 @@ static vaco_av1pf_status upload_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t 
 
 // Marks the end of the slot's frames on stream s; host_visible: the event is
 // handed on as the frames' completion (the server), so it carries the
-// system-scope fence.
+// system-scope fence. shared: the call used the context's own images or
+// batch workspace, so later calls on other streams must order after it.
 static vaco_av1pf_status close_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t s,
-                                    bool host_visible = false)
+                                    bool host_visible = false, bool shared = true)
 {
     a->done_ev = host_visible ? a->done_sys : a->done;
     a->stream = s;
     HIPCHECK(hipEventRecord(a->done_ev, s));
     a->pending = true;
+    if (shared) {
+        ctx->last_stream = s;
+        ctx->last_done = a->done_ev;
+    }
+    return VACO_AV1PF_OK;
+}
+
+// A call about to use the context's images or batch workspace on stream s:
+// if the previous such call ran on another stream, s waits for its end.
+static vaco_av1pf_status order_after_previous(vaco_av1pf_ctx *ctx, hipStream_t s)
+{
+    if (ctx->last_done && s != ctx->last_stream)
+        HIPCHECK(hipStreamWaitEvent(s, ctx->last_done, 0));
     return VACO_AV1PF_OK;
 }
 
@@ -1183,6 +1203,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame(vaco_av1pf_ctx *ctx, const vaco_av
         return st;
     if ((st = grow_slot(ctx, a, arena_need(*desc, g))) != VACO_AV1PF_OK)
         return st;
+    if ((st = order_after_previous(ctx, ctx->stream)) != VACO_AV1PF_OK)
+        return st;
     DevPlane din[3], dout[3];
     for (int p = 0; p < g.nplanes; p++) {
         din[p] = DevPlane{ctx->img[0][p], g.in_w[p]};
@@ -1244,6 +1266,8 @@ static vaco_av1pf_status frame_device_to(vaco_av1pf_ctx *ctx, const vaco_av1pf_f
     if ((st = take_slot(ctx, &a)) != VACO_AV1PF_OK)
         return st;
     if ((st = grow_slot(ctx, a, arena_need(*desc, g))) != VACO_AV1PF_OK)
+        return st;
+    if ((st = order_after_previous(ctx, s)) != VACO_AV1PF_OK)
         return st;
     DevPlane din[3];
     for (int p = 0; p < g.nplanes; p++)
@@ -1402,6 +1426,99 @@ size_t av1pf_frame_device_bytes(const vaco_av1pf_frame_desc &d)
 
 int av1pf_ctx_device(const vaco_av1pf_ctx *ctx) { return ctx->device; }
 
+size_t av1pf_frame_stage_bytes(const vaco_av1pf_frame_desc &d)
+{
+    FrameGeo g;
+    frame_geometry(d, &g);
+    return arena_need(d, g);
+}
+
+size_t av1pf_frame_scratch_samples(const vaco_av1pf_frame_desc &d)
+{
+    FrameGeo g;
+    frame_geometry(d, &g);
+    return scratch_need(g);
+}
+
+// The largest frame a context accepts: 4:4:4 at its maximum geometry, with
+// CDEF, superres and 32-sample restoration units in every plane.
+static void worst_frame(const vaco_av1pf_ctx *ctx, vaco_av1pf_frame_desc &d, FrameGeo &g)
+{
+    static const int8_t idx = 0;
+    static const uint8_t skip = 0;
+    memset(&d, 0, sizeof d);
+    d.coded_width = ctx->max_w - 8;
+    d.upscaled_width = ctx->max_w;
+    d.coded_height = ctx->max_h;
+    d.use_superres = 1;
+    d.bit_depth = 12;
+    d.cdef.enabled = 1;
+    d.cdef.y[0].primary = 1;
+    d.cdef_idx = &idx;
+    d.skip8 = &skip;
+    d.cdef_stride = (ctx->max_w / 4 + 15) / 16;
+    for (int p = 0; p < 3; p++) {
+        d.lr_type[p] = VACO_AV1PF_RESTORE_SWITCHABLE;
+        d.lr_unit_size[p] = 32;
+        uint32_t c, r;
+        lr_unit_counts(ctx->max_w, ctx->max_h, 32, &c, &r);
+        d.lr_unit_count[p] = c * r;
+    }
+    frame_geometry(d, &g);
+    // the Mi-padded input of a frame coded max_w wide (no superres)
+    g.in_w[0] = g.in_w[1] = g.in_w[2] = ctx->max_w;
+}
+
+size_t av1pf_worst_frame_stage_bytes(const vaco_av1pf_ctx *ctx)
+{
+    vaco_av1pf_frame_desc d;
+    FrameGeo g;
+    worst_frame(ctx, d, g);
+    return arena_need(d, g);
+}
+
+size_t av1pf_worst_frame_scratch_samples(const vaco_av1pf_ctx *ctx)
+{
+    vaco_av1pf_frame_desc d;
+    FrameGeo g;
+    worst_frame(ctx, d, g);
+    return scratch_need(g);
+}
+
+vaco_av1pf_status av1pf_reserve_staging(vaco_av1pf_ctx *ctx, size_t bytes)
+{
+    bytes = (bytes + 255) & ~size_t(255);
+    HIPCHECK(hipSetDevice(ctx->device));
+    for (int i = 0; i < kSlots; i++) {
+        Arena &a = ctx->slot[i];
+        if (a.cap >= bytes)
+            continue;
+        if (a.pending)
+            HIPCHECK(hipEventSynchronize(a.done_ev));
+        HIPCHECK(alloc_slot(a, bytes));
+    }
+    return VACO_AV1PF_OK;
+}
+
+size_t av1pf_staging_capacity(const vaco_av1pf_ctx *ctx)
+{
+    size_t c = ctx->slot[0].cap;
+    for (int i = 1; i < kSlots; i++)
+        c = std::min(c, ctx->slot[i].cap);
+    return c;
+}
+
+int av1pf_next_slot(const vaco_av1pf_ctx *ctx) { return ctx->next_slot; }
+
+void av1pf_settle_slot(vaco_av1pf_ctx *ctx, int slot)
+{
+    if (slot < 0 || slot >= kSlots)
+        return;
+    Arena &a = ctx->slot[slot];
+    a.pending = false;
+    a.stream = nullptr;
+}
+
 // Settle one frame's sample-range flag: replay the reference's checks over a
 // host copy of its device input. Returns the reference's outcome.
 vaco_av1pf_status av1pf_replay_device_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_desc &d,
@@ -1432,7 +1549,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32
     if (!ctx)
         return VACO_AV1PF_INVALID_ARGUMENT;
     return av1pf_enqueue_batch(ctx, n, descs, ins, outs,
-                               stream ? (hipStream_t)stream : ctx->stream, nullptr, nullptr);
+                               stream ? (hipStream_t)stream : ctx->stream, nullptr, nullptr, nullptr,
+                               nullptr);
 }
 
 // The batched call. With `statuses` set, the caller takes over the frames'
@@ -1443,7 +1561,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32
 vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
                                       const vaco_av1pf_frame_desc *descs,
                                       const vaco_av1pf_picture *ins, vaco_av1pf_picture *outs,
-                                      hipStream_t s, const uint32_t **statuses, hipEvent_t *done)
+                                      hipStream_t s, const uint32_t **statuses, hipEvent_t *done,
+                                      const av1pf_workspace *own_ws, int *slot_index)
 {
     if (!ctx || (n && (!descs || !ins || !outs)))
         return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "null argument");
@@ -1456,7 +1575,7 @@ vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
     if (!s)
         s = ctx->stream;
     std::vector<FrameGeo> geo(n);
-    size_t ws_need = 0, arena = 1 << 16;
+    size_t ws_need = 0, arena = kBatchStageBase;
     for (uint32_t i = 0; i < n; i++) {
         frame_geometry(descs[i], &geo[i]);
         vaco_av1pf_status st = check_planes(ctx, geo[i], descs[i].cdef_stride, &ins[i], &outs[i]);
@@ -1485,11 +1604,21 @@ vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
         ws_need += scratch_need(geo[i]);
         arena += arena_need(descs[i], geo[i]);
     }
-    // Intermediates of every frame of the batch; grown only when a batch
-    // needs more than any batch before it.
-    if (ws_need > ctx->ws_cap) {
+    Scratch ws;
+    if (own_ws) {
+        // The caller's workspace and staging capacity were sized for this
+        // batch (the server forms batches to fit them): nothing grows.
+        if (ws_need > own_ws->cap)
+            return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "batch exceeds its workspace");
+        ws.base = own_ws->base;
+        ws.cap = own_ws->cap;
+    } else if (ws_need > ctx->ws_cap) {
+        // Intermediates of every frame of the batch; grown only when a batch
+        // needs more than any batch before it.
         HIPCHECK(hipStreamSynchronize(s));
         HIPCHECK(hipStreamSynchronize(ctx->stream));
+        if (ctx->last_done)
+            HIPCHECK(hipEventSynchronize(ctx->last_done));
         if (ctx->ws)
             HIPCHECK(hipFree(ctx->ws));
         ctx->ws = nullptr;
@@ -1497,15 +1626,22 @@ vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
         HIPCHECK(hipMalloc((void **)&ctx->ws, ws_need * sizeof(uint16_t)));
         ctx->ws_cap = ws_need;
     }
+    if (!own_ws) {
+        ws.base = ctx->ws;
+        ws.cap = ctx->ws_cap;
+    }
+    if (slot_index)
+        *slot_index = ctx->next_slot;
     Arena *a;
     vaco_av1pf_status st = take_slot(ctx, &a);
     if (st != VACO_AV1PF_OK)
         return st;
+    if (own_ws && arena > a->cap)
+        return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "batch exceeds its staging slot");
     if ((st = grow_slot(ctx, a, arena)) != VACO_AV1PF_OK)
         return st;
-    Scratch ws;
-    ws.base = ctx->ws;
-    ws.cap = ctx->ws_cap;
+    if (!own_ws && (st = order_after_previous(ctx, s)) != VACO_AV1PF_OK)
+        return st;
     Plan plan;
     plan.tune = ctx->tune;
     for (uint32_t i = 0; i < n; i++) {
@@ -1520,7 +1656,7 @@ vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
     }
     if ((st = launch_plan(ctx, plan, a, s)) != VACO_AV1PF_OK)
         return st;
-    if ((st = close_slot(ctx, a, s, done != nullptr)) != VACO_AV1PF_OK)
+    if ((st = close_slot(ctx, a, s, done != nullptr, own_ws == nullptr)) != VACO_AV1PF_OK)
         return st;
     if (statuses) {
         *statuses = a->status;
@@ -1623,13 +1759,14 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
     auto measure = [&](const vaco_av1pf_tuning &t, int kernel, double *res) {
         ctx->tune = t;
         *res = 0.0;
-        vaco_av1pf_status r = av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr);
+        vaco_av1pf_status r =
+            av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr, nullptr, nullptr);
         if (r == VACO_AV1PF_OK)
             r = vaco_av1pf_finish(ctx);
         if (r == VACO_AV1PF_OK)
             r = vaco_av1pf_profile_read(ctx, nullptr, nullptr, nullptr);
         for (uint32_t i = 0; i < reps && r == VACO_AV1PF_OK; i++) {
-            r = av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr);
+            r = av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr, nullptr, nullptr);
             if (r == VACO_AV1PF_OK)
                 r = vaco_av1pf_finish(ctx);
             if (r == VACO_AV1PF_OK)

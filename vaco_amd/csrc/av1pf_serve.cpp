@@ -4,10 +4,19 @@
 // One worker thread owns the context. Submitters hand requests over through
 // a mutex-protected queue; the worker admits them FIFO while the device
 // bytes in flight stay within the budget, launches batches through the
-// batched path (av1pf_enqueue_batch) on the context's stream, retires
-// batches whose event has completed, settles each frame's status and
-// publishes completions. Frames in one batch run the same kernels as a
-// frame alone, so results do not depend on what they were batched with.
+// batched path (av1pf_enqueue_batch), retires every batch whose event has
+// completed, settles each frame's status and publishes completions. Frames
+// in one batch run the same kernels as a frame alone, so results do not
+// depend on what they were batched with.
+//
+// Each batch in flight has a stream and an intermediate-image workspace of
+// its own (max_inflight of each), so batches overlap on the device when the
+// runtime gives each stream a hardware queue (its pool holds
+// GPU_MAX_HW_QUEUES queues per process; streams beyond that share them and
+// then run one after the other). The
+// workspaces and the context's staging slots are sized at server_create;
+// the worker closes a batch early rather than let it outgrow either, so no
+// device or pinned memory is allocated after creation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,22 +49,38 @@ struct Request {
     vaco_av1pf_frame_desc desc;
     vaco_av1pf_picture in, out;
     uint64_t tag = 0, t_submit = 0, t_launch = 0;
-    size_t bytes = 0;
+    size_t bytes = 0;    // device bytes in flight (admission)
+    size_t stage = 0;    // staging-slot bytes
+    size_t scratch = 0;  // workspace samples
     bool replay = false;  // parameters can fail: settle with the reference's checks first
+    uint64_t gpu_start = 0, gpu_end = 0;  // device clock of its batch
 };
 
 struct Batch {
     std::vector<Request> reqs;
     const uint32_t *status = nullptr;
     hipEvent_t done = nullptr;
+    int stream = -1;  // index into the server's streams and workspaces
+    int slot = -1;    // the context's staging slot holding the status words
+    uint64_t gpu_start = 0, gpu_end = 0;  // device clock (publish)
 };
+
+constexpr int kMaxStreams = 4;
 
 }  // namespace
 
 struct vaco_av1pf_server {
     vaco_av1pf_ctx *ctx = nullptr;
     vaco_av1pf_server_config cfg{};
-    hipStream_t stream = nullptr;
+    hipStream_t streams[kMaxStreams] = {};
+    // Device-clock markers: t0 when the server started, and a start / end
+    // pair per stream around the batch in flight on it.
+    hipEvent_t t0 = nullptr;
+    hipEvent_t mark[kMaxStreams][2] = {};
+    uint16_t *ws_mem = nullptr;  // max_inflight workspaces of ws_cap samples
+    size_t ws_cap = 0;
+    size_t stage_cap = 0;  // staging bytes of one batch
+    bool stream_busy[kMaxStreams] = {};
     std::mutex mu;
     std::condition_variable cv;
     std::deque<Request> incoming;
@@ -65,7 +90,7 @@ struct vaco_av1pf_server {
     std::thread worker;
     // worker-owned
     std::deque<Request> waiting;
-    std::deque<Batch> inflight;
+    std::vector<Batch> inflight;
     size_t bytes_in_flight = 0;
 };
 
@@ -79,6 +104,8 @@ static void publish(vaco_av1pf_server *s, const Request &r, int32_t status, uint
     c.t_submit_ns = r.t_submit;
     c.t_launch_ns = r.t_launch;
     c.t_done_ns = t_done;
+    c.t_gpu_start_ns = r.gpu_start;
+    c.t_gpu_end_ns = r.gpu_end;
     std::lock_guard<std::mutex> lk(s->mu);
     s->completions.push_back(c);
     s->completed.fetch_add(1);
@@ -87,9 +114,25 @@ static void publish(vaco_av1pf_server *s, const Request &r, int32_t status, uint
 
 // Publish a finished batch. `device_ok` false: the device reported an error
 // for the batch's event, and every frame of it fails with DEVICE_ERROR.
+// The marker's device time, ns since t0 (0 if it cannot be read).
+static uint64_t since_t0_ns(vaco_av1pf_server *s, hipEvent_t e)
+{
+    float ms = 0.f;
+    if (hipEventSynchronize(e) != hipSuccess || hipEventElapsedTime(&ms, s->t0, e) != hipSuccess || ms <= 0.f)
+        return 0;
+    return (uint64_t)((double)ms * 1e6);
+}
+
 static void retire(vaco_av1pf_server *s, Batch &b, bool device_ok)
 {
     const uint64_t t = vaco_av1pf_now_ns();
+    if (device_ok) {
+        const uint64_t g0 = since_t0_ns(s, s->mark[b.stream][0]), g1 = since_t0_ns(s, s->mark[b.stream][1]);
+        for (auto &r : b.reqs) {
+            r.gpu_start = g0;
+            r.gpu_end = g1;
+        }
+    }
     for (size_t i = 0; i < b.reqs.size(); i++) {
         int32_t st = device_ok ? VACO_AV1PF_OK : VACO_AV1PF_DEVICE_ERROR;
         if (device_ok && (b.status[i] & kFlagSampleRange)) {
@@ -99,12 +142,41 @@ static void retire(vaco_av1pf_server *s, Batch &b, bool device_ok)
         s->bytes_in_flight -= b.reqs[i].bytes;
         publish(s, b.reqs[i], st, (uint32_t)b.reqs.size(), t);
     }
+    // The batch's event has completed: the context must not wait on it (or
+    // its stream) again, since the stream goes away with the server.
+    av1pf_settle_slot(s->ctx, b.slot);
+    s->stream_busy[b.stream] = false;
+}
+
+// Retire every batch that has completed; with `wait_slot` >= 0, first wait
+// for the batch whose status words are in that staging slot (the next call
+// takes it and clears them).
+static void retire_done(vaco_av1pf_server *s, int wait_slot = -1)
+{
+    for (size_t i = 0; i < s->inflight.size();) {
+        Batch &b = s->inflight[i];
+        hipError_t q = b.slot == wait_slot ? hipEventSynchronize(b.done) : hipEventQuery(b.done);
+        if (q == hipErrorNotReady) {
+            i++;
+            continue;
+        }
+        retire(s, b, q == hipSuccess);
+        s->inflight.erase(s->inflight.begin() + (long)i);
+    }
 }
 
 static void launch(vaco_av1pf_server *s)
 {
     Batch b;
-    size_t budget = s->bytes_in_flight;
+    for (int i = 0; i < (int)s->cfg.max_inflight; i++)
+        if (!s->stream_busy[i]) {
+            b.stream = i;
+            break;
+        }
+    if (b.stream < 0)
+        return;
+    retire_done(s, av1pf_next_slot(s->ctx));
+    size_t budget = s->bytes_in_flight, stage = kBatchStageBase, scratch = 0;
     while (!s->waiting.empty() && b.reqs.size() < s->cfg.max_batch) {
         Request &r = s->waiting.front();
         if (r.replay) {  // parameter errors are reported without running
@@ -120,7 +192,11 @@ static void launch(vaco_av1pf_server *s)
         }
         if (budget + r.bytes > s->cfg.hbm_budget_bytes)
             break;  // FIFO: wait for memory to free up
+        if (!b.reqs.empty() && (stage + r.stage > s->stage_cap || scratch + r.scratch > s->ws_cap))
+            break;  // the rest goes into the next batch
         budget += r.bytes;
+        stage += r.stage;
+        scratch += r.scratch;
         b.reqs.push_back(r);
         s->waiting.pop_front();
     }
@@ -136,14 +212,22 @@ static void launch(vaco_av1pf_server *s)
         outs[i] = b.reqs[i].out;
         b.reqs[i].t_launch = t;
     }
-    const vaco_av1pf_status st = av1pf_enqueue_batch(s->ctx, (uint32_t)n, descs.data(), ins.data(),
-                                                     outs.data(), s->stream, &b.status, &b.done);
+    const av1pf_workspace ws{s->ws_mem + (size_t)b.stream * s->ws_cap, s->ws_cap};
+    hipStream_t q = s->streams[b.stream];
+    vaco_av1pf_status st = hipEventRecord(s->mark[b.stream][0], q) == hipSuccess ? VACO_AV1PF_OK
+                                                                                : VACO_AV1PF_DEVICE_ERROR;
+    if (st == VACO_AV1PF_OK)
+        st = av1pf_enqueue_batch(s->ctx, (uint32_t)n, descs.data(), ins.data(), outs.data(), q, &b.status,
+                                 &b.done, &ws, &b.slot);
+    if (st == VACO_AV1PF_OK && hipEventRecord(s->mark[b.stream][1], q) != hipSuccess)
+        st = VACO_AV1PF_DEVICE_ERROR;
     if (st != VACO_AV1PF_OK) {
         for (auto &r : b.reqs)
             publish(s, r, st, (uint32_t)n, vaco_av1pf_now_ns());
         return;
     }
     s->bytes_in_flight = budget;
+    s->stream_busy[b.stream] = true;
     s->inflight.push_back(std::move(b));
 }
 
@@ -164,14 +248,8 @@ static void work(vaco_av1pf_server *s)
                 s->incoming.pop_front();
             }
         }
-        // Retire finished batches (in launch order).
-        while (!s->inflight.empty()) {
-            const hipError_t q = hipEventQuery(s->inflight.front().done);
-            if (q == hipErrorNotReady)
-                break;
-            retire(s, s->inflight.front(), q == hipSuccess);
-            s->inflight.pop_front();
-        }
+        // Retire finished batches, in whatever order they finish.
+        retire_done(s);
         // Launch: continuous batching with a window while the GPU is busy.
         if (!s->waiting.empty() && s->inflight.size() < s->cfg.max_inflight) {
             const uint64_t now = vaco_av1pf_now_ns();
@@ -197,7 +275,49 @@ extern "C" vaco_av1pf_status vaco_av1pf_server_create(vaco_av1pf_ctx *ctx,
         return VACO_AV1PF_DEVICE_ERROR;
     s->ctx = ctx;
     s->cfg = *cfg;
-    s->stream = nullptr;  // the context's stream
+    // Every buffer the worker uses is sized here. A workspace holds the
+    // intermediates of the largest frame the context accepts and otherwise
+    // up to 2 GiB (or the budget's share of one batch); a staging slot the
+    // largest frame's side information and otherwise up to 16 MiB.
+    const size_t worst_ws = av1pf_worst_frame_scratch_samples(ctx);
+    const size_t share = std::min<uint64_t>(2ull << 30, cfg->hbm_budget_bytes / cfg->max_inflight) / 2;
+    s->ws_cap = (std::max(worst_ws, share) + 127) & ~size_t(127);
+    const size_t worst_stage = kBatchStageBase + av1pf_worst_frame_stage_bytes(ctx);
+    const size_t want_stage =
+        std::max(worst_stage, std::min<size_t>(16u << 20, kBatchStageBase + cfg->max_batch *
+                                                           av1pf_worst_frame_stage_bytes(ctx)));
+    vaco_av1pf_status st = VACO_AV1PF_OK;
+    if (hipSetDevice(av1pf_ctx_device(ctx)) != hipSuccess)
+        st = VACO_AV1PF_DEVICE_ERROR;
+    if (st == VACO_AV1PF_OK)
+        st = av1pf_reserve_staging(ctx, want_stage);
+    s->stage_cap = av1pf_staging_capacity(ctx);
+    for (uint32_t i = 0; st == VACO_AV1PF_OK && i < cfg->max_inflight; i++)
+        if (hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&s->mark[i][0]) != hipSuccess || hipEventCreate(&s->mark[i][1]) != hipSuccess)
+            st = VACO_AV1PF_DEVICE_ERROR;
+    if (st == VACO_AV1PF_OK && (hipEventCreate(&s->t0) != hipSuccess ||
+                                hipEventRecord(s->t0, s->streams[0]) != hipSuccess ||
+                                hipEventSynchronize(s->t0) != hipSuccess))
+        st = VACO_AV1PF_DEVICE_ERROR;
+    if (st == VACO_AV1PF_OK &&
+        hipMalloc((void **)&s->ws_mem, s->ws_cap * cfg->max_inflight * sizeof(uint16_t)) != hipSuccess)
+        st = VACO_AV1PF_DEVICE_ERROR;
+    if (st != VACO_AV1PF_OK) {
+        for (int i = 0; i < kMaxStreams; i++) {
+            if (s->streams[i])
+                hipStreamDestroy(s->streams[i]);
+            for (auto &e : s->mark[i])
+                if (e)
+                    hipEventDestroy(e);
+        }
+        if (s->t0)
+            hipEventDestroy(s->t0);
+        if (s->ws_mem)
+            hipFree(s->ws_mem);
+        delete s;
+        return st;
+    }
     s->worker = std::thread(work, s);
     *out = s;
     return VACO_AV1PF_OK;
@@ -216,7 +336,10 @@ extern "C" vaco_av1pf_status vaco_av1pf_server_submit(vaco_av1pf_server *s,
     r.out = *out;
     r.tag = tag;
     r.bytes = av1pf_frame_device_bytes(*desc);
-    if (r.bytes > s->cfg.hbm_budget_bytes)
+    r.stage = av1pf_frame_stage_bytes(*desc);
+    r.scratch = av1pf_frame_scratch_samples(*desc);
+    if (r.bytes > s->cfg.hbm_budget_bytes || kBatchStageBase + r.stage > s->stage_cap ||
+        r.scratch > s->ws_cap)
         return VACO_AV1PF_LIMIT_EXCEEDED;
     FrameGeo g;
     frame_geometry(*desc, &g);
@@ -274,5 +397,18 @@ extern "C" void vaco_av1pf_server_destroy(vaco_av1pf_server *s)
     s->cv.notify_all();
     if (s->worker.joinable())
         s->worker.join();
+    for (int i = 0; i < kMaxStreams; i++) {
+        if (s->streams[i]) {
+            hipStreamSynchronize(s->streams[i]);
+            hipStreamDestroy(s->streams[i]);
+        }
+        for (auto &e : s->mark[i])
+            if (e)
+                hipEventDestroy(e);
+    }
+    if (s->t0)
+        hipEventDestroy(s->t0);
+    if (s->ws_mem)
+        hipFree(s->ws_mem);
     delete s;
 }

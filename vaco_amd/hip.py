@@ -31,6 +31,9 @@ def rt():
         R.hipSetDevice.argtypes = [C.c_int]
         R.hipGetErrorString.argtypes = [C.c_int]
         R.hipGetErrorString.restype = C.c_char_p
+        R.hipStreamCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+        R.hipStreamDestroy.argtypes = [C.c_void_p]
+        R.hipStreamSynchronize.argtypes = [C.c_void_p]
         _rt = R
     return _rt
 
@@ -90,5 +93,29 @@ class DeviceArray:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    """A non-blocking HIP stream at the default priority (caller streams for
+    the library's *_device entry points; `handle` is the hipStream_t)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(rt().hipStreamCreateWithFlags(C.byref(h), 1), "hipStreamCreateWithFlags")
+        self.handle = h.value
+
+    def synchronize(self):
+        check(rt().hipStreamSynchronize(C.c_void_p(self.handle)), "hipStreamSynchronize")
+
+    def destroy(self):
+        if self.handle:
+            rt().hipStreamDestroy(C.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
         except Exception:
             pass
