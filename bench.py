@@ -22,7 +22,10 @@ Beside the headline figure the same line carries:
 * ``batched``: 64 mixed frames, one batched call against 64 per-frame calls;
 * ``served``: the serving loop under 32 Poisson streams at 3,000 frames/s,
   p50/p99 latency;
-* ``cpu_baseline``: the oracle on one core.
+* ``cpu_baseline``: the oracle on one core (the reference is single-threaded),
+  with the same oracle on N host threads, single-threaded samples of the
+  352x288 and 4K 10-bit workloads, and the host CPU's model;
+* ``per_config_gpu``: the plain per-frame rate of the 352x288 workload.
 
 Prints one JSON line (rank 0).
 """
@@ -130,6 +133,92 @@ def measure_served(av1pf, hip, workloads, seconds: float, fps: float = 3000.0, w
     exact = all(np.array_equal(a, b) for s, k, planes in run.checked.values()
                 for a, b in zip(planes, run.reference[(s, k)]))
     return {**run.stats, "bit_exact_spot_checks": bool(exact)}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads() -> int:
+    """Host threads for the N-core baseline: the cores this process may
+    run on, at most 16 (the host share of one GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+def cpu_extra_samples(O, av1pf, workloads, frames, seconds):
+    """Beside the 1-core figure: the oracle on N host threads (one frame per
+    thread at a time; the C oracle runs without the interpreter lock), and
+    single-threaded samples of the 352x288 and 4K 10-bit superres workloads."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    threads = cpu_threads()
+    mpix = frames[0].output_luma_pixels / 1e6
+    deadline = time.perf_counter() + seconds / 2
+
+    def worker(w):
+        runs = 0
+        while time.perf_counter() < deadline:
+            f = frames[(w + runs) % len(frames)]
+            O.frame(f.desc, f.planes)
+            runs += 1
+        return runs
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        runs = sum(ex.map(worker, range(threads)))
+    wall = time.perf_counter() - t0
+    c0 = [av1pf.generate(workloads.cfg0(), k) for k in range(4)]
+    t0, n0 = time.perf_counter(), 0
+    while time.perf_counter() - t0 < 2.0:
+        f = c0[n0 % 4]
+        O.frame(f.desc, f.planes)
+        n0 += 1
+    t_c0 = time.perf_counter() - t0
+    c2 = av1pf.generate(workloads.cfg2(), 1)  # denominator 9, the widest coded frame
+    t0 = time.perf_counter()
+    O.frame(c2.desc, c2.planes)
+    t_c2 = time.perf_counter() - t0
+    return {
+        "cpu_model": cpu_model(),
+        "n_core": {"value": round(mpix * runs / wall, 4), "unit": "Mpix/s", "cores": threads,
+                   "sample": f"{runs} runs of the oracle frame stage on the 1080p frames over {wall:.1f} s, "
+                             f"{threads} threads (the project's own extension; the reference is single-threaded)"},
+        "cfg0_352x288": {"value": round(c0[0].output_luma_pixels / 1e6 * n0 / t_c0, 4), "unit": "Mpix/s",
+                         "cores": 1, "sample": f"{n0} runs over 4 seeded frames ({t_c0:.1f} s)"},
+        "cfg2_4k_10bit_superres": {"value": round(c2.output_luma_pixels / 1e6 / t_c2, 4), "unit": "Mpix/s",
+                                   "cores": 1, "sample": f"frame 1 (denominator 9) once ({t_c2:.1f} s)"},
+    }
+
+
+def measure_plain(av1pf, hip, cfg, n_frames, steps):
+    """Plain per-frame GPU rate of a workload: `steps` device calls cycling
+    over `n_frames` seeded frames, default launch variants."""
+    frames = [av1pf.generate(cfg, k) for k in range(n_frames)]
+    ctx = av1pf.Context(int(cfg.width), (int(cfg.height) + 63) // 64 * 64)
+    dev_in = [[hip.DeviceArray.from_host(p) for p in f.planes] for f in frames]
+    dev_out = [[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames]
+    calls = [ctx.device_call(f, [t.ptr for t in dev_in[k]], [t.ptr for t in dev_out[k]])
+             for k, f in enumerate(frames)]
+    for c in calls:
+        c()
+    ctx.finish()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        calls[i % len(calls)]()
+    ctx.finish()
+    hip.synchronize()
+    t = time.perf_counter() - t0
+    ctx.close()
+    return {"mpix_s": round(frames[0].output_luma_pixels / 1e6 * steps / t, 2),
+            "us_per_frame": round(t * 1e6 / steps, 2)}
 
 
 def kernel_table(prof, steps):
@@ -400,6 +489,8 @@ def main():
     log("batched leg:", json.dumps(batched))
     served = measure_served(av1pf, hip, workloads, args.serve_seconds)
     log("served leg:", json.dumps(served))
+    per_config = {"cfg0_352x288_8bit": measure_plain(av1pf, hip, workloads.cfg0(), 4, 500)}
+    log("352x288 leg:", json.dumps(per_config))
     saturated = measure_served(av1pf, hip, workloads, max(1.0, args.serve_seconds / 2), fps=SATURATING_FPS,
                                window_us=500)
     served["saturating_load"] = saturated
@@ -417,6 +508,7 @@ def main():
         want = O.frame(f.desc, f.planes)
         cpu_times.append(time.perf_counter() - t)
         exact &= all(np.array_equal(g, w) for g, w in zip(got, want))
+    cpu_extra = {}
     if not args.no_cpu:
         budget_end = time.perf_counter() + args.cpu_seconds
         k = 0
@@ -426,6 +518,7 @@ def main():
             O.frame(f.desc, f.planes)
             cpu_times.append(time.perf_counter() - t)
             k += 1
+        cpu_extra = cpu_extra_samples(O, av1pf, workloads, frames, args.cpu_seconds)
     mpix = frames[0].output_luma_pixels / 1e6
     cpu_mpix_s = mpix * len(cpu_times) / sum(cpu_times)
 
@@ -481,7 +574,9 @@ def main():
             "kind": "port",
             "sample": f"{len(cpu_times)} runs of the oracle frame stage over the same seeded 1080p frames "
                       f"({sum(cpu_times):.1f} s single-threaded)",
+            **cpu_extra,
         },
+        "per_config_gpu": per_config,
     }
     print(json.dumps(rec), flush=True)
 

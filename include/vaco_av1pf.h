@@ -433,7 +433,9 @@ vaco_av1pf_status vaco_av1pf_set_tuning(vaco_av1pf_ctx *ctx, const vaco_av1pf_tu
  * kernels by their dispatch events (the fastest of the `reps` runs counts),
  * keeps the fastest variant per kernel as the context's tuning and returns
  * it in *out. A field leaves its current value only for a variant at least
- * 2% faster. Synchronous; the outputs hold the frames' results. */
+ * 2% faster, and for the CDEF order that moves more HBM bytes
+ * (cdef_xcd_order 0) at least 5% faster. Synchronous; the outputs hold the
+ * frames' results. */
 vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n,
                                       const vaco_av1pf_frame_desc *descs,
                                       const vaco_av1pf_picture *ins, vaco_av1pf_picture *outs,

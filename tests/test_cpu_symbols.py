@@ -294,10 +294,11 @@ def lr_desc(w, h, types, units, ssx=1, ssy=1, mono=False):
     return d, counts
 
 
-def encode_units(rng, d, counts, sb128):
+def encode_units(rng, d, counts, sb128, per_sb=None):
     """Random units per plane, and the tile bytes coding them in the order
     the decoder reads them: superblocks in raster order, planes in order, then
-    the units whose top-left sample lies inside the superblock."""
+    the units whose top-left sample lies inside the superblock. per_sb(enc,
+    r, c), if given, codes the rest of each superblock after its units."""
     mi_cols = 2 * ((d.coded_width + 7) >> 3)
     mi_rows = 2 * ((d.coded_height + 7) >> 3)
     sb4 = 32 if sb128 else 16
@@ -358,6 +359,8 @@ def encode_units(rng, d, counts, sb128):
                             want[p][(ur, uc)] = ("S", s, tuple(x))
                         else:
                             want[p][(ur, uc)] = ("N",)
+            if per_sb:
+                per_sb(enc, r, c)
     return enc.finish(), want, order
 
 

@@ -1779,10 +1779,14 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
     vaco_av1pf_tuning best = ctx->tune, t;
     // One field at a time: the current value is timed first, and another
     // replaces it only when at least 2% faster, so that timing noise between
-    // equivalent variants does not flip the choice. A kernel absent from the
-    // batch times 0 and keeps its value.
+    // equivalent variants does not flip the choice. A value listed in
+    // `costly` moves more HBM bytes than the others (the plain CDEF order
+    // reads each superblock's halo lines into a different L2 than its
+    // neighbours: 2.3x the algorithmic bytes against 1.1x), so it must be at
+    // least 5% faster. A kernel absent from the batch times 0 and keeps its
+    // value.
     auto tune_field = [&](uint8_t vaco_av1pf_tuning::*field, std::initializer_list<uint8_t> values,
-                          int kernel) {
+                          int kernel, int costly = -1) {
         double keep_ms = 0, tm = 0;
         if (st != VACO_AV1PF_OK || (st = measure(best, kernel, &keep_ms)) != VACO_AV1PF_OK || keep_ms <= 0)
             return;
@@ -1794,7 +1798,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
             t.*field = v;
             if ((st = measure(t, kernel, &tm)) != VACO_AV1PF_OK)
                 return;
-            if (tm > 0 && tm < keep_ms * 0.98) {
+            if (tm > 0 && tm < keep_ms * (v == costly ? 0.95 : 0.98)) {
                 keep_ms = tm;
                 keep = v;
             }
@@ -1802,7 +1806,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
         best.*field = keep;
     };
     tune_field(&vaco_av1pf_tuning::superres_rows, {4, 8, 16}, VACO_AV1PF_KERNEL_SUPERRES);
-    tune_field(&vaco_av1pf_tuning::cdef_xcd_order, {0, 1}, VACO_AV1PF_KERNEL_CDEF);
+    tune_field(&vaco_av1pf_tuning::cdef_xcd_order, {0, 1}, VACO_AV1PF_KERNEL_CDEF, 0);
     tune_field(&vaco_av1pf_tuning::cdef_waves, {16, 8}, VACO_AV1PF_KERNEL_CDEF);
     tune_field(&vaco_av1pf_tuning::lr_order, {0, 1}, VACO_AV1PF_KERNEL_RESTORATION);
     tune_field(&vaco_av1pf_tuning::lr_tile_rows, {64, 32, 16}, VACO_AV1PF_KERNEL_RESTORATION);
