@@ -77,48 +77,87 @@ def measure_mixed_batch(av1pf, hip, workloads, reps: int, n: int = 64):
     """64 seeded frames of mixed geometry (360p/720p/1080p/2160p), bit depth
     (8/10) and superres (half the frames): the per-frame path (one call per
     frame) against one batched call over all 64, on the same device planes.
-    The batched outputs must equal the per-frame outputs bit for bit."""
+    The per-frame path runs with the variants tuned on a single 1080p frame
+    and with those tuned on the batch, and the faster counts; the batched
+    call with the batch's. Both are timed in steady state, `reps` calls back
+    to back. The batched outputs must equal the per-frame outputs bit for
+    bit."""
     frames = [av1pf.generate(c, i) for c, i in workloads.mixed_batch(n)]
+    ctx_pf = av1pf.Context(3840, 2176)
     ctx = av1pf.Context(3840, 2176)
     dev_in = [[hip.DeviceArray.from_host(p) for p in f.planes] for f in frames]
     out_pf = [[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames]
     out_b = [[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames]
     ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
-    per_frame = [ctx.device_call(f, ptr(dev_in)[k], ptr(out_pf)[k]) for k, f in enumerate(frames)]
+    per_frame = [ctx_pf.device_call(f, ptr(dev_in)[k], ptr(out_pf)[k]) for k, f in enumerate(frames)]
     batch = ctx.batch_call(frames, ptr(dev_in), ptr(out_b))
+    k1080 = next(k for k, f in enumerate(frames) if f.desc.coded_width == 1920)
+    tuning_pf = tune(ctx_pf, frames[k1080:k1080 + 1], ptr(dev_in)[k1080:k1080 + 1], ptr(out_pf)[k1080:k1080 + 1],
+                     reps=3)
     tuning = tune(ctx, frames, ptr(dev_in), ptr(out_b), reps=4)
     for c in per_frame:
         c()
     batch()
+    ctx_pf.finish()
     ctx.finish()
     hip.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        for c in per_frame:
-            c()
-    ctx.finish()
-    hip.synchronize()
-    t_pf = (time.perf_counter() - t0) / reps
+    def time_per_frame():
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for c in per_frame:
+                c()
+        ctx_pf.finish()
+        hip.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    t_pf_own = time_per_frame()
+    ctx_pf.tuning = tuning
+    for c in per_frame:
+        c()
+    ctx_pf.finish()
+    t_pf_batch_tuning = time_per_frame()
+    t_pf = min(t_pf_own, t_pf_batch_tuning)
     t0 = time.perf_counter()
     for _ in range(reps):
         batch()
     ctx.finish()
     hip.synchronize()
     t_b = (time.perf_counter() - t0) / reps
+    # host time of one batched call with the staging ring free, and the
+    # batch's kernel time from dispatch events
+    host = []
+    for _ in range(5):
+        t = time.perf_counter()
+        batch()
+        host.append(time.perf_counter() - t)
+        ctx.finish()
+    ctx.profile(True)
+    for _ in range(reps):
+        batch()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    kernel_ms = {k: round(v[0] / reps, 4) for k, v in prof.items() if v[1]}
     exact = all(np.array_equal(a.download(), b.download())
                 for fa, fb in zip(out_pf, out_b) for a, b in zip(fa, fb))
     mpix = sum(f.output_luma_pixels for f in frames) / 1e6
     ctx.close()
+    ctx_pf.close()
     return {
         "workload": f"{n} frames: 360p 25% / 720p 25% / 1080p 35% / 2160p 15%, 8- or 10-bit 4:2:0, "
                     "superres on half (denominator 9..16), CDEF + LR (64x64 units)",
         "mpix_per_batch": round(mpix, 3),
         "per_frame_mpix_s": round(mpix / t_pf, 2),
+        "per_frame_ms_by_tuning": {"tuned_on_1080p": round(t_pf_own * 1e3, 3),
+                                   "tuned_on_the_batch": round(t_pf_batch_tuning * 1e3, 3)},
         "batched_mpix_s": round(mpix / t_b, 2),
         "batched_over_per_frame": round(t_pf / t_b, 3),
         "batched_ms_per_batch": round(t_b * 1e3, 3),
+        "batched_kernel_ms_per_batch": kernel_ms,
+        "batched_kernel_sum_ms": round(sum(v for k, v in kernel_ms.items() if k != "upload"), 3),
+        "batched_host_enqueue_ms": round(sorted(host)[len(host) // 2] * 1e3, 3),
         "bit_exact_vs_per_frame": bool(exact),
         "reps": reps,
+        "tuning_per_frame": tuning_pf,
         "tuning": tuning,
     }
 
@@ -411,6 +450,8 @@ def measure_4k10(av1pf, hip, workloads, reps: int, n: int = 16):
     return {
         "workload": f"{n} frames 3840x2160 10-bit 4:2:0, superres denominator 9..16, CDEF + LR (64x64 units)",
         "per_frame_mpix_s": round(mpix / t_pf, 2),
+        "per_frame_ms_by_tuning": {"tuned_on_1080p": round(t_pf_own * 1e3, 3),
+                                   "tuned_on_the_batch": round(t_pf_batch_tuning * 1e3, 3)},
         "per_frame_ms_per_frame": round(t_pf * 1e3 / n, 4),
         "batched_mpix_s": round(mpix / t_b, 2),
         "stage_hbm_frac_per_frame": round(total_bytes / t_pf / 1e9 / HBM_PEAK_GBS, 5),
@@ -469,11 +510,11 @@ def main():
                                            window_us=500)}), flush=True)
         return
     if args.mode == "uhd":
-        print(json.dumps({"mode": "uhd", **measure_4k10(av1pf, hip, workloads, reps=3)}), flush=True)
+        print(json.dumps({"mode": "uhd", **measure_4k10(av1pf, hip, workloads, reps=10)}), flush=True)
         return
     if args.mode == "batched":
         print(json.dumps({"mode": "batched",
-                          **measure_mixed_batch(av1pf, hip, workloads, reps=max(3, args.steps // 20))}),
+                          **measure_mixed_batch(av1pf, hip, workloads, reps=max(10, args.steps // 2))}),
               flush=True)
         return
 
@@ -483,9 +524,9 @@ def main():
               flush=True)
         return
     log("per-frame leg:", json.dumps({k: v for k, v in per_frame.items() if k[0] != "_"}))
-    uhd = measure_4k10(av1pf, hip, workloads, reps=3)
+    uhd = measure_4k10(av1pf, hip, workloads, reps=10)
     log("4K 10-bit leg:", json.dumps(uhd))
-    batched = measure_mixed_batch(av1pf, hip, workloads, reps=max(3, args.steps // 20))
+    batched = measure_mixed_batch(av1pf, hip, workloads, reps=max(10, args.steps // 2))
     log("batched leg:", json.dumps(batched))
     served = measure_served(av1pf, hip, workloads, args.serve_seconds)
     log("served leg:", json.dumps(served))

@@ -42,9 +42,13 @@ def mixed_run():
     out_b = [[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames]
     out_pf = [[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames]
     ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
+    # the batch with the variants the tuner picks for it, the per-frame calls
+    # with the defaults
+    ctx.tuning = {"superres_rows": 4, "lr_tile_rows": 32, "cdef_waves": 8}
     say("mixed batch: one batched call over 64 frames")
     ctx.batch_device(frames, ptr(dev_in), ptr(out_b))
     ctx.finish()
+    ctx.tuning = {"superres_rows": 8, "lr_tile_rows": 64, "cdef_waves": 16}
     say("mixed batch: 64 per-frame calls")
     for k, f in enumerate(frames):
         ctx.frame_device(f, ptr(dev_in)[k], ptr(out_pf)[k])

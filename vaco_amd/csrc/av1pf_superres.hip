@@ -2,179 +2,18 @@
 //
 // Reference semantics (matteopolak/vaco, crates/codec/vaco-codec-av1/src/
 // superres.rs): subpel_start 145-152, upscale_plane 163-216. One workgroup
-// per kSrSeg output samples of one row; the eight taps are read from the
-// padded source row (edge-clamped to the padded width).
-#include "av1pf_common.h"
+// per kSrSeg output samples of R rows (sr_block); the eight taps are read
+// from the padded source row (edge-clamped to the padded width).
+#include "av1pf_superres_body.h"
 
 namespace av1pf {
 
-// UPSCALE_FILTER, superres.rs:25-90 (AV1 spec Upscale_Filter[64][8]).
-__constant__ int16_t kUpscale[64][8] = {
-    {0, 0, 0, 128, 0, 0, 0, 0},       {0, 0, -1, 128, 2, -1, 0, 0},
-    {0, 1, -3, 127, 4, -2, 1, 0},     {0, 1, -4, 127, 6, -3, 1, 0},
-    {0, 2, -6, 126, 8, -3, 1, 0},     {0, 2, -7, 125, 11, -4, 1, 0},
-    {-1, 2, -8, 125, 13, -5, 2, 0},   {-1, 3, -9, 124, 15, -6, 2, 0},
-    {-1, 3, -10, 123, 18, -6, 2, -1}, {-1, 3, -11, 122, 20, -7, 3, -1},
-    {-1, 4, -12, 121, 22, -8, 3, -1}, {-1, 4, -13, 120, 25, -9, 3, -1},
-    {-1, 4, -14, 118, 28, -9, 3, -1}, {-1, 4, -15, 117, 30, -10, 4, -1},
-    {-1, 5, -16, 116, 32, -11, 4, -1}, {-1, 5, -16, 114, 35, -12, 4, -1},
-    {-1, 5, -17, 112, 38, -12, 4, -1}, {-1, 5, -18, 111, 40, -13, 5, -1},
-    {-1, 5, -18, 109, 43, -14, 5, -1}, {-1, 6, -19, 107, 45, -14, 5, -1},
-    {-1, 6, -19, 105, 48, -15, 5, -1}, {-1, 6, -19, 103, 51, -16, 5, -1},
-    {-1, 6, -20, 101, 53, -16, 6, -1}, {-1, 6, -20, 99, 56, -17, 6, -1},
-    {-1, 6, -20, 97, 58, -17, 6, -1}, {-1, 6, -20, 95, 61, -18, 6, -1},
-    {-2, 7, -20, 93, 64, -18, 6, -2}, {-2, 7, -20, 91, 66, -19, 6, -1},
-    {-2, 7, -20, 88, 69, -19, 6, -1}, {-2, 7, -20, 86, 71, -19, 6, -1},
-    {-2, 7, -20, 84, 74, -20, 7, -2}, {-2, 7, -20, 81, 76, -20, 7, -1},
-    {-2, 7, -20, 79, 79, -20, 7, -2}, {-1, 7, -20, 76, 81, -20, 7, -2},
-    {-2, 7, -20, 74, 84, -20, 7, -2}, {-1, 6, -19, 71, 86, -20, 7, -2},
-    {-1, 6, -19, 69, 88, -20, 7, -2}, {-1, 6, -19, 66, 91, -20, 7, -2},
-    {-2, 6, -18, 64, 93, -20, 7, -2}, {-1, 6, -18, 61, 95, -20, 6, -1},
-    {-1, 6, -17, 58, 97, -20, 6, -1}, {-1, 6, -17, 56, 99, -20, 6, -1},
-    {-1, 6, -16, 53, 101, -20, 6, -1}, {-1, 5, -16, 51, 103, -19, 6, -1},
-    {-1, 5, -15, 48, 105, -19, 6, -1}, {-1, 5, -14, 45, 107, -19, 6, -1},
-    {-1, 5, -14, 43, 109, -18, 5, -1}, {-1, 5, -13, 40, 111, -18, 5, -1},
-    {-1, 4, -12, 38, 112, -17, 5, -1}, {-1, 4, -12, 35, 114, -16, 5, -1},
-    {-1, 4, -11, 32, 116, -16, 5, -1}, {-1, 4, -10, 30, 117, -15, 4, -1},
-    {-1, 3, -9, 28, 118, -14, 4, -1}, {-1, 3, -9, 25, 120, -13, 4, -1},
-    {-1, 3, -8, 22, 121, -12, 4, -1}, {-1, 3, -7, 20, 122, -11, 3, -1},
-    {-1, 2, -6, 18, 123, -10, 3, -1}, {0, 2, -6, 15, 124, -9, 3, -1},
-    {0, 2, -5, 13, 125, -8, 2, -1},   {0, 1, -4, 11, 125, -7, 2, 0},
-    {0, 1, -3, 8, 126, -6, 2, 0},     {0, 1, -3, 6, 127, -4, 1, 0},
-    {0, 1, -2, 4, 127, -3, 1, 0},     {0, 0, -1, 2, 128, -1, 0, 0},
-};
-
-constexpr int kSrThreads = 256;
-constexpr int kSrPer = kSrSeg / kSrThreads;  // output columns per thread
-constexpr int kSrWin = kSrSeg + 16;          // staged source samples per row
-
-// One segment of kSrSeg output columns over up to R rows
-// (superres.rs:173-214). Positions, phases and filters depend on the column
-// only, so each thread derives them once for its columns and applies them to
-// every row. Each row's source window (the union of the segment's taps,
-// edge-clamped to the padded width) is staged in LDS; because the step is
-// below one source sample the window is exactly the samples the taps read,
-// so the sample-range flag is raised exactly when the reference would reject
-// a tap.
 template <int R>
 __global__ __launch_bounds__(kSrThreads) void k_superres(const SrJob *__restrict__ jobs, int n_jobs)
 {
-    // win: each row's window; wsh: the same shifted by one sample (window
-    // index i at i + 1), so the eight taps from any position are four
-    // aligned sample pairs of one of the two copies.
-    __shared__ __align__(16) int16_t tab[64 * 8];
-    __shared__ __align__(16) uint16_t win[R][kSrWin];
-    __shared__ __align__(16) uint16_t wsh[R][kSrWin + 2];
+    __shared__ __align__(16) uint8_t lds[sr_lds_bytes<R>()];
     const SrJob &j = find_item(jobs, n_jobs, blockIdx.x);
-    const uint32_t local = blockIdx.x - j.block_begin;
-    const uint32_t rb = local / j.segs;
-    const uint32_t seg = local - rb * j.segs;
-    const int r0 = (int)rb * R;
-    const int nr = min(R, (int)j.n_rows - r0);
-    const int tid = threadIdx.x;
-    const int x0 = (int)seg * kSrSeg;
-    const int xn = min(x0 + kSrSeg, (int)j.out_width) - 1;
-    const int lo = ((-(1 << 14) + j.start + x0 * j.step) >> 14) - 3;
-    const int hi = ((-(1 << 14) + j.start + xn * j.step) >> 14) + 4;
-    const int last = (int)j.in_width - 1;
-    // Window origin: lo, or lo rounded down to a multiple of 4 where the
-    // window lies inside the row with room for whole 4-sample groups, which
-    // are then loaded 8 bytes at a time. Only samples lo .. hi are checked
-    // against the bit depth (the taps read no others).
-    const int groups = (hi - (lo & ~3)) / 4 + 1;
-    const bool vec = j.vec_ok && lo >= 0 && (lo & ~3) + 4 * groups - 1 <= last;
-    const int org = vec ? (lo & ~3) : lo;
-    const int maxv = j.max_sample;
-    bool bad = false;
-    size_t orow[R];  // output offset of each row
-    uint32_t yrow[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        yrow[r] = j.rows ? glb(j.rows)[r0 + min(r, nr - 1)] : (uint32_t)(r0 + r);
-        orow[r] = (size_t)yrow[r] * j.out_stride;
-    }
-    if (vec) {
-        // At most two groups per thread and row; every load is issued before
-        // any is used.
-        uint64_t q[R][2];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            gptr<const uint64_t> row =
-                reinterpret_cast<gptr<const uint64_t>>(glb(j.in) + (size_t)yrow[r] * j.in_stride + org);
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const int g = tid + k * kSrThreads;
-                q[r][k] = r < nr && g < groups ? row[g] : 0;
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const int g = tid + k * kSrThreads;
-                if (r >= nr || g >= groups)
-                    continue;
-                const uint32_t lo32 = (uint32_t)q[r][k], hi32 = (uint32_t)(q[r][k] >> 32);
-                const int s0 = org + 4 * g;
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int v = (int)(((t < 2 ? lo32 : hi32) >> (16 * (t & 1))) & 0xffffu);
-                    bad |= s0 + t >= lo && s0 + t <= hi && v > maxv;
-                }
-                *reinterpret_cast<uint2 *>(&win[r][4 * g]) = make_uint2(lo32, hi32);
-                uint16_t *u = &wsh[r][4 * g + 1];  // window index i at i + 1
-                u[0] = (uint16_t)lo32;
-                *reinterpret_cast<uint32_t *>(u + 1) = (lo32 >> 16) | (hi32 << 16);
-                u[3] = (uint16_t)(hi32 >> 16);
-            }
-        }
-    } else {
-        for (int r = 0; r < nr; r++) {
-            gptr<const uint16_t> row = glb(j.in) + (size_t)yrow[r] * j.in_stride;
-            for (int i = tid; i <= hi - lo; i += kSrThreads) {
-                const int v = row[min(max(lo + i, 0), last)];
-                bad |= v > maxv;
-                win[r][i] = (uint16_t)v;
-                wsh[r][i + 1] = (uint16_t)v;
-            }
-        }
-    }
-    for (int i = tid; i < 64 * 8; i += kSrThreads)
-        tab[i] = kUpscale[i >> 3][i & 7];
-    if (__syncthreads_or(bad) && tid == 0)
-        raise_status(j.status, kFlagSampleRange);
-    gptr<uint16_t> out16 = glb(j.out);
-    gptr<uint8_t> out8 = reinterpret_cast<gptr<uint8_t>>(out16);
-    const bool u8 = j.out_u8 != 0;
-#pragma unroll
-    for (int k = 0; k < kSrPer; k++) {
-        const int x = x0 + tid + k * kSrThreads;
-        if (x > xn)
-            break;
-        const int src_x = -(1 << 14) + j.start + x * j.step;
-        const int base = (src_x >> 14) - 3 - org;
-        // Taps as pairs, (0,1) (2,3) (4,5) (6,7), for v_dot2 over sample pairs.
-        const uint4 fq = *reinterpret_cast<const uint4 *>(&tab[((src_x & 0x3fff) >> 8) * 8]);
-        const s16x2 f0 = __builtin_bit_cast(s16x2, fq.x), f1 = __builtin_bit_cast(s16x2, fq.y);
-        const s16x2 f2 = __builtin_bit_cast(s16x2, fq.z), f3 = __builtin_bit_cast(s16x2, fq.w);
-        const int odd = base & 1;
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            if (r >= nr)
-                break;
-            const uint32_t *w = reinterpret_cast<const uint32_t *>(odd ? &wsh[r][base + 1] : &win[r][base]);
-            int sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[0]), f0, 0, false);
-            sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[1]), f1, sum, false);
-            sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[2]), f2, sum, false);
-            sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[3]), f3, sum, false);
-            const int v = min(max((sum + 64) >> 7, 0), maxv);
-            const size_t o = orow[r] + x;
-            if (u8)
-                out8[o] = (uint8_t)v;
-            else
-                out16[o] = (uint16_t)v;
-        }
-    }
+    sr_block<R>(j, blockIdx.x - j.block_begin, lds);
 }
 
 // rows: rows per workgroup (4, 8 or 16; the jobs' block_begin counts use
