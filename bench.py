@@ -450,8 +450,6 @@ def measure_4k10(av1pf, hip, workloads, reps: int, n: int = 16):
     return {
         "workload": f"{n} frames 3840x2160 10-bit 4:2:0, superres denominator 9..16, CDEF + LR (64x64 units)",
         "per_frame_mpix_s": round(mpix / t_pf, 2),
-        "per_frame_ms_by_tuning": {"tuned_on_1080p": round(t_pf_own * 1e3, 3),
-                                   "tuned_on_the_batch": round(t_pf_batch_tuning * 1e3, 3)},
         "per_frame_ms_per_frame": round(t_pf * 1e3 / n, 4),
         "batched_mpix_s": round(mpix / t_b, 2),
         "stage_hbm_frac_per_frame": round(total_bytes / t_pf / 1e9 / HBM_PEAK_GBS, 5),

@@ -154,8 +154,8 @@ __device__ __forceinline__ uint32_t dir_cost(const uint16_t *p, int shift)
                 part += acc[b].x;
             if (b - OFF >= 0 && b - OFF < 15 && bin_count_rows(D, b, 4) != 0)
                 part += acc[b - OFF].y;
-            const int s = part - 128 * cnt;
-            cost += (uint32_t)(s * s) * (uint32_t)(840 / cnt);
+            const uint32_t s = (uint32_t)(part - 128 * cnt);  // s * s mod 2^32 is exact for in-range samples
+            cost += s * s * (uint32_t)(840 / cnt);
         }
     }
     return cost;
@@ -541,18 +541,22 @@ __device__ __forceinline__ bool stage_planes(const CdefFrame &f, uint16_t *L, ui
 }
 
 // Filter (or pass through) every sample of a staged superblock and store it.
+// blist holds the superblock's 64 blocks, the nact blocks to filter first:
+// entries are taken in list order, so a wave's blocks are all filtered or
+// all passed through except at one boundary, and no wave runs the filter
+// with lanes idle for a skipped neighbour.
 // Luma: each half-wave takes one 8x8 block (32 lanes = 8 rows x 4 pairs of
-// columns), blocks hw, hw + T / 32, ... for half-wave hw; all its lanes then read
-// their taps at one direction's offsets, which the luma row stride keeps in
-// distinct banks. Chroma: 16 lanes per block, pairs of U then V. BORDER: the
+// columns), entries hw, hw + T / 32, ... for half-wave hw; all its lanes then
+// read their taps at one direction's offsets, which the luma row stride keeps
+// in distinct banks. Chroma: 16 lanes per block, pairs of U then V. BORDER: the
 // superblock touches a plane edge (unavailable taps; partial superblocks
 // store only samples inside the crop; widths are even, so a pair is all in
 // or all out, except at an odd crop width where the second sample is
 // dropped). L, U, V: the planes' tiles.
 template <bool BORDER, int SX, int SY, int T>
 __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, const uint16_t *U,
-                                          const uint16_t *V, const Blk (*blk)[64], const uint8_t *bact,
-                                          bool any, int x0, int y0, int ix)
+                                          const uint16_t *V, const Blk (*blk)[64], const uint8_t *blist,
+                                          int nact, int x0, int y0, int ix)
 {
     constexpr int CROW = chroma_row<SX>();
     // Strengths are per superblock preset (the luma primary is only ever
@@ -568,8 +572,8 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
         const int ost = f.out_stride[0], W0 = f.crop_w[0], H0 = f.crop_h[0];
 #pragma unroll
         for (int pass = 0; pass < 64 / HW; pass++) {
-            const int b = hw + HW * pass, bi = b >> 3, bj = b & 7;
-            const bool act = any && bact[b];
+            const int e = hw + HW * pass, b = blist[e], bi = b >> 3, bj = b & 7;
+            const bool act = e < nact;
             const int ci = (2 + bi * 8 + i) * kRow + 8 + bj * 8 + j;
             s16x2 res;
             if (act) {
@@ -588,9 +592,9 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
     for (int cp = 0; cp < 64 / CB; cp++) {
         if (f.nplanes == 1)
             break;
-        const int b = (tid >> 4) + CB * cp, q = tid & 15;
+        const int e = (tid >> 4) + CB * cp, b = blist[e], q = tid & 15;
         const int bi = b >> 3, bj = b & 7;
-        const bool act = any && bact[b];
+        const bool act = e < nact;
         Blk B = {};
         if (act)
             B = blk[1][b];
@@ -619,8 +623,22 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
 // One 64x64 luma superblock of one frame, all planes (decode.rs:772-856).
 // SX, SY: chroma subsampling (monochrome frames use <0, 0> with 1 plane).
 // XCD: superblocks numbered XCD-contiguously (xcd_remap).
-template <int SX, int SY, bool XCD, int T>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cdef(const CdefFrame *__restrict__ frames, int n_frames)
+//
+// ARG: one frame, whose descriptor is the kernel argument `single` (kernel
+// arguments are read through the scalar cache); otherwise `frames` holds
+// n_frames descriptors.
+//
+// up_n16 > 0: the call's staged side information has not been copied to the
+// device; the CDEF side information the descriptor points to is read in
+// place from coherent host memory, and the launch copies the up_n16 16-byte
+// units at up_src (the rest of the call's descriptors, for the kernels after
+// it) to up_dst: workgroup b's first wave copies units b * 64 + lane,
+// b * 64 + lane + 64 * gridDim.x, ..., the first issued with the staging
+// loads and stored after them.
+template <int SX, int SY, bool XCD, int T, bool ARG>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cdef(
+    const CdefFrame *__restrict__ frames, int n_frames, const CdefFrame single, const uint4 *__restrict__ up_src,
+    uint4 *__restrict__ up_dst, uint32_t up_n16)
 {
     static_assert(T == 512 || T == 1024, "the direction search takes 512 threads");
     constexpr int CW = 64 >> SX, CH = 64 >> SY;
@@ -634,10 +652,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(8, 8))) void 
     __shared__ Blk blk[2][64];
     __shared__ uint32_t cost[8][64];
     __shared__ uint8_t bact[64];
+    __shared__ uint8_t blist[64];  // blocks to filter (ascending), then the rest
+    __shared__ int snact;
     __shared__ int sidx;
 
     const uint32_t wg = XCD ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-    const CdefFrame &f = find_item(frames, n_frames, wg);
+    const CdefFrame &f = ARG ? single : find_item(frames, n_frames, wg);
     const uint32_t sb = wg - f.block_begin;
     const int sby = (int)(sb / f.sb_cols), sbx = (int)(sb - (uint32_t)sby * f.sb_cols);
     const int tid = threadIdx.x;
@@ -651,19 +671,30 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(8, 8))) void 
     const bool border = x0 == 0 || y0 == 0 || x0 + 64 >= W0 || y0 + 64 >= H0;
 
     // The superblock's cdef_idx and the skip flags of its 64 blocks
-    // (decode.rs:774-777, 892-896), loaded with the samples and stored to LDS
-    // with them; kept per lane until then, so that no wait for them comes
-    // before the sample loads are issued.
-    // Every wave loads them (lane l: block l & 63), so that no branch
-    // carries the result. Blocks outside the plane read the last block's
-    // flag (padded sizes are multiples of 8) and count as skipped.
-    uint32_t skipv;  // nonzero: skipped or outside the plane
-    int vidx;
-    {
+    // (decode.rs:774-777, 892-896), loaded by the first wave (lane l: block
+    // l) before the samples and left untouched until side_info() turns them
+    // into the block list after staging: vector loads return in order, so
+    // any earlier use would wait for them (they may come from host memory)
+    // before the sample loads are issued. Blocks outside the plane read the
+    // last block's flag (padded sizes are multiples of 8) and count as
+    // skipped. The same wave issues its share of the staged-bytes copy. The
+    // defaults of the other waves pass through an empty asm so that the
+    // compiler cannot fold the later tests into this branch as constants.
+    const uint32_t up_i = blockIdx.x * 64u + (uint32_t)tid;
+    const bool up_first = tid < 64 && up_i < up_n16;
+    uint4 up_v;
+    if (up_first)
+        up_v = glb(up_src)[up_i];
+    uint32_t skip_raw = 1;  // nonzero: skipped
+    int vidx = -1;
+    bool outside = true;  // the lane's block lies outside the plane
+    asm volatile("" : "+v"(skip_raw), "+v"(vidx));
+    if (tid < 64) {
         const int bx = x0 + (tid & 7) * 8, by = y0 + ((tid >> 3) & 7) * 8;
         const int bxc = min(bx, W0 - 8), byc = min(by, H0 - 8);
+        outside = bx >= W0 || by >= H0;
         vidx = glb(f.cdef_idx)[sby * f.idx_stride + sbx];
-        skipv = glb(f.skip8)[(size_t)(byc >> 3) * f.skip_stride + (bxc >> 3)] | (bx < W0 && by < H0 ? 0u : 1u);
+        skip_raw = glb(f.skip8)[(size_t)(byc >> 3) * f.skip_stride + (bxc >> 3)];
     }
     bool bad = false;
     const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
@@ -686,23 +717,43 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(8, 8))) void 
         else
             bad = stage_planes<SX, SY, false, T>(f, L, U, V, nplanes, x0, y0, maxv);
     }
-    if (tid < 64) {
-        bact[tid] = vidx >= 0 && skipv == 0;
-        if (tid == 0)
-            sidx = vidx;
-    }
-    if (__syncthreads_or(bad) && tid == 0)
-        raise_status(f.status, kFlagSampleRange);
-    const int idx = sidx;
-
-    if (idx >= 0) {
-        // Direction search: wave w < 8 computes direction w, lane = block.
+    // Wave 0 stores its share of the staged-bytes copy and turns the loaded
+    // skip flags and cdef_idx into the block list.
+    auto side_info = [&]() {
+        if (tid < 64) {
+            if (up_first) {
+                gptr<uint4> dst = glb(up_dst);
+                dst[up_i] = up_v;
+                for (uint32_t i = up_i + 64u * gridDim.x; i < up_n16; i += 64u * gridDim.x)
+                    dst[i] = glb(up_src)[i];
+            }
+            const bool a = vidx >= 0 && skip_raw == 0 && !outside;
+            bact[tid] = a;
+            const uint64_t m = __ballot(a);
+            const int below = __popcll(m & ((1ull << tid) - 1)), nact = __popcll(m);
+            blist[a ? below : nact + (tid - below)] = (uint8_t)tid;
+            if (tid == 0) {
+                snact = nact;
+                sidx = vidx;
+            }
+        }
+    };
+    // Direction search: wave w < 8 computes direction w, lane = block.
+    auto search = [&]() {
         if (tid < 512) {
             const int b = tid & 63;
             if (bact[b])
                 cost[tid >> 6][b] =
                     dir_cost_any(tid >> 6, &L[(2 + (b >> 3) * 8) * kRow + 8 + (b & 7) * 8], shift);
         }
+    };
+    side_info();
+    if (__syncthreads_or(bad) && tid == 0)
+        raise_status(f.status, kFlagSampleRange);
+    const int idx = sidx;
+
+    if (idx >= 0) {
+        search();
         __syncthreads();
         if (tid < 128) {
             const int b = tid & 63, cls = tid >> 6;
@@ -748,11 +799,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(8, 8))) void 
         __syncthreads();
     }
 
-    const bool any = idx >= 0;
+    const int nact = snact;  // 0 when the superblock has no index
     if (border)
-        filter_sb<true, SX, SY, T>(sf, L, U, V, blk, bact, any, x0, y0, idx & 7);
+        filter_sb<true, SX, SY, T>(sf, L, U, V, blk, blist, nact, x0, y0, idx & 7);
     else
-        filter_sb<false, SX, SY, T>(sf, L, U, V, blk, bact, any, x0, y0, idx & 7);
+        filter_sb<false, SX, SY, T>(sf, L, U, V, blk, blist, nact, x0, y0, idx & 7);
 }
 
 // Kernel-level cases (cdef::find_direction + cdef::filter_block on one
@@ -835,38 +886,71 @@ __global__ __launch_bounds__(64) void k_cdef_cases(const uint16_t *__restrict__ 
     }
 }
 
-template <bool XCD, int T>
-static void launch_cdef_mode(const CdefFrame *frames, int n, uint32_t blocks, int mode,
-                             hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+template <bool XCD, int T, bool ARG>
+static void launch_cdef_mode(const CdefFrame *frames, int n, const CdefFrame &single, uint32_t blocks, int mode,
+                             const uint4 *up_src, uint4 *up_dst, uint32_t up_n16, hipStream_t s, hipEvent_t ev0,
+                             hipEvent_t ev1)
 {
     const dim3 g(blocks), b(T);
     switch (mode) {
-    case 0: hipExtLaunchKernelGGL((k_cdef<1, 1, XCD, T>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
-    case 1: hipExtLaunchKernelGGL((k_cdef<0, 0, XCD, T>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
-    case 2: hipExtLaunchKernelGGL((k_cdef<1, 0, XCD, T>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
-    default: hipExtLaunchKernelGGL((k_cdef<0, 1, XCD, T>), g, b, 0, s, ev0, ev1, 0, frames, n); break;
+    case 0:
+        hipExtLaunchKernelGGL((k_cdef<1, 1, XCD, T, ARG>), g, b, 0, s, ev0, ev1, 0, frames, n, single, up_src,
+                              up_dst, up_n16);
+        break;
+    case 1:
+        hipExtLaunchKernelGGL((k_cdef<0, 0, XCD, T, ARG>), g, b, 0, s, ev0, ev1, 0, frames, n, single, up_src,
+                              up_dst, up_n16);
+        break;
+    case 2:
+        hipExtLaunchKernelGGL((k_cdef<1, 0, XCD, T, ARG>), g, b, 0, s, ev0, ev1, 0, frames, n, single, up_src,
+                              up_dst, up_n16);
+        break;
+    default:
+        hipExtLaunchKernelGGL((k_cdef<0, 1, XCD, T, ARG>), g, b, 0, s, ev0, ev1, 0, frames, n, single, up_src,
+                              up_dst, up_n16);
+        break;
     }
+}
+
+template <int T, bool ARG>
+static void launch_cdef_t(const CdefFrame *frames, int n, const CdefFrame &single, uint32_t blocks, int mode,
+                          bool xcd, const uint4 *us, uint4 *ud, uint32_t n16, hipStream_t s, hipEvent_t ev0,
+                          hipEvent_t ev1)
+{
+    if (xcd)
+        launch_cdef_mode<true, T, ARG>(frames, n, single, blocks, mode, us, ud, n16, s, ev0, ev1);
+    else
+        launch_cdef_mode<false, T, ARG>(frames, n, single, blocks, mode, us, ud, n16, s, ev0, ev1);
 }
 
 // mode: 0 = 4:2:0, 1 = 4:4:4 or monochrome, 2 = 4:2:2, 3 = 4:4:0. xcd:
 // XCD-contiguous superblock order. waves: 16 or 8 waves (1024 or 512
 // threads) per superblock's workgroup. ev0/ev1 (may be null) receive the
 // dispatch's start and end.
-hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, bool xcd, int waves,
-                       hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+// single (may be null): the one frame's descriptor, passed by value in the
+// kernel arguments (frames and n are then unused). up_bytes > 0: see k_cdef
+// (up_src, up_dst 16-byte aligned).
+hipError_t launch_cdef(const CdefFrame *frames, int n, const CdefFrame *single, uint32_t blocks, int mode,
+                       bool xcd, int waves, const void *up_src, void *up_dst, size_t up_bytes, hipStream_t s,
+                       hipEvent_t ev0, hipEvent_t ev1)
 {
     if (blocks == 0)
         return hipSuccess;
+    const uint4 *us = static_cast<const uint4 *>(up_src);
+    uint4 *ud = static_cast<uint4 *>(up_dst);
+    const uint32_t n16 = (uint32_t)((up_bytes + 15) / 16);
+    const CdefFrame none{};
+    const CdefFrame &one = single ? *single : none;
     if (waves == 8) {
-        if (xcd)
-            launch_cdef_mode<true, 512>(frames, n, blocks, mode, s, ev0, ev1);
+        if (single)
+            launch_cdef_t<512, true>(frames, n, one, blocks, mode, xcd, us, ud, n16, s, ev0, ev1);
         else
-            launch_cdef_mode<false, 512>(frames, n, blocks, mode, s, ev0, ev1);
+            launch_cdef_t<512, false>(frames, n, one, blocks, mode, xcd, us, ud, n16, s, ev0, ev1);
     } else {
-        if (xcd)
-            launch_cdef_mode<true, 1024>(frames, n, blocks, mode, s, ev0, ev1);
+        if (single)
+            launch_cdef_t<1024, true>(frames, n, one, blocks, mode, xcd, us, ud, n16, s, ev0, ev1);
         else
-            launch_cdef_mode<false, 1024>(frames, n, blocks, mode, s, ev0, ev1);
+            launch_cdef_t<1024, false>(frames, n, one, blocks, mode, xcd, us, ud, n16, s, ev0, ev1);
     }
     return hipGetLastError();
 }

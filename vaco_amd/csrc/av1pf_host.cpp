@@ -28,8 +28,9 @@
 #include "av1pf_internal.h"
 
 namespace av1pf {
-hipError_t launch_cdef(const CdefFrame *frames, int n, uint32_t blocks, int mode, bool xcd, int waves,
-                       hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_cdef(const CdefFrame *frames, int n, const CdefFrame *single, uint32_t blocks, int mode,
+                       bool xcd, int waves, const void *up_src, void *up_dst, size_t up_bytes, hipStream_t s,
+                       hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, int rows, hipStream_t s,
                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_lr(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd, int rows,
@@ -907,12 +908,39 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
 
 // Upload the plan's descriptors with the arena (one copy) and launch each
 // kernel once over all of its items.
-static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Arena *a, hipStream_t s)
+//
+// A single frame whose chain starts with CDEF (the per-frame path) has no
+// separate upload: the CDEF launch reads its descriptor, cdef_idx and skip
+// flags in place from the slot's coherent host memory (a few hundred bytes
+// per superblock, read with the samples) and copies the staged bytes to the
+// device mirror for the kernels after it. That saves a launch and a host
+// round trip per frame.
+static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Arena *a, hipStream_t s,
+                                     bool single_frame = false)
 {
+    int cdef_modes = 0;
+    for (int m = 0; m < 4; m++)
+        cdef_modes += !plan.cdef[m].empty();
+    // (Not when the chain is replayed as a graph: a graph keeps the kernel
+    // arguments it was captured with, and this descriptor changes per frame.)
+    const bool graphs = ctx && ctx->use_graphs && !ctx->prof;
+    const bool fused_upload = single_frame && cdef_modes == 1 && !a->pageable && a->hd && !graphs;
+    auto host_view = [&](const void *dev) -> const void * {
+        return a->hd + (static_cast<const uint8_t *>(dev) - a->d);
+    };
     const CdefFrame *dc[4] = {};
+    CdefFrame single;  // the frame's descriptor, passed as a kernel argument
+    memset(&single, 0, sizeof single);
     for (int m = 0; m < 4; m++) {
         if (plan.cdef[m].empty())
             continue;
+        if (fused_upload) {
+            single = plan.cdef[m][0];
+            single.cdef_idx = static_cast<const int8_t *>(host_view(single.cdef_idx));
+            single.skip8 = static_cast<const uint8_t *>(host_view(single.skip8));
+            dc[m] = &single;  // marks the mode; the kernel takes `single`
+            continue;
+        }
         TAKE(h, d, CdefFrame, plan.cdef[m].size());
         memcpy(h, plan.cdef[m].data(), plan.cdef[m].size() * sizeof(CdefFrame));
         dc[m] = d;
@@ -941,9 +969,12 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
         }
         dlist = dlk;
     }
-    vaco_av1pf_status st = upload_slot(ctx, a, s);
-    if (st != VACO_AV1PF_OK)
-        return st;
+    if (!fused_upload) {
+        const vaco_av1pf_status st = upload_slot(ctx, a, s);
+        if (st != VACO_AV1PF_OK)
+            return st;
+    }
+    const size_t up_bytes = fused_upload ? a->used : 0;
     for (const auto &c : plan.copies) {
         if (c.dst.u8)
             HIPCHECK(launch_to_bytes(c.src.data, c.src.stride, (uint8_t *)c.dst.data, c.dst.stride,
@@ -959,9 +990,11 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
                 HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CDEF, plan.cdef_bytes[m], s,
                                       [&](hipEvent_t e0, hipEvent_t e1) {
                                           return launch_cdef(dc[m], (int)plan.cdef[m].size(),
+                                                             fused_upload ? &single : nullptr,
                                                              plan.cdef_blocks[m], m,
                                                              plan.tune.cdef_xcd_order != 0,
-                                                             plan.tune.cdef_waves, s, e0, e1);
+                                                             plan.tune.cdef_waves, a->hd, a->d, up_bytes,
+                                                             s, e0, e1);
                                       }));
         if (dj)
             HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_SUPERRES,
@@ -1039,7 +1072,7 @@ static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_fra
     vaco_av1pf_status st = plan_frame(ctx, d, g, in, out, a, status, ws, plan);
     if (st != VACO_AV1PF_OK)
         return st;
-    return launch_plan(ctx, plan, a, s);
+    return launch_plan(ctx, plan, a, s, true);
 }
 
 // ---------------------------------------------------------- profiling --
