@@ -60,11 +60,13 @@ def test_gpu_frame_output_matches_pic_to_frame(ctx, case):
         assert np.array_equal(got, w), (case, p, int((got != w).sum()))
 
 
-def test_gpu_frame_output_of_the_1080p_workload_is_the_u8_picture(ctx):
+@pytest.mark.parametrize("workload,frames", [("cfg0", 2), ("cfg1", 1)],
+                         ids=["352x288-8bit", "1920x1080-8bit"])
+def test_gpu_frame_output_of_the_8bit_workloads_is_the_u8_picture(ctx, workload, frames):
     from vaco_amd import hip
-    say("frame output: cfg0 frames")
-    for k in range(2):
-        f = av1pf.generate(workloads.cfg0(), k)
+    say(f"frame output: {workload} frames")
+    for k in range(frames):
+        f = av1pf.generate(getattr(workloads, workload)(), k)
         geo, bps = av1pf.frame_layout(f)
         assert bps == 1
         want = [p.astype(np.uint8) for p in O.frame(f.desc, f.planes)]

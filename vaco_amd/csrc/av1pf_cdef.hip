@@ -517,10 +517,11 @@ struct SbFields {
 
 // Stage the superblock's windows of every plane (luma at x0-2, y0-2, 68x68;
 // chroma likewise subsampled) from 8-byte aligned planes: all loads first,
-// then all LDS stores. Returns whether a sample above maxv was read.
-template <int SX, int SY, bool BORDER, int T>
+// then `between()`, then all LDS stores. Returns whether a sample above maxv
+// was read.
+template <int SX, int SY, bool BORDER, int T, class Between>
 __device__ __forceinline__ bool stage_planes(const CdefFrame &f, uint16_t *L, uint16_t *U, uint16_t *V,
-                                             int nplanes, int x0, int y0, int maxv)
+                                             int nplanes, int x0, int y0, int maxv, Between &&between)
 {
     constexpr int CW = 64 >> SX, CH = 64 >> SY;
     constexpr int CROW = chroma_row<SX>(), CTILE = chroma_tile<SX, SY>();
@@ -532,6 +533,7 @@ __device__ __forceinline__ bool stage_planes(const CdefFrame &f, uint16_t *L, ui
         u.load(glb(f.in[1]), (int)f.width[1], (int)f.height[1], (int)f.in_stride[1], cx, cy);
         v.load(glb(f.in[2]), (int)f.width[2], (int)f.height[2], (int)f.in_stride[2], cx, cy);
     }
+    between();
     bool bad = l.template store<kRow, kTile>(L, maxv);
     if (nplanes > 1) {
         bad |= u.template store<CROW, CTILE>(U, maxv);
@@ -626,15 +628,15 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
 //
 // ARG: one frame, whose descriptor is the kernel argument `single` (kernel
 // arguments are read through the scalar cache); otherwise `frames` holds
-// n_frames descriptors.
+// n_frames descriptors and up_* are unused.
 //
-// up_n16 > 0: the call's staged side information has not been copied to the
-// device; the CDEF side information the descriptor points to is read in
-// place from coherent host memory, and the launch copies the up_n16 16-byte
-// units at up_src (the rest of the call's descriptors, for the kernels after
-// it) to up_dst: workgroup b's first wave copies units b * 64 + lane,
-// b * 64 + lane + 64 * gridDim.x, ..., the first issued with the staging
-// loads and stored after them.
+// ARG (up_n16 > 0): the call's staged side information has not been copied
+// to the device; the CDEF side information the descriptor points to is read
+// in place from coherent host memory, and the launch copies the up_n16
+// 16-byte units at up_src (the rest of the call's descriptors, for the
+// kernels after it) to up_dst: workgroup b's first wave copies units
+// b * 64 + lane, b * 64 + lane + 64 * gridDim.x, ..., the first issued with
+// the staging loads and stored after them.
 template <int SX, int SY, bool XCD, int T, bool ARG>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cdef(
     const CdefFrame *__restrict__ frames, int n_frames, const CdefFrame single, const uint4 *__restrict__ up_src,
@@ -672,33 +674,42 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(8, 8))) void 
 
     // The superblock's cdef_idx and the skip flags of its 64 blocks
     // (decode.rs:774-777, 892-896), loaded by the first wave (lane l: block
-    // l) before the samples and left untouched until side_info() turns them
-    // into the block list after staging: vector loads return in order, so
-    // any earlier use would wait for them (they may come from host memory)
-    // before the sample loads are issued. Blocks outside the plane read the
-    // last block's flag (padded sizes are multiples of 8) and count as
-    // skipped. The same wave issues its share of the staged-bytes copy. The
-    // defaults of the other waves pass through an empty asm so that the
-    // compiler cannot fold the later tests into this branch as constants.
+    // l) between issuing its sample loads and waiting for them, and left
+    // untouched until side_info() turns them into the block list after
+    // staging: vector loads complete in order, so loads issued before the
+    // samples, or used before the samples are stored, would hold up the
+    // staging by their latency (they may come from host memory). Every wave
+    // issues the same loads with no branch around them, so that the waits for
+    // the samples count them exactly; the other waves read the luma plane's
+    // first byte instead. Blocks outside the plane read the last block's flag
+    // (padded sizes are multiples of 8) and count as skipped. On the ARG
+    // path the first wave also loads its share of the staged-bytes copy
+    // (lanes past the end re-read unit 0).
     const uint32_t up_i = blockIdx.x * 64u + (uint32_t)tid;
-    const bool up_first = tid < 64 && up_i < up_n16;
+    const bool up_first = ARG && tid < 64 && up_i < up_n16;
     uint4 up_v;
-    if (up_first)
-        up_v = glb(up_src)[up_i];
-    uint32_t skip_raw = 1;  // nonzero: skipped
-    int vidx = -1;
-    bool outside = true;  // the lane's block lies outside the plane
-    asm volatile("" : "+v"(skip_raw), "+v"(vidx));
-    if (tid < 64) {
+    uint32_t skip_raw;  // nonzero: skipped
+    int vidx;
+    bool outside;  // the lane's block lies outside the plane
+    auto load_side_info = [&]() {
+        const bool w0 = tid < 64;
         const int bx = x0 + (tid & 7) * 8, by = y0 + ((tid >> 3) & 7) * 8;
         const int bxc = min(bx, W0 - 8), byc = min(by, H0 - 8);
         outside = bx >= W0 || by >= H0;
-        vidx = glb(f.cdef_idx)[sby * f.idx_stride + sbx];
-        skip_raw = glb(f.skip8)[(size_t)(byc >> 3) * f.skip_stride + (bxc >> 3)];
-    }
+        const gptr<const uint8_t> any = reinterpret_cast<gptr<const uint8_t>>(glb(f.in[0]));
+        const gptr<const int8_t> ia =
+            w0 ? glb(f.cdef_idx) + sby * f.idx_stride + sbx : reinterpret_cast<gptr<const int8_t>>(any);
+        const gptr<const uint8_t> sa =
+            w0 ? glb(f.skip8) + (size_t)(byc >> 3) * f.skip_stride + (bxc >> 3) : any;
+        if (ARG)
+            up_v = glb(up_src)[up_first ? up_i : 0u];
+        vidx = *ia;
+        skip_raw = *sa;
+    };
     bool bad = false;
     const int cx = (x0 >> SX) - 2, cy = (y0 >> SY) - 2;
     if (!fb.vec_ok()) {
+        load_side_info();
         bad = stage<68, 68, kRow, kTile, T>(L, glb(f.in[0]), W0, H0, f.in_stride[0], x0 - 2, y0 - 2, border,
                                          maxv);
         if (nplanes > 1) {
@@ -713,15 +724,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(8, 8))) void 
         // the superblock does not touch the last 8 rows or columns (likewise
         // for chroma), so no bounds are checked.
         if (border)
-            bad = stage_planes<SX, SY, true, T>(f, L, U, V, nplanes, x0, y0, maxv);
+            bad = stage_planes<SX, SY, true, T>(f, L, U, V, nplanes, x0, y0, maxv, load_side_info);
         else
-            bad = stage_planes<SX, SY, false, T>(f, L, U, V, nplanes, x0, y0, maxv);
+            bad = stage_planes<SX, SY, false, T>(f, L, U, V, nplanes, x0, y0, maxv, load_side_info);
     }
     // Wave 0 stores its share of the staged-bytes copy and turns the loaded
     // skip flags and cdef_idx into the block list.
     auto side_info = [&]() {
         if (tid < 64) {
-            if (up_first) {
+            if (ARG && up_first) {
                 gptr<uint4> dst = glb(up_dst);
                 dst[up_i] = up_v;
                 for (uint32_t i = up_i + 64u * gridDim.x; i < up_n16; i += 64u * gridDim.x)
@@ -928,8 +939,9 @@ static void launch_cdef_t(const CdefFrame *frames, int n, const CdefFrame &singl
 // threads) per superblock's workgroup. ev0/ev1 (may be null) receive the
 // dispatch's start and end.
 // single (may be null): the one frame's descriptor, passed by value in the
-// kernel arguments (frames and n are then unused). up_bytes > 0: see k_cdef
-// (up_src, up_dst 16-byte aligned).
+// kernel arguments (frames and n are then unused); up_bytes must then be
+// positive: see k_cdef (up_src, up_dst 16-byte aligned). Without single, up_*
+// are unused.
 hipError_t launch_cdef(const CdefFrame *frames, int n, const CdefFrame *single, uint32_t blocks, int mode,
                        bool xcd, int waves, const void *up_src, void *up_dst, size_t up_bytes, hipStream_t s,
                        hipEvent_t ev0, hipEvent_t ev1)
@@ -939,6 +951,8 @@ hipError_t launch_cdef(const CdefFrame *frames, int n, const CdefFrame *single, 
     const uint4 *us = static_cast<const uint4 *>(up_src);
     uint4 *ud = static_cast<uint4 *>(up_dst);
     const uint32_t n16 = (uint32_t)((up_bytes + 15) / 16);
+    if (single && n16 == 0)
+        return hipErrorInvalidValue;  // the single-frame kernels copy the staged bytes
     const CdefFrame none{};
     const CdefFrame &one = single ? *single : none;
     if (waves == 8) {
