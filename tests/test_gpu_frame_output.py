@@ -62,9 +62,15 @@ def test_gpu_frame_output_matches_pic_to_frame(ctx, case):
 
 @pytest.mark.parametrize("workload,frames", [("cfg0", 2), ("cfg1", 1)],
                          ids=["352x288-8bit", "1920x1080-8bit"])
-def test_gpu_frame_output_of_the_8bit_workloads_is_the_u8_picture(ctx, workload, frames):
+def test_gpu_frame_output_of_the_8bit_workloads_is_the_u8_picture(workload, frames):
     from vaco_amd import hip
     say(f"frame output: {workload} frames")
+    cfg = getattr(workloads, workload)()
+    with av1pf.Context(int(cfg.width), (int(cfg.height) + 63) // 64 * 64) as ctx:
+        check_u8_frames(ctx, workload, frames, hip)
+
+
+def check_u8_frames(ctx, workload, frames, hip):
     for k in range(frames):
         f = av1pf.generate(getattr(workloads, workload)(), k)
         geo, bps = av1pf.frame_layout(f)

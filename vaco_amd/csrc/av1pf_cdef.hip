@@ -437,18 +437,50 @@ struct VecStage {
 // Store a pair of output samples at element index i of a plane of u16
 // samples, or of bytes when u8 (8-bit frame output); the second sample only
 // when `second` (a crop to an odd width can end between the two).
-__device__ __forceinline__ void store_pair(gptr<uint16_t> plane, size_t i, s16x2 v, bool u8, bool second)
+__device__ __forceinline__ void store_pair(gptr<uint16_t> plane, size_t i, s16x2 v, bool u8, bool nt,
+                                           bool second)
 {
+    // Pairs at 4-byte (2-byte for bytes) aligned addresses take one store.
+    // nt: non-temporal stores, for an output that no launch reads from the
+    // L2 right after: lines left dirty in the L2 would be written back by the
+    // release at the launch's end, which the next launch waits for.
+    if (!nt) {
+        if (u8) {
+            gptr<uint8_t> o = reinterpret_cast<gptr<uint8_t>>(plane) + i;
+            o[0] = (uint8_t)v.x;
+            if (second)
+                o[1] = (uint8_t)v.y;
+        } else {
+            gptr<uint16_t> o = plane + i;
+            if (second && ((uintptr_t)o & 3) == 0) {
+                *reinterpret_cast<gptr<uint32_t>>(o) = (uint32_t)(uint16_t)v.x | ((uint32_t)(uint16_t)v.y << 16);
+            } else {
+                o[0] = (uint16_t)v.x;
+                if (second)
+                    o[1] = (uint16_t)v.y;
+            }
+        }
+        return;
+    }
     if (u8) {
         gptr<uint8_t> o = reinterpret_cast<gptr<uint8_t>>(plane) + i;
-        o[0] = (uint8_t)v.x;
-        if (second)
-            o[1] = (uint8_t)v.y;
+        if (second && ((uintptr_t)o & 1) == 0) {
+            __builtin_nontemporal_store((uint16_t)((v.x & 255) | (v.y << 8)), reinterpret_cast<gptr<uint16_t>>(o));
+        } else {
+            __builtin_nontemporal_store((uint8_t)v.x, o);
+            if (second)
+                __builtin_nontemporal_store((uint8_t)v.y, o + 1);
+        }
     } else {
         gptr<uint16_t> o = plane + i;
-        o[0] = (uint16_t)v.x;
-        if (second)
-            o[1] = (uint16_t)v.y;
+        if (second && ((uintptr_t)o & 3) == 0) {
+            __builtin_nontemporal_store((uint32_t)(uint16_t)v.x | ((uint32_t)(uint16_t)v.y << 16),
+                                        reinterpret_cast<gptr<uint32_t>>(o));
+        } else {
+            __builtin_nontemporal_store((uint16_t)v.x, o);
+            if (second)
+                __builtin_nontemporal_store((uint16_t)v.y, o + 1);
+        }
     }
 }
 
@@ -464,7 +496,8 @@ struct FrameBytes {
                       offsetof(CdefFrame, nplanes) == offsetof(CdefFrame, bit_depth) + 3 &&
                       offsetof(CdefFrame, damping) == offsetof(CdefFrame, bit_depth) + 4 &&
                       offsetof(CdefFrame, vec_ok) == offsetof(CdefFrame, bit_depth) + 5 &&
-                      offsetof(CdefFrame, out_u8) == offsetof(CdefFrame, bit_depth) + 6,
+                      offsetof(CdefFrame, out_u8) == offsetof(CdefFrame, bit_depth) + 6 &&
+                      offsetof(CdefFrame, out_nt) == offsetof(CdefFrame, bit_depth) + 7,
                       "CdefFrame byte fields are read as two words");
         __builtin_memcpy(&w0, &f.bit_depth, 4);
         __builtin_memcpy(&w1, &f.damping, 4);
@@ -474,6 +507,7 @@ struct FrameBytes {
     __device__ __forceinline__ int damping() const { return (int)(w1 & 255u); }
     __device__ __forceinline__ bool vec_ok() const { return ((w1 >> 8) & 255u) != 0; }
     __device__ __forceinline__ bool out_u8() const { return ((w1 >> 16) & 255u) != 0; }
+    __device__ __forceinline__ bool out_nt() const { return (w1 >> 24) != 0; }
 };
 
 // The descriptor fields the phases after staging use, read at the start of
@@ -484,7 +518,7 @@ struct SbFields {
     int crop_w[3], crop_h[3];
     uint32_t str[8];  // y_pri, y_sec, uv_pri, uv_sec: 8 presets each, 4 per word
     int damping, nplanes;
-    bool out_u8;
+    bool out_u8, out_nt;
 
     __device__ __forceinline__ void load(const CdefFrame &f, const FrameBytes &fb)
     {
@@ -505,6 +539,7 @@ struct SbFields {
         damping = fb.damping();
         nplanes = fb.nplanes();
         out_u8 = fb.out_u8();
+        out_nt = fb.out_nt();
     }
 
     // Preset ix of table t (0 y_pri, 1 y_sec, 2 uv_pri, 3 uv_sec).
@@ -586,7 +621,7 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
             }
             const int x = x0 + bj * 8 + j, y = y0 + bi * 8 + i;
             if (!BORDER || (x < W0 && y < H0))
-                store_pair(dst, (size_t)y * ost + x, res, f.out_u8, x + 1 < W0);
+                store_pair(dst, (size_t)y * ost + x, res, f.out_u8, f.out_nt, x + 1 < W0);
         }
     }
     constexpr int CB = T / 16;  // chroma blocks per pass
@@ -616,7 +651,7 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
                 res = ld2(t, ci);
             if (!BORDER || (x < W1 && y < H1))
                 store_pair(p == 1 ? f.out[1] : f.out[2],
-                           (size_t)y * (p == 1 ? f.out_stride[1] : f.out_stride[2]) + x, res, f.out_u8,
+                           (size_t)y * (p == 1 ? f.out_stride[1] : f.out_stride[2]) + x, res, f.out_u8, f.out_nt,
                            x + 1 < W1);
         }
     }

@@ -44,6 +44,7 @@ struct CdefFrame {
     uint8_t damping;                   // CdefDamping (includes +3)
     uint8_t vec_ok;                    // inputs 8-byte aligned, strides multiples of 4
     uint8_t out_u8;                    // out[] holds one byte per sample (8-bit frame output)
+    uint8_t out_nt;                    // store out[] non-temporally (no superres reads it next)
     uint32_t crop_w[3], crop_h[3];     // stores limited to this region of each output plane
     uint8_t y_pri[8], y_sec[8], uv_pri[8], uv_sec[8];
 };

@@ -729,6 +729,10 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             }
         }
         f.out_u8 = cdef_out[0].u8;
+        // Superres reads the CDEF output from the L2 right after; restoration
+        // and the caller gain from non-temporal stores (nothing dirty is left
+        // for the release at the launch's end to write back).
+        f.out_nt = !g.superres;
         f.cdef_idx = didx;
         f.skip8 = dskip;
         f.idx_stride = d.cdef_stride;

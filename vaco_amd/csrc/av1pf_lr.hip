@@ -98,14 +98,17 @@ __device__ __forceinline__ TileOut tile_out(const Tile &t, int x0, int y0)
                    t.out_u8()};
 }
 
-// Store output sample v at row r, column c of the tile.
+// Store output sample v at row r, column c of the tile. Outputs are stored
+// non-temporally: nothing in the launch reads them, and lines left dirty in
+// the L2 would be written back by the release at the launch's end, which
+// the next launch waits for.
 __device__ __forceinline__ void put(const TileOut &o, int r, int c, int v)
 {
     const uint32_t i = __umul24((uint32_t)r, o.stride) + (uint32_t)c;
     if (o.u8)
-        o.base8[i] = (uint8_t)v;
+        __builtin_nontemporal_store((uint8_t)v, o.base8 + i);
     else
-        o.base16[i] = (uint16_t)v;
+        __builtin_nontemporal_store((uint16_t)v, o.base16 + i);
 }
 
 // Stage Source::get (restoration.rs:233-244) over rows y0-3 .. y0+th+2 and
@@ -246,10 +249,11 @@ __device__ __forceinline__ void lr_copy(const Tile &t)
             bad |= (int)m > maxv;
             const uint32_t i = __umul24((uint32_t)r, o.stride) + (uint32_t)c;
             if (o.u8)  // samples <= 255 when none is out of range
-                *reinterpret_cast<gptr<uint32_t>>(o.base8 + i) =
-                    (lo & 0xffu) | ((lo >> 8) & 0xff00u) | ((hi & 0xffu) << 16) | ((hi >> 16) << 24);
+                __builtin_nontemporal_store(
+                    (lo & 0xffu) | ((lo >> 8) & 0xff00u) | ((hi & 0xffu) << 16) | ((hi >> 16) << 24),
+                    reinterpret_cast<gptr<uint32_t>>(o.base8 + i));
             else
-                *reinterpret_cast<gptr<uint64_t>>(o.base16 + i) = w;
+                __builtin_nontemporal_store(w, reinterpret_cast<gptr<uint64_t>>(o.base16 + i));
         }
         if (__syncthreads_or(bad) && threadIdx.x == 0)
             raise_status(P.status, kFlagSampleRange);
