@@ -584,6 +584,41 @@ void lr_cut_rows(const std::vector<LrRowSeg> &segs, int rows, std::vector<LrRowS
         }
 }
 
+// k_lr's tile list: self-guided tiles, then Wiener, then copies (by_kind), or
+// the one plane-order list (remapped in the kernel, xcd_remap), written to
+// `out`. In a by-kind list, each kind's tiles (in plane order) are dealt to
+// the XCDs round-robin in chunks of kLrChunk consecutive tiles (workgroup i
+// runs on XCD i % 8, each XCD with its own L2): neighbours in a chunk share
+// halo lines in one L2, while every XCD still gets a share of each kind's
+// long tiles.
+constexpr size_t kLrChunk = 4;
+static void lr_tile_list(const std::vector<uint64_t> (&tiles)[3], bool by_kind, uint64_t *out)
+{
+    size_t a = 0;
+    for (int k = 2; k >= 0; k--) {
+        const std::vector<uint64_t> &t = tiles[k];
+        const size_t n = t.size(), b = a + n;
+        if (!by_kind) {
+            std::copy(t.begin(), t.end(), out + a);
+        } else {
+            // the next free position of each XCD within [a, b)
+            size_t next[8];
+            for (size_t x = 0; x < 8; x++)
+                next[x] = a + (x + 8 - a % 8) % 8;
+            size_t x = 0;
+            for (size_t i = 0; i < n; i++) {
+                if (i % kLrChunk == 0 && i)
+                    x = (x + 1) % 8;
+                while (next[x] >= b)  // this XCD's positions are used up
+                    x = (x + 1) % 8;
+                out[next[x]] = t[i];
+                next[x] += 8;
+            }
+        }
+        a = b;
+    }
+}
+
 // Pre-CDEF rows that Source::get can read at stripe borders
 // (restoration.rs:236-241), for the upscaled pre-CDEF image.
 void lr_border_rows(const std::vector<LrRowSeg> &rows, uint32_t height, std::vector<uint32_t> &out)
@@ -965,12 +1000,8 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
         size_t n = 0;
         for (int k = 0; k < 3; k++)
             n += nlist[k] = (uint32_t)plan.lr_tiles[k].size();
-        // self-guided tiles first, then Wiener, then copies (k_lr's order)
         TAKE(hl, dlk, uint64_t, n);
-        for (int k = 2; k >= 0; k--) {
-            memcpy(hl, plan.lr_tiles[k].data(), nlist[k] * sizeof(uint64_t));
-            hl += nlist[k];
-        }
+        lr_tile_list(plan.lr_tiles, plan.tune.lr_order == 0, hl);
         dlist = dlk;
     }
     if (!fused_upload) {
@@ -2100,14 +2131,13 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
     p8 += (cols.size() * sizeof(LrColSeg) + 255) & ~size_t(255);
     LrUnit *du = (LrUnit *)p8;
     p8 += (n_units * sizeof(LrUnit) + 255) & ~size_t(255);
-    // one list for k_lr: self-guided tiles, then Wiener, then copies
+    // one list for k_lr (lr_tile_list)
     uint64_t *dlist = (uint64_t *)p8;
     uint32_t nlist[3];
-    std::vector<uint64_t> list;
     for (int k = 0; k < 3; k++)
         nlist[k] = (uint32_t)tiles[k].size();
-    for (int k = 2; k >= 0; k--)
-        list.insert(list.end(), tiles[k].begin(), tiles[k].end());
+    std::vector<uint64_t> list(ntiles);
+    lr_tile_list(tiles, tune.lr_order == 0, list.data());
     p8 += (ntiles * 8 + 255) & ~size_t(255);
     uint32_t *dst = (uint32_t *)p8;
     LrPlane L;

@@ -98,6 +98,19 @@ __device__ __forceinline__ TileOut tile_out(const Tile &t, int x0, int y0)
                    t.out_u8()};
 }
 
+// Store output samples v0, v1 at row r, columns c (even) and c + 1 of the
+// tile as one store; the output must be 8-byte aligned with a row stride that
+// is a multiple of 4 (Tile::out_vec), so the pair is aligned.
+__device__ __forceinline__ void put2(const TileOut &o, int r, int c, int v0, int v1)
+{
+    const uint32_t i = __umul24((uint32_t)r, o.stride) + (uint32_t)c;
+    if (o.u8)
+        __builtin_nontemporal_store((uint16_t)(v0 | (v1 << 8)), reinterpret_cast<gptr<uint16_t>>(o.base8 + i));
+    else
+        __builtin_nontemporal_store((uint32_t)v0 | ((uint32_t)v1 << 16),
+                                    reinterpret_cast<gptr<uint32_t>>(o.base16 + i));
+}
+
 // Store output sample v at row r, column c of the tile. Outputs are stored
 // non-temporally: nothing in the launch reads them, and lines left dirty in
 // the L2 would be written back by the release at the launch's end, which
@@ -338,6 +351,7 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
     {
         const int cb = tid & 31, rb = tid >> 5;
         const int c = 2 * cb, r0 = 4 * rb;
+        const bool pair = t.out_vec();
         int lo[10], hi[10];
 #pragma unroll
         for (int k = 0; k < 10; k++) {
@@ -357,9 +371,15 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
                          v2 * (hi[j + 2] + hi[j + 4]) + v3 * hi[j + 3];
                 s0 = min(max(round2i(s0, 14 - round_h), 0), maxv);
                 s1 = min(max(round2i(s1, 14 - round_h), 0), maxv);
-                put(o, r, c, s0);
-                if (c + 1 < tw)
-                    put(o, r, c + 1, s1);
+                // one store per pair: a store instruction then writes whole
+                // row segments, not every other sample
+                if (pair && c + 1 < tw) {
+                    put2(o, r, c, s0, s1);
+                } else {
+                    put(o, r, c, s0);
+                    if (c + 1 < tw)
+                        put(o, r, c + 1, s1);
+                }
             }
         }
     }
