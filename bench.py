@@ -281,11 +281,15 @@ def read_traffic(kernel, tuning):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE / WRITE_SIZE, scaled by the copy calibration), for the
     launch variant `tuning` selects (work items numbered XCD-contiguously or
-    not; for CDEF also the workgroup size), or None."""
+    not; then the CDEF workgroup size or the restoration tile height), or
+    None."""
     path = os.path.join(ROOT, "profiles", "pmc_hbm_traffic_1080p_per_frame.md")
     xcd = {"cdef": tuning.get("cdef_xcd_order"), "restoration": tuning.get("lr_order")}.get(kernel)
     order = "xcd" if xcd else "plain"
-    key = f"{order}-{64 * tuning.get('cdef_waves', 16)}" if kernel == "cdef" else order
+    if kernel == "cdef":
+        key = f"{order}-{64 * tuning.get('cdef_waves', 16)}"
+    else:
+        key = f"{order}-{tuning.get('lr_tile_rows', 64)}"
     try:
         for line in open(path):
             if line.startswith("traffic-json:"):

@@ -428,7 +428,9 @@ vaco_av1pf_status vaco_av1pf_get_tuning(const vaco_av1pf_ctx *ctx, vaco_av1pf_tu
 /* INVALID_ARGUMENT for values outside the ranges above. Applies to calls
  * made after it returns. */
 vaco_av1pf_status vaco_av1pf_set_tuning(vaco_av1pf_ctx *ctx, const vaco_av1pf_tuning *tuning);
-/* Runs `n` frames (device planes, as vaco_av1pf_batch_device) `reps` times
+/* Runs `n` frames (device planes; one frame as vaco_av1pf_frame_device,
+ * several as one vaco_av1pf_batch_device call, so that each path is tuned on
+ * its own kernels) `reps` times
  * with each variant of each kernel (after one untimed run), times the
  * kernels by their dispatch events (the fastest of the `reps` runs counts),
  * keeps the fastest variant per kernel as the context's tuning and returns

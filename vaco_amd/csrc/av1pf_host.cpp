@@ -1824,17 +1824,24 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
     // Mean time of `kernel` per batch with tuning t (one untimed batch first).
     // Fastest of `reps` batches (the minimum is robust to a batch slowed by
     // something else on the device), after one untimed batch.
+    // One frame is timed through the per-frame path (whose CDEF launch reads
+    // its side information in place and copies the staged bytes), several
+    // through one batched call: each path's own kernels.
+    auto run = [&]() {
+        if (n == 1)
+            return vaco_av1pf_frame_device(ctx, descs, ins, outs, nullptr);
+        return av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr, nullptr, nullptr);
+    };
     auto measure = [&](const vaco_av1pf_tuning &t, int kernel, double *res) {
         ctx->tune = t;
         *res = 0.0;
-        vaco_av1pf_status r =
-            av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr, nullptr, nullptr);
+        vaco_av1pf_status r = run();
         if (r == VACO_AV1PF_OK)
             r = vaco_av1pf_finish(ctx);
         if (r == VACO_AV1PF_OK)
             r = vaco_av1pf_profile_read(ctx, nullptr, nullptr, nullptr);
         for (uint32_t i = 0; i < reps && r == VACO_AV1PF_OK; i++) {
-            r = av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr, nullptr, nullptr);
+            r = run();
             if (r == VACO_AV1PF_OK)
                 r = vaco_av1pf_finish(ctx);
             if (r == VACO_AV1PF_OK)
