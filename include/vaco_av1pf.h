@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define VACO_AV1PF_ABI_VERSION 2
+#define VACO_AV1PF_ABI_VERSION 3
 
 /* vaco_core::Error classes (crates/core/vaco-core/src/error.rs:14-59). */
 typedef enum vaco_av1pf_status {
@@ -422,6 +422,10 @@ typedef struct vaco_av1pf_tuning {
                                 1: in plane order, numbered XCD-contiguously */
     uint8_t lr_tile_rows;    /* restoration tile height: 64, 32 or 16 rows */
     uint8_t cdef_waves;      /* waves per CDEF workgroup (one superblock): 16 or 8 */
+    uint8_t chain;           /* 1: a single frame with CDEF and restoration but no
+                                superres runs both in one launch (restoration tiles
+                                wait for the superblocks they read; 8-wave CDEF
+                                workgroups, cdef_waves unused); 0: one launch each */
 } vaco_av1pf_tuning;
 
 vaco_av1pf_status vaco_av1pf_get_tuning(const vaco_av1pf_ctx *ctx, vaco_av1pf_tuning *out);
@@ -436,12 +440,24 @@ vaco_av1pf_status vaco_av1pf_set_tuning(vaco_av1pf_ctx *ctx, const vaco_av1pf_tu
  * keeps the fastest variant per kernel as the context's tuning and returns
  * it in *out. A field leaves its current value only for a variant at least
  * 2% faster, and for the CDEF order that moves more HBM bytes
- * (cdef_xcd_order 0) at least 5% faster. Synchronous; the outputs hold the
- * frames' results. */
+ * (cdef_xcd_order 0) at least 5% faster. `chain` is decided on one frame by
+ * the chained launch's time against the sum of the separate launches', and
+ * the orders and tile height are then tuned on the chained launch.
+ * Synchronous; the outputs hold the frames' results. */
 vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n,
                                       const vaco_av1pf_frame_desc *descs,
                                       const vaco_av1pf_picture *ins, vaco_av1pf_picture *outs,
                                       uint32_t reps, vaco_av1pf_tuning *out);
+
+/* No reference counterpart (test hook). Sets how many times a restoration
+ * tile of a chained launch (tuning.chain) polls for the superblocks it reads
+ * before it gives up; a tile that gives up lets the frame fail with
+ * DEVICE_ERROR ("... gave up waiting for its CDEF superblocks") instead of
+ * waiting on. The default (2^22 polls, about a second) is never reached:
+ * workgroups are dispatched in order, so every superblock a tile waits for
+ * is being filtered. 0 makes every tile that finds a superblock unfinished
+ * give up at once, so the error path can be exercised without a stall. */
+vaco_av1pf_status vaco_av1pf_set_chain_spin_limit(vaco_av1pf_ctx *ctx, uint32_t polls);
 
 /* ----------------------------------------------------------- serving --- */
 
@@ -554,7 +570,8 @@ enum {
     VACO_AV1PF_KERNEL_SUPERRES = 1,
     VACO_AV1PF_KERNEL_RESTORATION = 2,
     VACO_AV1PF_KERNEL_UPLOAD = 3,
-    VACO_AV1PF_KERNEL_COUNT = 4
+    VACO_AV1PF_KERNEL_CHAIN = 4, /* CDEF and restoration in one launch (tuning.chain) */
+    VACO_AV1PF_KERNEL_COUNT = 5
 };
 
 /* No reference counterpart (measurement hook). While enabled, every kernel
@@ -567,7 +584,8 @@ enum {
  * sample it writes once, 2 bytes each (CDEF: its planes in and out;
  * superres: visible source rows in, output rows out; restoration: its
  * tiles in and out; upload: the staged side information, read from host
- * memory and written to the device). Any argument may be NULL. Enabling
+ * memory and written to the device; chain: the input planes in and the
+ * restored planes out). Any argument may be NULL. Enabling
  * allocates the event pool. */
 vaco_av1pf_status vaco_av1pf_profile(vaco_av1pf_ctx *ctx, int enable);
 vaco_av1pf_status vaco_av1pf_profile_read(vaco_av1pf_ctx *ctx,

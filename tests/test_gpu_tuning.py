@@ -20,7 +20,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-DEFAULT = {"superres_rows": 8, "cdef_xcd_order": 1, "lr_order": 0, "lr_tile_rows": 64, "cdef_waves": 16}
+DEFAULT = {"superres_rows": 8, "cdef_xcd_order": 1, "lr_order": 0, "lr_tile_rows": 64, "cdef_waves": 16, "chain": 0}
 
 
 @pytest.fixture(scope="module")
@@ -65,7 +65,7 @@ def test_gpu_tuning_defaults_and_validation(ctx):
     say("tuning: defaults and rejection")
     assert ctx.tuning == DEFAULT
     for bad in ({"superres_rows": 5}, {"cdef_xcd_order": 2}, {"lr_order": 3}, {"lr_tile_rows": 48},
-                {"cdef_waves": 4}, {"cdef_waves": 12}):
+                {"cdef_waves": 4}, {"cdef_waves": 12}, {"chain": 2}):
         with pytest.raises(av1pf.Av1Error) as e:
             ctx.tuning = bad
         assert e.value.status == abi.INVALID_ARGUMENT
@@ -78,8 +78,10 @@ def test_gpu_every_launch_variant_matches_oracle(ctx, batch, rows, cdef_xcd, lr_
     frames, want, dev_in, dev_out = batch
     label = f"rows={rows} cdef_xcd={cdef_xcd} lr_order={lr_order} lr_rows={lr_rows} cdef_waves={waves}"
     say("variant", label)
+    # chain off: the per-frame calls below run the separate kernels of this
+    # variant (the chained launch has tests of its own)
     ctx.tuning = {"superres_rows": rows, "cdef_xcd_order": cdef_xcd, "lr_order": lr_order, "lr_tile_rows": lr_rows,
-                  "cdef_waves": waves}
+                  "cdef_waves": waves, "chain": 0}
     try:
         ctx.batch_device(frames, ptrs(dev_in), ptrs(dev_out))
         ctx.finish()
@@ -97,7 +99,7 @@ def test_gpu_autotune_returns_a_valid_choice_and_exact_outputs(ctx, batch):
     say("autotune")
     t = ctx.autotune(frames, ptrs(dev_in), ptrs(dev_out), reps=2)
     assert t["superres_rows"] in (4, 8, 16) and t["cdef_xcd_order"] in (0, 1) and t["lr_order"] in (0, 1)
-    assert t["lr_tile_rows"] in (64, 32, 16) and t["cdef_waves"] in (16, 8)
+    assert t["lr_tile_rows"] in (64, 32, 16) and t["cdef_waves"] in (16, 8) and t["chain"] in (0, 1)
     assert ctx.tuning == t
     check(dev_out, want, "after autotune")
     ctx.batch_device(frames, ptrs(dev_in), ptrs(dev_out))

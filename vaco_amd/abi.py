@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 # VACO_AV1PF_ABI_VERSION of include/vaco_av1pf.h that these layouts follow.
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OK = 0
 INVALID_DATA = 1
@@ -179,6 +179,7 @@ class Tuning(C.Structure):
         ("lr_order", C.c_uint8),
         ("lr_tile_rows", C.c_uint8),
         ("cdef_waves", C.c_uint8),
+        ("chain", C.c_uint8),
     ]
 
 

@@ -153,6 +153,8 @@ def lib():
         L.vaco_av1pf_read_cdef.argtypes = [P(abi.SymbolDecoder), C.c_int, C.c_uint32, P(C.c_int8), C.c_uint32,
                                            C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint8, C.c_int]
         L.vaco_av1pf_read_cdef.restype = None
+        L.vaco_av1pf_set_chain_spin_limit.argtypes = [ctxp, C.c_uint32]
+        L.vaco_av1pf_set_chain_spin_limit.restype = C.c_int
         L.vaco_av1pf_get_tuning.argtypes = [ctxp, P(abi.Tuning)]
         L.vaco_av1pf_set_tuning.argtypes = [ctxp, P(abi.Tuning)]
         L.vaco_av1pf_autotune.argtypes = [ctxp, C.c_uint32, P(abi.FrameDesc), P(abi.Picture), P(abi.Picture),
@@ -379,7 +381,7 @@ class Context:
 
     # ---- tuning
 
-    TUNING_FIELDS = ("superres_rows", "cdef_xcd_order", "lr_order", "lr_tile_rows", "cdef_waves")
+    TUNING_FIELDS = ("superres_rows", "cdef_xcd_order", "lr_order", "lr_tile_rows", "cdef_waves", "chain")
 
     @property
     def tuning(self) -> dict:
@@ -398,6 +400,10 @@ class Context:
         st = self._lib.vaco_av1pf_set_tuning(self.handle, C.byref(t))
         if st != abi.OK:
             raise Av1Error(st, f"invalid tuning {values}")
+
+    def set_chain_spin_limit(self, polls: int):
+        """Polls a chained launch's restoration tile makes before giving up (test hook)."""
+        self._check(self._lib.vaco_av1pf_set_chain_spin_limit(self.handle, polls))
 
     def autotune(self, frames, in_ptrs, out_ptrs, reps: int = 3) -> dict:
         """Time every launch variant on these frames (device planes) and keep
@@ -424,7 +430,7 @@ class Context:
 
     # ---- kernel timing
 
-    KERNELS = ("cdef", "superres", "restoration", "upload")
+    KERNELS = ("cdef", "superres", "restoration", "upload", "chain")
 
     def profile(self, enable: bool = True):
         self._check(self._lib.vaco_av1pf_profile(self.handle, 1 if enable else 0))

@@ -8,8 +8,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvaco_av1pf.so")
-SOURCES = ["av1pf_cdef.hip", "av1pf_superres.hip", "av1pf_lr.hip", "av1pf_stage.hip", "av1pf_host.cpp", "av1pf_replay.cpp",
-           "av1pf_gen.cpp", "av1pf_serve.cpp", "av1pf_syntax.cpp", "av1pf_static.hip"]
+SOURCES = ["av1pf_cdef.hip", "av1pf_superres.hip", "av1pf_lr.hip", "av1pf_chain.hip", "av1pf_stage.hip",
+           "av1pf_host.cpp", "av1pf_replay.cpp", "av1pf_gen.cpp", "av1pf_serve.cpp", "av1pf_syntax.cpp",
+           "av1pf_static.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -48,6 +48,11 @@ __device__ __forceinline__ void raise_status(uint32_t *status, uint32_t bits)
     atomicOr_system(status, bits);
 }
 
+// Wait until this wave's vector memory operations, stores included, have
+// completed (s_waitcnt vmcnt(0); the expcnt and lgkmcnt fields at their
+// maximum are not waited on).
+__device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
 // XCD b % 8) and each XCD has its own L2. The remap gives each XCD one
 // contiguous range of the n work items, so neighbouring items, which share

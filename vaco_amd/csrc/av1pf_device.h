@@ -122,4 +122,25 @@ struct LrPlane {
     uint8_t out_vec;                   // out 8-byte aligned, out_stride a multiple of 4
 };
 
+// ------------------------------------------------------ chained launch --
+
+// Ordering state of a chained launch (CDEF, then restoration in the same
+// grid; restoration tiles wait for the superblocks they read). Both arrays
+// live in device memory owned by the context and are never reset: flags[sb]
+// holds the epoch of the last chained launch that finished superblock sb,
+// and *copied counts the workgroups, over every chained launch of the
+// context, that have finished their share of the staged-bytes copy.
+struct ChainSync {
+    uint32_t *flags;
+    unsigned long long *copied;
+    unsigned long long copied_target;  // *copied once this launch's copy is complete
+    uint32_t epoch;                    // nonzero, distinct from every earlier launch's
+    uint32_t spin_limit;               // polls before a waiting workgroup gives up
+};
+
+// Bit set in a frame's status word when a chained launch's restoration tile
+// gave up waiting for its superblocks (never expected: workgroups are
+// dispatched in order, so every superblock's workgroup is running or done).
+constexpr uint32_t kFlagChainStall = 2u;
+
 }  // namespace av1pf

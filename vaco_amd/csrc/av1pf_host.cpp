@@ -33,6 +33,10 @@ hipError_t launch_cdef(const CdefFrame *frames, int n, const CdefFrame *single, 
                        hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, int rows, hipStream_t s,
                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_chain(const CdefFrame &f, int mode, bool xcd, const LrPlane *planes, const uint64_t *list,
+                        uint32_t n_tiles, bool lr_xcd, const void *up_src, void *up_dst, size_t up_bytes,
+                        const ChainSync &cs, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
+uint32_t chain_copiers(uint32_t n_sb, size_t up_bytes);
 hipError_t launch_lr(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd, int rows,
                      hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s,
@@ -53,7 +57,12 @@ constexpr int kSlots = 8;
 constexpr uint32_t kMaxBatch = 256;  // frames per batched call
 constexpr uint32_t kMaxSuperresWidth = 65536;  // k_superres's 32-bit source positions
 // Launch variants of a new context (vaco_av1pf_autotune may change them).
-constexpr vaco_av1pf_tuning kDefaultTuning = {8, 1, 0, 64, 16};
+constexpr vaco_av1pf_tuning kDefaultTuning = {8, 1, 0, 64, 16, 0};
+constexpr const char *kChainStallMsg =
+    "a restoration tile of a chained launch gave up waiting for its CDEF superblocks";
+// Polls (each ~0.1-0.3 us) a chained launch's restoration tile makes before
+// it gives up on its superblocks: about a second, far beyond any frame.
+constexpr uint32_t kChainSpinLimit = 1u << 22;
 // Staging bytes of the static-scratch entry (the rest of its scratch holds
 // intermediate images).
 constexpr size_t kStaticStageBytes = 2u << 20;
@@ -131,6 +140,17 @@ struct vaco_av1pf_ctx {
     // those buffers.
     hipStream_t last_stream = nullptr;
     hipEvent_t last_done = nullptr;
+    // Chained launches (tuning.chain): per-superblock completion flags and
+    // the staged-bytes copy counter (device memory, zeroed at creation,
+    // never reset); the counter's value once every chained launch so far has
+    // completed (calls on a context are ordered on the device), and the
+    // epoch of the last launch.
+    uint32_t *chain_flags = nullptr;
+    unsigned long long *chain_copied = nullptr;
+    size_t chain_sbs = 0;
+    uint64_t chain_copied_total = 0;
+    uint32_t chain_epoch = 0;
+    uint32_t chain_spin_limit = kChainSpinLimit;
     // kernel timing (vaco_av1pf_profile)
     struct ProfRec {
         int kernel;
@@ -364,6 +384,16 @@ extern "C" vaco_av1pf_status vaco_av1pf_create(int device, uint32_t max_width, u
         if (e != hipSuccess)
             return bail(e);
     }
+    // Chained-launch state: one flag per superblock of the largest frame,
+    // then the copy counter.
+    ctx->chain_sbs = (size_t)((ctx->max_w + 63) / 64) * ((ctx->max_h + 63) / 64);
+    const size_t flag_bytes = (ctx->chain_sbs * sizeof(uint32_t) + 255) & ~size_t(255);
+    e = hipMalloc((void **)&ctx->chain_flags, flag_bytes + 256);
+    if (e == hipSuccess)
+        e = hipMemset(ctx->chain_flags, 0, flag_bytes + 256);
+    if (e != hipSuccess)
+        return bail(e);
+    ctx->chain_copied = reinterpret_cast<unsigned long long *>(reinterpret_cast<uint8_t *>(ctx->chain_flags) + flag_bytes);
     // Status words: coherent so kernel atomics are visible to the host after
     // the frame's event completes; one 64-byte line per slot.
     e = hipHostMalloc((void **)&ctx->h_status, kSlots * kMaxBatch * sizeof(uint32_t),
@@ -412,6 +442,8 @@ extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
         hipFree(ctx->pool);
     if (ctx->ws)
         hipFree(ctx->ws);
+    if (ctx->chain_flags)
+        hipFree(ctx->chain_flags);
     if (ctx->stream)
         hipStreamDestroy(ctx->stream);
     if (ctx->up_stream)
@@ -1019,6 +1051,36 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
                                       c.w * 2, c.h, hipMemcpyDeviceToDevice, s));
     }
     const uint32_t n_tiles = nlist[0] + nlist[1] + nlist[2];
+    // A single frame with CDEF and restoration but no superres: both in one
+    // launch (k_chain), if the tuning says so.
+    int chain_mode = -1;
+    if (fused_upload && ctx && ctx->tune.chain && dj == nullptr && dl && n_tiles && plan.copies.empty())
+        for (int m = 0; m < 4; m++)
+            if (dc[m] && plan.cdef[m].size() == 1 &&
+                (size_t)plan.cdef[m][0].sb_cols * plan.cdef[m][0].sb_rows <= ctx->chain_sbs)
+                chain_mode = m;
+    if (chain_mode >= 0) {
+        const int m = chain_mode;
+        ChainSync cs;
+        cs.flags = ctx->chain_flags;
+        cs.copied = ctx->chain_copied;
+        if (++ctx->chain_epoch == 0) {  // after 2^32 launches: clear the flags, start again
+            HIPCHECK(hipMemsetAsync(ctx->chain_flags, 0, ctx->chain_sbs * sizeof(uint32_t), s));
+            ctx->chain_epoch = 1;
+        }
+        cs.epoch = ctx->chain_epoch;
+        cs.copied_target = ctx->chain_copied_total + chain_copiers(single.sb_cols * single.sb_rows, up_bytes);
+        ctx->chain_copied_total = cs.copied_target;
+        cs.spin_limit = ctx->chain_spin_limit;
+        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CHAIN,
+                              plan.cdef_bytes[m] / 2 + plan.bytes[VACO_AV1PF_KERNEL_RESTORATION] / 2, s,
+                              [&](hipEvent_t e0, hipEvent_t e1) {
+                                  return launch_chain(single, m, plan.tune.cdef_xcd_order != 0, dl, dlist, n_tiles,
+                                                      plan.tune.lr_order != 0, a->hd, a->d, up_bytes, cs, s, e0,
+                                                      e1);
+                              }));
+        return VACO_AV1PF_OK;
+    }
     auto launch_all = [&]() -> vaco_av1pf_status {
         for (int m = 0; m < 4; m++)
             if (dc[m])
@@ -1291,6 +1353,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame(vaco_av1pf_ctx *ctx, const vaco_av
     if ((st = close_slot(ctx, a, ctx->stream)) != VACO_AV1PF_OK)
         return st;
     HIPCHECK(hipStreamSynchronize(ctx->stream));
+    if (*a->status & kFlagChainStall)
+        return fail(ctx, VACO_AV1PF_DEVICE_ERROR, kChainStallMsg);
     if (*a->status & kFlagSampleRange) {
         const char *msg = "";
         vaco_av1pf_status r = replay_checks(*desc, g, hin, fuel, &msg);
@@ -1433,6 +1497,11 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame_device_to_frame(vaco_av1pf_ctx *ct
 static vaco_av1pf_status resolve_slot(vaco_av1pf_ctx *ctx, Arena &a)
 {
     for (size_t i = 0; i < a.descs.size(); i++) {
+        if ((a.status[i] & kFlagChainStall) && ctx->deferred == VACO_AV1PF_OK) {
+            ctx->deferred = VACO_AV1PF_DEVICE_ERROR;
+            ctx->deferred_msg = kChainStallMsg;
+            continue;
+        }
         if (!(a.status[i] & kFlagSampleRange) || ctx->deferred != VACO_AV1PF_OK)
             continue;
         const vaco_av1pf_frame_desc &d = a.descs[i];
@@ -1786,7 +1855,7 @@ static bool tuning_ok(const vaco_av1pf_tuning &t)
     return (t.superres_rows == 4 || t.superres_rows == 8 || t.superres_rows == 16) &&
            t.cdef_xcd_order <= 1 && t.lr_order <= 1 &&
            (t.lr_tile_rows == 64 || t.lr_tile_rows == 32 || t.lr_tile_rows == 16) &&
-           (t.cdef_waves == 16 || t.cdef_waves == 8);
+           (t.cdef_waves == 16 || t.cdef_waves == 8) && t.chain <= 1;
 }
 
 extern "C" vaco_av1pf_status vaco_av1pf_get_tuning(const vaco_av1pf_ctx *ctx, vaco_av1pf_tuning *out)
@@ -1846,12 +1915,24 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
                 r = vaco_av1pf_finish(ctx);
             if (r == VACO_AV1PF_OK)
                 r = vaco_av1pf_profile_read(ctx, ms, launches, nullptr);
-            if (r == VACO_AV1PF_OK && launches[kernel] && (i == 0 || ms[kernel] < *res))
-                *res = ms[kernel];
+            // kernel < 0: every kernel of the call, summed
+            double v = 0;
+            uint64_t nl = 0;
+            for (int k = 0; k < VACO_AV1PF_KERNEL_COUNT; k++)
+                if (kernel < 0 || k == kernel) {
+                    v += ms[k];
+                    nl += launches[k];
+                }
+            if (r == VACO_AV1PF_OK && nl && (i == 0 || v < *res))
+                *res = v;
         }
         return r;
     };
     vaco_av1pf_tuning best = ctx->tune, t;
+    // The separate kernels are tuned first; for one frame the chain is
+    // decided after them, batches (which never chain) keep the value.
+    const uint8_t chain_in = best.chain;
+    best.chain = 0;
     // One field at a time: the current value is timed first, and another
     // replaces it only when at least 2% faster, so that timing noise between
     // equivalent variants does not flip the choice. A value listed in
@@ -1885,12 +1966,42 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
     tune_field(&vaco_av1pf_tuning::cdef_waves, {16, 8}, VACO_AV1PF_KERNEL_CDEF);
     tune_field(&vaco_av1pf_tuning::lr_order, {0, 1}, VACO_AV1PF_KERNEL_RESTORATION);
     tune_field(&vaco_av1pf_tuning::lr_tile_rows, {64, 32, 16}, VACO_AV1PF_KERNEL_RESTORATION);
+    // One frame: the chained launch against the separate ones (every kernel
+    // of the call, summed; the chain also saves the gap between launches,
+    // which no kernel time holds, so it needs only to be no slower). If it
+    // is kept, the orders and the tile height are tuned again on it.
+    best.chain = chain_in;
+    if (n == 1 && st == VACO_AV1PF_OK) {
+        double sep_ms = 0, chain_ms = 0;
+        t = best;
+        t.chain = 0;
+        if ((st = measure(t, -1, &sep_ms)) == VACO_AV1PF_OK) {
+            t.chain = 1;
+            st = measure(t, -1, &chain_ms);
+        }
+        if (st == VACO_AV1PF_OK) {
+            best.chain = chain_ms > 0 && chain_ms <= sep_ms ? 1 : 0;
+            if (best.chain) {
+                tune_field(&vaco_av1pf_tuning::cdef_xcd_order, {0, 1}, VACO_AV1PF_KERNEL_CHAIN, 0);
+                tune_field(&vaco_av1pf_tuning::lr_order, {0, 1}, VACO_AV1PF_KERNEL_CHAIN);
+                tune_field(&vaco_av1pf_tuning::lr_tile_rows, {64, 32, 16}, VACO_AV1PF_KERNEL_CHAIN);
+            }
+        }
+    }
     ctx->tune = best;
     const vaco_av1pf_status ps = vaco_av1pf_profile(ctx, was_profiling ? 1 : 0);
     if (st != VACO_AV1PF_OK)
         return st;
     *out = best;
     return ps;
+}
+
+extern "C" vaco_av1pf_status vaco_av1pf_set_chain_spin_limit(vaco_av1pf_ctx *ctx, uint32_t polls)
+{
+    if (!ctx)
+        return VACO_AV1PF_INVALID_ARGUMENT;
+    ctx->chain_spin_limit = polls;
+    return VACO_AV1PF_OK;
 }
 
 // ------------------------------------------------ static-scratch entry --
