@@ -152,9 +152,9 @@ static void launch_cdef_t(const CdefFrame *frames, int n, const CdefFrame &singl
 // threads) per superblock's workgroup. ev0/ev1 (may be null) receive the
 // dispatch's start and end.
 // single (may be null): the one frame's descriptor, passed by value in the
-// kernel arguments (frames and n are then unused); up_bytes must then be
-// positive: see k_cdef (up_src, up_dst 16-byte aligned). Without single, up_*
-// are unused.
+// kernel arguments (frames and n are then unused); up_src (the slot's host
+// memory, at least 16 bytes, read even when up_bytes is 0) and up_dst are
+// then 16-byte aligned: see cdef_sb. Without single, up_* are unused.
 hipError_t launch_cdef(const CdefFrame *frames, int n, const CdefFrame *single, uint32_t blocks, int mode,
                        bool xcd, int waves, const void *up_src, void *up_dst, size_t up_bytes, hipStream_t s,
                        hipEvent_t ev0, hipEvent_t ev1)
@@ -164,8 +164,8 @@ hipError_t launch_cdef(const CdefFrame *frames, int n, const CdefFrame *single, 
     const uint4 *us = static_cast<const uint4 *>(up_src);
     uint4 *ud = static_cast<uint4 *>(up_dst);
     const uint32_t n16 = (uint32_t)((up_bytes + 15) / 16);
-    if (single && n16 == 0)
-        return hipErrorInvalidValue;  // the single-frame kernels copy the staged bytes
+    if (single && !up_src)
+        return hipErrorInvalidValue;  // the single-frame kernels read (at least) unit 0 of the staged bytes
     const CdefFrame none{};
     const CdefFrame &one = single ? *single : none;
     if (waves == 8) {

@@ -101,16 +101,37 @@ struct Arena {
     };
     std::vector<Graph> graphs;
 
+    size_t tail = 0;  // bytes taken from the end of the slot (take_host)
+
     template <class T>
     T *take(size_t n, T **dev)
     {
         size_t a = (used + 255) & ~size_t(255);
         size_t bytes = n * sizeof(T);
-        if (a + bytes > cap)
+        if (a + bytes + tail > cap)
             return nullptr;
         used = a + bytes;
         *dev = reinterpret_cast<T *>(d + a);
         return reinterpret_cast<T *>(h + a);
+    }
+
+    // Bytes that only the CDEF kernel reads (cdef_idx, skip flags): taken
+    // from the end of the slot and never uploaded; *dev is their address in
+    // the slot's coherent host memory as the device sees it, which the CDEF
+    // kernel reads in place (a few hundred bytes per superblock, issued with
+    // its sample loads). A slot of ordinary host memory has no such view:
+    // there they are uploaded like the rest.
+    template <class T>
+    T *take_host(size_t n, T **dev)
+    {
+        if (pageable || !hd)
+            return take(n, dev);
+        const size_t bytes = (n * sizeof(T) + 255) & ~size_t(255);
+        if (((used + 255) & ~size_t(255)) + tail + bytes > cap)
+            return nullptr;
+        tail += bytes;
+        *dev = reinterpret_cast<T *>(hd + cap - tail);
+        return reinterpret_cast<T *>(h + cap - tail);
     }
 };
 
@@ -205,6 +226,7 @@ static hipError_t alloc_slot(Arena &a, size_t bytes)
         (void)hipFree(a.d);
     a.h = a.hd = a.d = nullptr;
     a.cap = 0;
+    a.used = a.tail = 0;
     hipError_t e = hipHostMalloc((void **)&a.h, bytes, hipHostMallocCoherent);
     if (e == hipSuccess)
         e = hipHostGetDevicePointer((void **)&a.hd, a.h, 0);
@@ -229,6 +251,7 @@ static vaco_av1pf_status take_slot(vaco_av1pf_ctx *ctx, Arena **out)
     if (st != VACO_AV1PF_OK)
         return st;
     a.used = 0;
+    a.tail = 0;
     memset(a.status, 0, kMaxBatch * sizeof(uint32_t));
     *out = &a;
     return VACO_AV1PF_OK;
@@ -768,8 +791,12 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
     if (g.cdef) {
         const size_t nidx = (size_t)g.idx_rows * d.cdef_stride;
         const size_t nskip = (size_t)(g.mi_rows / 2) * (g.mi_cols / 2);
-        TAKE(hidx, didx, int8_t, nidx);
-        TAKE(hskip, dskip, uint8_t, nskip);
+        int8_t *didx;
+        uint8_t *dskip;
+        int8_t *hidx = a->take_host<int8_t>(nidx, &didx);
+        uint8_t *hskip = a->take_host<uint8_t>(nskip, &dskip);
+        if (!hidx || !hskip)
+            return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
         memcpy(hidx, d.cdef_idx, nidx);
         memcpy(hskip, d.skip8, nskip);
         CdefFrame f;
@@ -996,9 +1023,6 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
     // arguments it was captured with, and this descriptor changes per frame.)
     const bool graphs = ctx && ctx->use_graphs && !ctx->prof;
     const bool fused_upload = single_frame && cdef_modes == 1 && !a->pageable && a->hd && !graphs;
-    auto host_view = [&](const void *dev) -> const void * {
-        return a->hd + (static_cast<const uint8_t *>(dev) - a->d);
-    };
     const CdefFrame *dc[4] = {};
     CdefFrame single;  // the frame's descriptor, passed as a kernel argument
     memset(&single, 0, sizeof single);
@@ -1006,9 +1030,7 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
         if (plan.cdef[m].empty())
             continue;
         if (fused_upload) {
-            single = plan.cdef[m][0];
-            single.cdef_idx = static_cast<const int8_t *>(host_view(single.cdef_idx));
-            single.skip8 = static_cast<const uint8_t *>(host_view(single.skip8));
+            single = plan.cdef[m][0];  // its side information is in the slot's host memory
             dc[m] = &single;  // marks the mode; the kernel takes `single`
             continue;
         }
