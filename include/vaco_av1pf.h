@@ -259,8 +259,9 @@ vaco_av1pf_status vaco_av1pf_batch_device(vaco_av1pf_ctx *ctx, uint32_t n,
  * output planes are written, with the status and, in *msg (may be NULL),
  * the message vaco_av1pf_frame would report. Frames whose staging exceeds
  * 2 MiB or whose intermediates exceed the rest of the array (a 4:2:0 frame
- * with restoration up to 3840x2176 without superres, or 1920x1088 with it)
- * return VACO_AV1PF_LIMIT_EXCEEDED before any device work. Every call on a
+ * with restoration up to 3840x2176, with or without superres when every
+ * plane restores; not a 4:4:4 frame of that size) return
+ * VACO_AV1PF_LIMIT_EXCEEDED before any device work. Every call on a
  * GPU shares that GPU's array and leaves the last call's contents in it; the
  * library serialises this process's calls on one GPU, and calls from other
  * processes use their own copy. */

@@ -38,12 +38,13 @@ CASES = [
     dict(seed=310, coded_w=190, coded_h=190, idx_neg_p=0.6, skip_p=0.5, unit_kinds=(2,)),
     dict(seed=311, coded_w=128, coded_h=128, zero_strengths=True, unit_kinds=(1,)),  # CDEF off: not chained
     dict(seed=312, coded_w=129, coded_h=65, lr_types=(0, 1, 1), unit_kinds=(0,)),
+    dict(seed=313, coded_w=171, coded_h=130, upscaled_w=300, bd=10),  # superres: not chained
 ]
 
 
 # Zero strengths everywhere turn CDEF off (decode.rs:741-748), so that frame
-# runs restoration alone.
-CHAINED = [not c.get("zero_strengths") for c in CASES] + [True]
+# runs restoration alone; a superres frame keeps its launches apart.
+CHAINED = [not c.get("zero_strengths") and not c.get("upscaled_w") for c in CASES] + [True]
 
 
 @pytest.fixture(scope="module")

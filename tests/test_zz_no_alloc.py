@@ -74,9 +74,22 @@ def test_static_entry_reports_the_reference_error_for_an_out_of_range_sample():
     assert (got.value.status, got.value.message) == (want.value.status, want.value.message)
 
 
+def test_static_entry_runs_a_4k_superres_frame():
+    """4K 10-bit with superres (denominator 9) and restoration: restoration
+    upscales inside its staging, so the only intermediate is the CDEF image
+    (~22 MB), which fits the static scratch."""
+    f = av1pf.generate(workloads.cfg2(), 1)
+    say("static: 4K superres frame")
+    got = run_static(f)
+    want = run_device(f)
+    for p, (g, w) in enumerate(zip(got, want)):
+        assert np.array_equal(g, w), p
+
+
 def test_static_entry_refuses_a_frame_larger_than_its_scratch():
     from vaco_amd import hip
-    f = av1pf.generate(workloads.cfg2(), 1)  # 4K with superres: ~72 MB of intermediates
+    # 4K 4:4:4 with CDEF and restoration: a ~50 MB CDEF image
+    f = F.random_frame(501, 3840, 2160, ssx=0, ssy=0, unit=256)
     dev_in = [hip.DeviceArray.from_host(p) for p in f.planes]
     dev_out = [hip.DeviceArray(g) for g in f.output_geometry()]
     with pytest.raises(av1pf.Av1Error) as e:

@@ -105,8 +105,8 @@ struct LrUnit {
 
 // One plane's restoration work: row segments x column segments.
 struct LrPlane {
-    const uint16_t *before;            // pre-CDEF (upscaled when superres)
-    const uint16_t *after;             // CDEF output (upscaled when superres)
+    const uint16_t *before;            // pre-CDEF (upscaled when superres, unless sr_step)
+    const uint16_t *after;             // CDEF output (upscaled when superres, unless sr_step)
     uint16_t *out;
     const LrRowSeg *rows;
     const LrColSeg *cols;
@@ -116,6 +116,12 @@ struct LrPlane {
     uint32_t width, height;            // visible geometry
     uint32_t n_rows, n_cols, unit_cols;
     uint32_t block_begin;
+    // Superres folded into the staging (superres.rs:163-216): sr_step != 0
+    // means `before` and `after` are the images before upscaling, sr_in_width
+    // samples wide (padded), and each staged sample is upscaled where it is
+    // staged, with this plane's step and initial position.
+    int32_t sr_step, sr_start;
+    uint32_t sr_in_width;
     uint8_t bit_depth, ss_y;
     uint8_t vec_ok;                    // before/after 8-byte aligned, strides multiples of 4
     uint8_t out_u8;                    // out holds one byte per sample (8-bit frame output)

@@ -674,25 +674,6 @@ static void lr_tile_list(const std::vector<uint64_t> (&tiles)[3], bool by_kind, 
     }
 }
 
-// Pre-CDEF rows that Source::get can read at stripe borders
-// (restoration.rs:236-241), for the upscaled pre-CDEF image.
-void lr_border_rows(const std::vector<LrRowSeg> &rows, uint32_t height, std::vector<uint32_t> &out)
-{
-    out.clear();
-    const int H = (int)height;
-    for (const LrRowSeg &s : rows) {
-        for (int y = s.y0 - 3; y < s.y0 + s.h + 3; y++) {
-            const int yc = std::min(std::max(y, 0), H - 1);
-            if (yc < s.stripe_start)
-                out.push_back((uint32_t)std::max(yc, s.stripe_start - 2));
-            else if (yc > s.stripe_end)
-                out.push_back((uint32_t)std::min(yc, s.stripe_end + 2));
-        }
-    }
-    std::sort(out.begin(), out.end());
-    out.erase(std::unique(out.begin(), out.end()), out.end());
-}
-
 }  // namespace
 
 // Template instance of the CDEF kernel for a frame's chroma layout.
@@ -722,16 +703,29 @@ struct Scratch {
     }
 };
 
-// Samples of intermediates a frame needs (see plan_frame).
-static size_t scratch_need(const FrameGeo &g)
+// Samples of intermediates a frame needs (see plan_frame): the CDEF image
+// when a stage follows CDEF, and with superres and restoration, the upscaled
+// CDEF image of each plane that restoration only copies (planes it filters
+// upscale inside restoration's staging).
+static size_t scratch_need(const vaco_av1pf_frame_desc &d, const FrameGeo &g)
 {
     size_t n = 0;
     for (int p = 0; p < g.nplanes; p++) {
         if (g.cdef && (g.superres || g.lr))
             n += (size_t)g.in_w[p] * g.in_h[p] + 128;
-        if (g.superres && g.lr)
-            n += 2 * ((size_t)g.sr_out_w[p] * g.sr_h[p] + 128);
+        if (g.superres && g.lr && d.lr_type[p] == VACO_AV1PF_RESTORE_NONE)
+            n += (size_t)g.sr_out_w[p] * g.sr_h[p] + 128;
     }
+    return n;
+}
+
+// An upper bound of scratch_need over every parameter choice of frames of
+// geometry g: both images in every plane.
+static size_t scratch_bound(const FrameGeo &g)
+{
+    size_t n = 0;
+    for (int p = 0; p < g.nplanes; p++)
+        n += (size_t)g.in_w[p] * g.in_h[p] + (size_t)g.sr_out_w[p] * g.sr_h[p] + 256;
     return n;
 }
 
@@ -752,6 +746,7 @@ struct Plan {
         uint32_t w, h;
     };  // w x h samples; dst.u8: to bytes
     std::vector<Copy> copies;  // frames in which no stage runs
+    bool superres = false;  // some frame is upscaled (restoration then upscales inside its staging)
     vaco_av1pf_tuning tune = kDefaultTuning;  // launch variants this plan is built for
     // algorithmic bytes per kernel id (samples read once + written once, 2 B)
     uint64_t bytes[VACO_AV1PF_KERNEL_COUNT] = {};
@@ -870,59 +865,45 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             lr_col_segments(g.lr_w[p], d.lr_unit_size[p], lr_cols[p]);
         }
     }
+    // A plane that restoration filters is upscaled inside restoration's
+    // staging (LrPlane::sr_step): no superres job, no upscaled images.
+    plan.superres |= g.superres;
+    auto folded = [&](int p) { return g.superres && g.lr && d.lr_type[p] != VACO_AV1PF_RESTORE_NONE; };
     if (g.superres) {
-        for (int img = 0; img < (g.lr ? 2 : 1); img++) {
-            for (int p = 0; p < np; p++) {
-                // img 0: the CDEF image; img 1: the pre-CDEF image, only at
-                // the rows restoration reads across stripe borders.
-                const uint32_t *drows = nullptr;
-                uint32_t nrows = g.sr_h[p];
-                if (img == 1) {
-                    if (d.lr_type[p] == VACO_AV1PF_RESTORE_NONE)
-                        continue;
-                    std::vector<uint32_t> rows;
-                    lr_border_rows(lr_rows[p], g.lr_h[p], rows);
-                    if (rows.empty())
-                        continue;
-                    TAKE(hr, dr, uint32_t, rows.size());
-                    memcpy(hr, rows.data(), rows.size() * 4);
-                    drows = dr;
-                    nrows = (uint32_t)rows.size();
-                }
-                DevPlane src = img == 0 ? cur[p] : in[p];
-                DevPlane dst = out[p];  // the frame plane equals the upscaled plane
-                if (g.lr) {
-                    dst = DevPlane{ws.take((size_t)g.sr_out_w[p] * g.sr_h[p]), g.sr_out_w[p]};
-                    if (!dst.data)
-                        return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "workspace exhausted");
-                }
-                SrJob j;
-                memset(&j, 0, sizeof j);
-                j.in = src.data;
-                j.out = dst.data;
-                j.status = status;
-                j.rows = drows;
-                j.in_width = g.in_w[p];
-                j.in_stride = src.stride;
-                j.out_stride = dst.stride;
-                j.out_width = g.sr_out_w[p];
-                j.n_rows = nrows;
-                j.segs = (g.sr_out_w[p] + kSrSeg - 1) / kSrSeg;
-                j.step = g.sr_step[p];
-                j.start = g.sr_start[p];
-                j.block_begin = plan.sr_blocks;
-                j.max_sample = (uint16_t)((1u << d.bit_depth) - 1);
-                j.out_u8 = dst.u8;
-                j.vec_ok = !((uintptr_t)j.in & 7) && !(j.in_stride & 3);
-                plan.sr_blocks += sr_job_blocks(j.n_rows, j.segs, plan.tune.superres_rows);
-                plan.bytes[VACO_AV1PF_KERNEL_SUPERRES] +=
-                    (uint64_t)j.n_rows * (g.sr_vis[p] + g.sr_out_w[p]) * 2;
-                plan.sr.push_back(j);
-                if (img == 0)
-                    cur[p] = dst;
-                else
-                    pre[p] = dst;
+        for (int p = 0; p < np; p++) {
+            // the CDEF image (the frame's output, or with restoration the
+            // image a plane without restoration is copied from)
+            if (folded(p))
+                continue;
+            DevPlane src = cur[p];
+            DevPlane dst = out[p];  // the frame plane equals the upscaled plane
+            if (g.lr) {
+                dst = DevPlane{ws.take((size_t)g.sr_out_w[p] * g.sr_h[p]), g.sr_out_w[p]};
+                if (!dst.data)
+                    return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "workspace exhausted");
             }
+            SrJob j;
+            memset(&j, 0, sizeof j);
+            j.in = src.data;
+            j.out = dst.data;
+            j.status = status;
+            j.rows = nullptr;  // every row
+            j.in_width = g.in_w[p];
+            j.in_stride = src.stride;
+            j.out_stride = dst.stride;
+            j.out_width = g.sr_out_w[p];
+            j.n_rows = g.sr_h[p];
+            j.segs = (g.sr_out_w[p] + kSrSeg - 1) / kSrSeg;
+            j.step = g.sr_step[p];
+            j.start = g.sr_start[p];
+            j.block_begin = plan.sr_blocks;
+            j.max_sample = (uint16_t)((1u << d.bit_depth) - 1);
+            j.out_u8 = dst.u8;
+            j.vec_ok = !((uintptr_t)j.in & 7) && !(j.in_stride & 3);
+            plan.sr_blocks += sr_job_blocks(j.n_rows, j.segs, plan.tune.superres_rows);
+            plan.bytes[VACO_AV1PF_KERNEL_SUPERRES] += (uint64_t)j.n_rows * (g.sr_vis[p] + g.sr_out_w[p]) * 2;
+            plan.sr.push_back(j);
+            cur[p] = dst;
         }
     }
 
@@ -964,6 +945,11 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             }
             L.before = pre[p].data;
             L.after = cur[p].data;
+            if (folded(p)) {  // before and after are the images before upscaling
+                L.sr_step = g.sr_step[p];
+                L.sr_start = g.sr_start[p];
+                L.sr_in_width = g.in_w[p];
+            }
             L.out = out[p].data;
             L.rows = drs;
             L.cols = dcs;
@@ -1074,9 +1060,12 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
     }
     const uint32_t n_tiles = nlist[0] + nlist[1] + nlist[2];
     // A single frame with CDEF and restoration but no superres: both in one
-    // launch (k_chain), if the tuning says so.
+    // launch (k_chain), if the tuning says so. (With superres a restoration
+    // tile reads CDEF columns through the upscale, which the chain's
+    // superblock dependencies do not follow.)
     int chain_mode = -1;
-    if (fused_upload && ctx && ctx->tune.chain && dj == nullptr && dl && n_tiles && plan.copies.empty())
+    if (fused_upload && ctx && ctx->tune.chain && !plan.superres && dj == nullptr && dl && n_tiles &&
+        plan.copies.empty())
         for (int m = 0; m < 4; m++)
             if (dc[m] && plan.cdef[m].size() == 1 &&
                 (size_t)plan.cdef[m][0].sb_cols * plan.cdef[m][0].sb_rows <= ctx->chain_sbs)
@@ -1568,7 +1557,6 @@ static size_t arena_need(const vaco_av1pf_frame_desc &d, const FrameGeo &g)
         n += (size_t)d.lr_unit_count[p] * sizeof(LrUnit) + 512;
         n += rows * sizeof(LrRowSeg) + cols * sizeof(LrColSeg) + 512;
         n += rows * cols * sizeof(uint64_t) + 256;            // tile list
-        n += ((size_t)g.lr_h[p] / 4 + 16) * sizeof(uint32_t);  // superres border rows
     }
     return n;
 }
@@ -1577,7 +1565,7 @@ size_t av1pf_frame_device_bytes(const vaco_av1pf_frame_desc &d)
 {
     FrameGeo g;
     frame_geometry(d, &g);
-    size_t n = scratch_need(g);
+    size_t n = scratch_need(d, g);
     for (int p = 0; p < g.nplanes; p++)
         n += (size_t)g.in_w[p] * g.in_h[p] + (size_t)g.out_w[p] * g.out_h[p];
     return n * sizeof(uint16_t);
@@ -1596,7 +1584,7 @@ size_t av1pf_frame_scratch_samples(const vaco_av1pf_frame_desc &d)
 {
     FrameGeo g;
     frame_geometry(d, &g);
-    return scratch_need(g);
+    return scratch_need(d, g);
 }
 
 // The largest frame a context accepts: 4:4:4 at its maximum geometry, with
@@ -1641,7 +1629,7 @@ size_t av1pf_worst_frame_scratch_samples(const vaco_av1pf_ctx *ctx)
     vaco_av1pf_frame_desc d;
     FrameGeo g;
     worst_frame(ctx, d, g);
-    return scratch_need(g);
+    return scratch_bound(g);
 }
 
 vaco_av1pf_status av1pf_reserve_staging(vaco_av1pf_ctx *ctx, size_t bytes)
@@ -1760,7 +1748,7 @@ vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
             if (r != VACO_AV1PF_OK)
                 return fail(ctx, r, msg);
         }
-        ws_need += scratch_need(geo[i]);
+        ws_need += scratch_need(descs[i], geo[i]);
         arena += arena_need(descs[i], geo[i]);
     }
     Scratch ws;
@@ -2044,7 +2032,7 @@ vaco_av1pf_status av1pf_frame_static(const vaco_av1pf_frame_desc *desc, const va
     vaco_av1pf_status st = check_geometry(g, desc->cdef_stride, in, out, msg);
     if (st != VACO_AV1PF_OK)
         return st;
-    const size_t inter = scratch_need(g) * sizeof(uint16_t);
+    const size_t inter = scratch_need(*desc, g) * sizeof(uint16_t);
     if (arena_need(*desc, g) > kStaticStageBytes || bytes < kStaticStageBytes ||
         inter > bytes - kStaticStageBytes) {
         *msg = "frame needs more device memory than the static scratch holds";

@@ -26,6 +26,7 @@
 #pragma once
 
 #include "av1pf_common.h"
+#include "av1pf_superres_body.h"
 
 namespace av1pf {
 
@@ -136,9 +137,105 @@ __device__ __forceinline__ void put(const TileOut &o, int r, int c, int v)
 // output. Both clamps together are one clamp to [start - 2, end + 2]. Every
 // thread issues all of its loads before it stores any of them; positions
 // past the window load a position inside it and are not stored.
+// Source window row of the folded superres staging (samples; 16-byte rows):
+// the taps of 70 output columns span at most 78 source samples, because the
+// step is below one source sample per output.
+constexpr int kSrWinStride = 88;
+
+// LDS the folded superres staging uses past the staged source: the source
+// window of every staged row, the same shifted by one sample, and the filter
+// table.
+template <int ROWS>
+constexpr int sr_stage_lds() { return 2 * (ROWS + 6) * kSrWinStride * 2 + 64 * 8 * 2; }
+
+// Source::get over the upscaled images with superres folded into the
+// staging: rows as stage_source picks them (pre-CDEF outside the stripe,
+// CDEF output inside, both before upscaling); the sample at plane column X
+// (clamped to the visible width) is upscale_plane's output there
+// (superres.rs:173-214): eight taps at (src_x >> 14) - 3 .., edge-clamped to
+// the padded width, weighted by the phase's filter, rounded by 7 bits and
+// clipped. The workgroup stages, per row, the union of its columns' taps
+// (the window lo .. hi, at most 78 samples) and a copy shifted by one sample
+// in `scratch` (sr_stage_lds bytes), so that a column's eight taps are four
+// aligned sample pairs summed with v_dot2; then each thread filters one
+// column over several rows (position, phase and filter depend on the column
+// only). Returns whether a source sample above maxv was read (the window
+// holds exactly the samples the taps read).
 template <int ROWS, int THREADS>
-__device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int maxv)
+__device__ __forceinline__ bool stage_source_sr(uint16_t *S, const Tile &t, int maxv, uint8_t *scratch)
 {
+    const LrPlane &P = *t.P;
+    const int W = P.width, H = P.height;
+    const int x0 = t.cs.x0, y0 = t.rs.y0, ss = t.rs.stripe_start, se = t.rs.stripe_end;
+    const int srows = t.rs.h + 6, scols = t.cs.w + 6;
+    const int step = P.sr_step, start = P.sr_start, last = (int)P.sr_in_width - 1;
+    uint16_t *W0 = reinterpret_cast<uint16_t *>(scratch);
+    uint16_t *W1 = W0 + (ROWS + 6) * kSrWinStride;  // window index i at i + 1
+    int16_t *tab = reinterpret_cast<int16_t *>(W1 + (ROWS + 6) * kSrWinStride);
+    const gptr<const uint16_t> before = glb(P.before), after = glb(P.after);
+    const uint32_t bst = P.before_stride, ast = P.after_stride;
+    auto row_of = [&](int r) {
+        const int yc = min(max(y0 - 3 + r, 0), H - 1);
+        const bool out = yc < ss || yc > se;
+        const int yr = min(max(yc, ss - 2), se + 2);
+        return (out ? before : after) + (size_t)(uint32_t)yr * (out ? bst : ast);  // 0 <= yr < H
+    };
+    // -(1 << 14) + initial_subpel_x + x * step of staged column c
+    auto pos = [&](int c) { return -(1 << 14) + start + min(max(x0 - 3 + c, 0), W - 1) * step; };
+    const int lo = (pos(0) >> 14) - 3;
+    const int wn = (pos(scols - 1) >> 14) + 4 - lo + 1;  // <= 78
+    const int tid = (int)threadIdx.x;
+    for (int i = tid; i < 64 * 8; i += THREADS)
+        tab[i] = kUpscale[i >> 3][i & 7];
+    // Window samples, every load issued before any is stored.
+    constexpr int kWin = 80, N = (ROWS + 6) * kWin, K = (N + THREADS - 1) / THREADS;
+    uint16_t q[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int e = min(tid + k * THREADS, N - 1);
+        const int r = min(e / kWin, srows - 1), i = min(e - (e / kWin) * kWin, wn - 1);
+        q[k] = row_of(r)[min(max(lo + i, 0), last)];
+    }
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int e = tid + k * THREADS;
+        const int r = e / kWin, i = e - r * kWin;
+        if (e >= N || r >= srows)
+            break;
+        if (i < wn) {
+            bad |= q[k] > maxv;
+            W0[r * kSrWinStride + i] = q[k];
+            W1[r * kSrWinStride + i + 1] = q[k];
+        }
+    }
+    __syncthreads();
+    constexpr int kC = 72, kR = THREADS / kC;  // column slots, row lanes
+    const int c = tid % kC, rl = tid / kC;
+    if (rl < kR && c < scols) {
+        const int p = pos(c);
+        const int b = (p >> 14) - 3 - lo;  // window index of the first tap: 0 <= b, b + 7 < wn
+        const uint4 fq = *reinterpret_cast<const uint4 *>(&tab[((p & 0x3fff) >> 8) * 8]);
+        const s16x2 f0 = __builtin_bit_cast(s16x2, fq.x), f1 = __builtin_bit_cast(s16x2, fq.y);
+        const s16x2 f2 = __builtin_bit_cast(s16x2, fq.z), f3 = __builtin_bit_cast(s16x2, fq.w);
+        const uint16_t *src = (b & 1) ? W1 + b + 1 : W0 + b;
+        for (int r = rl; r < srows; r += kR) {
+            const uint32_t *w = reinterpret_cast<const uint32_t *>(src + r * kSrWinStride);
+            int sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[0]), f0, 0, false);
+            sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[1]), f1, sum, false);
+            sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[2]), f2, sum, false);
+            sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, w[3]), f3, sum, false);
+            S[r * kS + c] = (uint16_t)min(max((sum + 64) >> 7, 0), maxv);
+        }
+    }
+    return bad;
+}
+
+template <int ROWS, int THREADS>
+__device__ __forceinline__ bool stage_source(uint16_t *S, const Tile &t, int maxv, uint8_t *scratch)
+{
+    if (t.P->sr_step)
+        return stage_source_sr<ROWS, THREADS>(S, t, maxv, scratch);
     const LrPlane &P = *t.P;
     const int W = P.width, H = P.height;
     const int x0 = t.cs.x0, y0 = t.rs.y0, ss = t.rs.stripe_start, se = t.rs.stripe_end;
@@ -239,8 +336,10 @@ struct LrShape {
     static constexpr int kWienerLds = kSrcBytes + (ROWS + 6) * 64 * 2;   // + row-filtered M
     static constexpr int kXBytes = (ROWS + 2) * kS * 3 / 2 * 4;          // VS + VQ, then A/B
     static constexpr int kSgrLds = kSrcBytes + kXBytes + 256 * 2;        // + x/(x+1) table
-    // The staged unit (UnitStage) sits past both paths' LDS.
-    static constexpr int kUnitLds = ((kSgrLds > kWienerLds ? kSgrLds : kWienerLds) + 15) & ~15;
+    static constexpr int kSrLds = kSrcBytes + sr_stage_lds<ROWS>();      // staging with superres folded in
+    static constexpr int kMax2 = kSgrLds > kWienerLds ? kSgrLds : kWienerLds;
+    // The staged unit (UnitStage) sits past every path's LDS.
+    static constexpr int kUnitLds = ((kMax2 > kSrLds ? kMax2 : kSrLds) + 15) & ~15;
     static constexpr int kLds = kUnitLds + 32;  // the workgroup's LDS
 };
 
@@ -308,7 +407,7 @@ __device__ __forceinline__ void lr_wiener(const Tile &t, uint8_t *lds)
     uint16_t *su = reinterpret_cast<uint16_t *>(lds + Sh::kUnitLds);
     UnitStage us;
     us.load(t);
-    const bool bad = stage_source<ROWS, T>(S, t, maxv);
+    const bool bad = stage_source<ROWS, T>(S, t, maxv, lds + Sh::kSrcBytes);
     us.store(su);
     if (__syncthreads_or(bad) && tid == 0)
         raise_status(P.status, kFlagSampleRange);
@@ -550,16 +649,19 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
     const int tw = t.cs.w, th = t.rs.h, x0 = t.cs.x0, y0 = t.rs.y0;
     const TileOut o = tile_out(t, x0, y0);
     const int tid = threadIdx.x;
-    for (int z = tid; z < 256; z += T)  // x / (x + 1) table, restoration.rs:330-336
-        xbx1[z] = z == 0 ? 1 : (uint16_t)(((z << 8) + z / 2) / (z + 1));
     uint16_t *su = reinterpret_cast<uint16_t *>(lds + Sh::kUnitLds);
     UnitStage us;
     us.load(t);
-    const bool bad = stage_source<ROWS, T>(S, t, maxv);
+    const bool bad = stage_source<ROWS, T>(S, t, maxv, lds + Sh::kSrcBytes);
     us.store(su);
     if (__syncthreads_or(bad) && tid == 0)
         raise_status(P.status, kFlagSampleRange);
     const LrUnit u = *reinterpret_cast<const LrUnit *>(su);
+    // x / (x + 1) table (restoration.rs:330-336), past the grids; written
+    // after the staging (whose superres window may cover it), read after the
+    // grid's first barrier.
+    for (int z = tid; z < 256; z += T)
+        xbx1[z] = z == 0 ? 1 : (uint16_t)(((z << 8) + z / 2) / (z + 1));
 
     // self_guided, restoration.rs:373-384, with guided_pass 341-371.
     const int set = u.set & 15;
@@ -611,12 +713,36 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
 }
 
 
+// RestorationUnit::None on a plane with superres folded in: the upscaled
+// CDEF samples of the tile (inside the stripe, so the staged rows are the
+// CDEF image's), staged as the filters stage them.
+template <int ROWS>
+__device__ __forceinline__ void lr_copy_sr(const Tile &t, uint8_t *lds)
+{
+    using Sh = LrShape<ROWS>;
+    constexpr int T = Sh::kThreads;
+    uint16_t *S = reinterpret_cast<uint16_t *>(lds);
+    const int maxv = (1 << t.bit_depth()) - 1;
+    const int tw = t.cs.w, th = t.rs.h;
+    const TileOut o = tile_out(t, t.cs.x0, t.rs.y0);
+    const bool bad = stage_source_sr<ROWS, T>(S, t, maxv, lds + Sh::kSrcBytes);
+    if (__syncthreads_or(bad) && threadIdx.x == 0)
+        raise_status(t.P->status, kFlagSampleRange);
+    for (int e = threadIdx.x; e < th * 64; e += T) {
+        const int r = e >> 6, c = e & 63;
+        if (c < tw)
+            put(o, r, c, S[(r + 3) * kS + c + 3]);
+    }
+}
+
 // One tile of any kind, in a workgroup of LrShape<ROWS>::kThreads threads
 // over `lds` (LrShape<ROWS>::kLds bytes).
 template <int ROWS>
 __device__ __forceinline__ void lr_tile(const Tile &t, uint8_t *lds)
 {
-    if (t.kind == kUnitSelfGuided)
+    if (t.kind == kUnitNone && t.P->sr_step)
+        lr_copy_sr<ROWS>(t, lds);
+    else if (t.kind == kUnitSelfGuided)
         lr_sgr<ROWS>(t, lds);
     else if (t.kind == kUnitWiener)
         lr_wiener<ROWS>(t, lds);
