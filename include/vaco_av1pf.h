@@ -419,8 +419,10 @@ vaco_av1pf_status vaco_av1pf_use_graphs(vaco_av1pf_ctx *ctx, int enable);
 typedef struct vaco_av1pf_tuning {
     uint8_t superres_rows;   /* rows per superres workgroup: 4, 8 or 16 */
     uint8_t cdef_xcd_order;  /* 1: superblocks numbered XCD-contiguously */
-    uint8_t lr_order;        /* 0: restoration tiles by unit type, self-guided first;
-                                1: in plane order, numbered XCD-contiguously */
+    uint8_t lr_order;        /* 0: restoration tiles by unit type, self-guided first
+                                (a tile list the host builds);
+                                1: in plane order, numbered XCD-contiguously (no
+                                list: each workgroup finds its tile) */
     uint8_t lr_tile_rows;    /* restoration tile height: 64, 32 or 16 rows */
     uint8_t cdef_waves;      /* waves per CDEF workgroup (one superblock): 16 or 8 */
     uint8_t chain;           /* 1: a single frame with CDEF and restoration but no

@@ -68,7 +68,8 @@ __device__ __forceinline__ bool wave_wait(Ready &&ready, uint32_t limit)
 
 template <int SX, int SY, bool XCD>
 __global__ __launch_bounds__(kChainThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_chain(
-    const CdefFrame single, const LrPlane *planes, const uint64_t *list, uint32_t n_sb, uint32_t lr_first,
+    const CdefFrame single, const LrPlane *planes, uint32_t n_items, const uint64_t *list, uint32_t n_sb,
+    uint32_t lr_first,
     uint32_t n_tiles, uint32_t lr_xcd, const uint4 *__restrict__ up_src, uint4 *up_dst, uint32_t up_n16,
     const ChainSync cs)
 {
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(kChainThreads) __attribute__((amdgpu_waves_per_eu(6
             raise_status(single.status, kFlagChainStall);
         return;
     }
-    const Tile t = fetch_tile(planes, list, wg);
+    const Tile t = fetch_tile(planes, n_items, list, wg);
 
     // Superblocks whose CDEF output the tile reads: Source::get's rows
     // (restoration.rs:233-244) inside the stripe, within 3 rows of the tile;
@@ -147,29 +148,31 @@ __global__ __launch_bounds__(kChainThreads) __attribute__((amdgpu_waves_per_eu(6
 }
 
 template <int SX, int SY>
-static void launch_chain_mode(const CdefFrame &f, bool xcd, const LrPlane *planes, const uint64_t *list,
+static void launch_chain_mode(const CdefFrame &f, bool xcd, const LrPlane *planes, uint32_t n_items,
+                              const uint64_t *list,
                               uint32_t n_sb, uint32_t lr_first, uint32_t n_tiles, uint32_t lr_xcd,
                               const uint4 *us, uint4 *ud, uint32_t n16, const ChainSync &cs, hipStream_t s,
                               hipEvent_t ev0, hipEvent_t ev1)
 {
     const dim3 g(lr_first + n_tiles), blk(kChainThreads);
     if (xcd)
-        hipExtLaunchKernelGGL((k_chain<SX, SY, true>), g, blk, 0, s, ev0, ev1, 0, f, planes, list, n_sb, lr_first,
-                              n_tiles, lr_xcd, us, ud, n16, cs);
-    else
-        hipExtLaunchKernelGGL((k_chain<SX, SY, false>), g, blk, 0, s, ev0, ev1, 0, f, planes, list, n_sb,
+        hipExtLaunchKernelGGL((k_chain<SX, SY, true>), g, blk, 0, s, ev0, ev1, 0, f, planes, n_items, list, n_sb,
                               lr_first, n_tiles, lr_xcd, us, ud, n16, cs);
+    else
+        hipExtLaunchKernelGGL((k_chain<SX, SY, false>), g, blk, 0, s, ev0, ev1, 0, f, planes, n_items, list,
+                              n_sb, lr_first, n_tiles, lr_xcd, us, ud, n16, cs);
 }
 
 // One frame's CDEF and restoration in one launch. f: the frame's CDEF
 // descriptor (side information read in place, as the single-frame CDEF
-// launch does); planes / list: its restoration items (one per plane, in
-// plane order) and n_tiles tile entries, in the device mirror that the
-// launch fills from up_src (up_bytes). mode, xcd: as launch_cdef; lr_xcd: the
+// launch does); planes / list: its n_items restoration items (one per
+// plane, in plane order) and n_tiles tile entries (no list: tiles in plane
+// order), in the device mirror that the launch fills from up_src (up_bytes). mode, xcd: as launch_cdef; lr_xcd: the
 // list is in plane order, numbered XCD-contiguously. cs: flags for the
 // frame's superblocks, the copy counter and its target, the epoch.
-hipError_t launch_chain(const CdefFrame &f, int mode, bool xcd, const LrPlane *planes, const uint64_t *list,
-                        uint32_t n_tiles, bool lr_xcd, const void *up_src, void *up_dst, size_t up_bytes,
+hipError_t launch_chain(const CdefFrame &f, int mode, bool xcd, const LrPlane *planes, uint32_t n_items,
+                        const uint64_t *list, uint32_t n_tiles, bool lr_xcd, const void *up_src, void *up_dst,
+                        size_t up_bytes,
                         const ChainSync &cs, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     const uint32_t n_sb = f.sb_cols * f.sb_rows;
@@ -181,16 +184,20 @@ hipError_t launch_chain(const CdefFrame &f, int mode, bool xcd, const LrPlane *p
     uint4 *ud = static_cast<uint4 *>(up_dst);
     switch (mode) {
     case 0:
-        launch_chain_mode<1, 1>(f, xcd, planes, list, n_sb, lr_first, n_tiles, lr_xcd, us, ud, n16, cs, s, ev0, ev1);
+        launch_chain_mode<1, 1>(f, xcd, planes, n_items, list, n_sb, lr_first, n_tiles, lr_xcd, us, ud, n16,
+                                cs, s, ev0, ev1);
         break;
     case 1:
-        launch_chain_mode<0, 0>(f, xcd, planes, list, n_sb, lr_first, n_tiles, lr_xcd, us, ud, n16, cs, s, ev0, ev1);
+        launch_chain_mode<0, 0>(f, xcd, planes, n_items, list, n_sb, lr_first, n_tiles, lr_xcd, us, ud, n16,
+                                cs, s, ev0, ev1);
         break;
     case 2:
-        launch_chain_mode<1, 0>(f, xcd, planes, list, n_sb, lr_first, n_tiles, lr_xcd, us, ud, n16, cs, s, ev0, ev1);
+        launch_chain_mode<1, 0>(f, xcd, planes, n_items, list, n_sb, lr_first, n_tiles, lr_xcd, us, ud, n16,
+                                cs, s, ev0, ev1);
         break;
     default:
-        launch_chain_mode<0, 1>(f, xcd, planes, list, n_sb, lr_first, n_tiles, lr_xcd, us, ud, n16, cs, s, ev0, ev1);
+        launch_chain_mode<0, 1>(f, xcd, planes, n_items, list, n_sb, lr_first, n_tiles, lr_xcd, us, ud, n16,
+                                cs, s, ev0, ev1);
         break;
     }
     return hipGetLastError();

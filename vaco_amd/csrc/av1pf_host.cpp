@@ -33,11 +33,12 @@ hipError_t launch_cdef(const CdefFrame *frames, int n, const CdefFrame *single, 
                        hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_superres(const SrJob *jobs, int n, uint32_t blocks, int rows, hipStream_t s,
                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-hipError_t launch_chain(const CdefFrame &f, int mode, bool xcd, const LrPlane *planes, const uint64_t *list,
-                        uint32_t n_tiles, bool lr_xcd, const void *up_src, void *up_dst, size_t up_bytes,
+hipError_t launch_chain(const CdefFrame &f, int mode, bool xcd, const LrPlane *planes, uint32_t n_items,
+                        const uint64_t *list, uint32_t n_tiles, bool lr_xcd, const void *up_src, void *up_dst,
+                        size_t up_bytes,
                         const ChainSync &cs, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
 uint32_t chain_copiers(uint32_t n_sb, size_t up_bytes);
-hipError_t launch_lr(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd, int rows,
+hipError_t launch_lr(const LrPlane *planes, uint32_t n_items, const uint64_t *list, uint32_t n, bool xcd, int rows,
                      hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s,
                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
@@ -57,7 +58,7 @@ constexpr int kSlots = 8;
 constexpr uint32_t kMaxBatch = 256;  // frames per batched call
 constexpr uint32_t kMaxSuperresWidth = 65536;  // k_superres's 32-bit source positions
 // Launch variants of a new context (vaco_av1pf_autotune may change them).
-constexpr vaco_av1pf_tuning kDefaultTuning = {8, 1, 0, 64, 16, 0};
+constexpr vaco_av1pf_tuning kDefaultTuning = {8, 1, 1, 64, 16, 0};
 constexpr const char *kChainStallMsg =
     "a restoration tile of a chained launch gave up waiting for its CDEF superblocks";
 // Polls (each ~0.1-0.3 us) a chained launch's restoration tile makes before
@@ -747,6 +748,11 @@ struct Plan {
     };  // w x h samples; dst.u8: to bytes
     std::vector<Copy> copies;  // frames in which no stage runs
     bool superres = false;  // some frame is upscaled (restoration then upscales inside its staging)
+    // Restoration units stay in the slot's host memory (read in place by the
+    // restoration tiles, as CDEF reads its side information): batches, whose
+    // deep grids hide that latency, so that their upload carries descriptors
+    // only.
+    bool host_units = false;
     vaco_av1pf_tuning tune = kDefaultTuning;  // launch variants this plan is built for
     // algorithmic bytes per kernel id (samples read once + written once, 2 B)
     uint64_t bytes[VACO_AV1PF_KERNEL_COUNT] = {};
@@ -936,7 +942,11 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             memcpy(hcs, cols->data(), cols->size() * sizeof(LrColSeg));
             if (!none) {
                 static_assert(sizeof(LrUnit) == sizeof(vaco_av1pf_lr_unit), "unit layout");
-                TAKE(hu, du, LrUnit, d.lr_unit_count[p]);
+                LrUnit *du;
+                LrUnit *hu = plan.host_units ? a->take_host<LrUnit>(d.lr_unit_count[p], &du)
+                                             : a->take<LrUnit>(d.lr_unit_count[p], &du);
+                if (!hu)
+                    return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "staging arena exhausted");
                 memcpy(hu, d.lr_units[p], sizeof(LrUnit) * d.lr_unit_count[p]);
                 L.units = du;
                 uint32_t ucols, urows;
@@ -970,20 +980,26 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
             L.out_vec = !((uintptr_t)L.out & 7) && !(L.out_stride & 3);
             plan.lr_blocks += L.n_rows * L.n_cols;
             const uint32_t item = (uint32_t)plan.lr.size();
-            for (uint32_t ri = 0; ri < L.n_rows; ri++)
-                for (uint32_t ci = 0; ci < L.n_cols; ci++) {
-                    int kind = 0;
-                    if (!none) {
-                        const uint8_t k =
-                            d.lr_units[p][(*rows)[ri].unit_row * L.unit_cols + (*cols)[ci].unit_col].kind;
-                        kind = k == VACO_AV1PF_UNIT_WIENER ? 1 : k == VACO_AV1PF_UNIT_SELF_GUIDED ? 2 : 0;
+            uint64_t rows_h = 0, cols_w = 0;
+            for (const LrRowSeg &r : *rows)
+                rows_h += (uint64_t)r.h;
+            for (const LrColSeg &c : *cols)
+                cols_w += (uint64_t)c.w;
+            plan.bytes[VACO_AV1PF_KERNEL_RESTORATION] += rows_h * cols_w * 4;
+            // Tiles in plane order (lr_order 1) need no list: the kernel finds
+            // each workgroup's item and tile (fetch_tile). By kind: three lists.
+            if (plan.tune.lr_order == 0)
+                for (uint32_t ri = 0; ri < L.n_rows; ri++)
+                    for (uint32_t ci = 0; ci < L.n_cols; ci++) {
+                        int kind = 0;
+                        if (!none) {
+                            const uint8_t k =
+                                d.lr_units[p][(*rows)[ri].unit_row * L.unit_cols + (*cols)[ci].unit_col].kind;
+                            kind = k == VACO_AV1PF_UNIT_WIENER ? 1 : k == VACO_AV1PF_UNIT_SELF_GUIDED ? 2 : 0;
+                        }
+                        plan.lr_tiles[kind].push_back((uint64_t)(ri * L.n_cols + ci) << 32 | (uint64_t)item << 2 |
+                                                      (uint64_t)kind);
                     }
-                    // plane order (one list) or by kind (three lists)
-                    plan.lr_tiles[plan.tune.lr_order ? 0 : kind].push_back(
-                        (uint64_t)(ri * L.n_cols + ci) << 32 | (uint64_t)item << 2 | (uint64_t)kind);
-                    plan.bytes[VACO_AV1PF_KERNEL_RESTORATION] +=
-                        (uint64_t)(*rows)[ri].h * (*cols)[ci].w * 4;
-                }
             plan.lr.push_back(L);
         }
     }
@@ -1037,13 +1053,18 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
         TAKE(h, d, LrPlane, plan.lr.size());
         memcpy(h, plan.lr.data(), plan.lr.size() * sizeof(LrPlane));
         dl = d;
-        size_t n = 0;
-        for (int k = 0; k < 3; k++)
-            n += nlist[k] = (uint32_t)plan.lr_tiles[k].size();
-        TAKE(hl, dlk, uint64_t, n);
-        lr_tile_list(plan.lr_tiles, plan.tune.lr_order == 0, hl);
-        dlist = dlk;
+        if (plan.tune.lr_order == 0) {  // by kind: the list
+            size_t n = 0;
+            for (int k = 0; k < 3; k++)
+                n += nlist[k] = (uint32_t)plan.lr_tiles[k].size();
+            TAKE(hl, dlk, uint64_t, n);
+            lr_tile_list(plan.lr_tiles, true, hl);
+            dlist = dlk;
+        } else {  // plane order: no list
+            nlist[0] = plan.lr_blocks;
+        }
     }
+    const uint32_t n_lr_items = (uint32_t)plan.lr.size();
     if (!fused_upload) {
         const vaco_av1pf_status st = upload_slot(ctx, a, s);
         if (st != VACO_AV1PF_OK)
@@ -1086,7 +1107,8 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
         HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_CHAIN,
                               plan.cdef_bytes[m] / 2 + plan.bytes[VACO_AV1PF_KERNEL_RESTORATION] / 2, s,
                               [&](hipEvent_t e0, hipEvent_t e1) {
-                                  return launch_chain(single, m, plan.tune.cdef_xcd_order != 0, dl, dlist, n_tiles,
+                                  return launch_chain(single, m, plan.tune.cdef_xcd_order != 0, dl, n_lr_items,
+                                                      dlist, n_tiles,
                                                       plan.tune.lr_order != 0, a->hd, a->d, up_bytes, cs, s, e0,
                                                       e1);
                               }));
@@ -1115,7 +1137,7 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
             HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_RESTORATION,
                                   plan.bytes[VACO_AV1PF_KERNEL_RESTORATION], s,
                                   [&](hipEvent_t e0, hipEvent_t e1) {
-                                      return launch_lr(dl, dlist, n_tiles, plan.tune.lr_order != 0,
+                                      return launch_lr(dl, n_lr_items, dlist, n_tiles, plan.tune.lr_order != 0,
                                                        plan.tune.lr_tile_rows, s, e0, e1);
                                   }));
         return VACO_AV1PF_OK;
@@ -1131,9 +1153,9 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
     for (int m = 0; m < 4; m++)
         key.insert(key.end(), {(uint64_t)(uintptr_t)dc[m], plan.cdef_blocks[m], plan.cdef[m].size()});
     key.insert(key.end(), {(uint64_t)(uintptr_t)dj, plan.sr_blocks, plan.sr.size(),
-                           (uint64_t)(uintptr_t)dl, (uint64_t)(uintptr_t)dlist, n_tiles,
+                           (uint64_t)(uintptr_t)dl, (uint64_t)(uintptr_t)dlist, n_tiles, n_lr_items,
                            plan.tune.superres_rows, plan.tune.cdef_xcd_order, plan.tune.lr_order,
-                           plan.tune.cdef_waves});
+                           plan.tune.cdef_waves, plan.tune.lr_tile_rows});
     hipGraphExec_t exec = nullptr;
     for (auto &gr : a->graphs)
         if (gr.key == key)
@@ -1791,6 +1813,7 @@ vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
         return st;
     Plan plan;
     plan.tune = ctx->tune;
+    plan.host_units = true;
     for (uint32_t i = 0; i < n; i++) {
         DevPlane din[3], dout[3];
         for (int p = 0; p < geo[i].nplanes; p++) {
@@ -1946,10 +1969,11 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
     // One field at a time: the current value is timed first, and another
     // replaces it only when at least 2% faster, so that timing noise between
     // equivalent variants does not flip the choice. A value listed in
-    // `costly` moves more HBM bytes than the others (the plain CDEF order
-    // reads each superblock's halo lines into a different L2 than its
-    // neighbours: 2.3x the algorithmic bytes against 1.1x), so it must be at
-    // least 5% faster. A kernel absent from the batch times 0 and keeps its
+    // `costly` costs more outside the kernel's time than the others (the
+    // plain CDEF order reads each superblock's halo lines into a different L2
+    // than its neighbours: 2.3x the algorithmic bytes against 1.1x; the
+    // by-kind restoration order makes the host build a tile list), so it
+    // must be at least 5% faster. A kernel absent from the batch times 0 and keeps its
     // value.
     auto tune_field = [&](uint8_t vaco_av1pf_tuning::*field, std::initializer_list<uint8_t> values,
                           int kernel, int costly = -1) {
@@ -1974,7 +1998,10 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
     tune_field(&vaco_av1pf_tuning::superres_rows, {4, 8, 16}, VACO_AV1PF_KERNEL_SUPERRES);
     tune_field(&vaco_av1pf_tuning::cdef_xcd_order, {0, 1}, VACO_AV1PF_KERNEL_CDEF, 0);
     tune_field(&vaco_av1pf_tuning::cdef_waves, {16, 8}, VACO_AV1PF_KERNEL_CDEF);
-    tune_field(&vaco_av1pf_tuning::lr_order, {0, 1}, VACO_AV1PF_KERNEL_RESTORATION);
+    // By kind, the host builds the tile list (a pass over every tile); for a
+    // batch that host time rivals the kernels', so plane order (no list)
+    // yields only to a clearly faster kernel.
+    tune_field(&vaco_av1pf_tuning::lr_order, {0, 1}, VACO_AV1PF_KERNEL_RESTORATION, n > 1 ? 0 : -1);
     tune_field(&vaco_av1pf_tuning::lr_tile_rows, {64, 32, 16}, VACO_AV1PF_KERNEL_RESTORATION);
     // One frame: the chained launch against the separate ones (every kernel
     // of the call, summed; the chain also saves the gap between launches,
@@ -1987,7 +2014,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
         t.chain = 0;
         if ((st = measure(t, -1, &sep_ms)) == VACO_AV1PF_OK) {
             t.chain = 1;
-            st = measure(t, -1, &chain_ms);
+            // 0 when the frame does not chain (no CDEF, no restoration, or superres)
+            st = measure(t, VACO_AV1PF_KERNEL_CHAIN, &chain_ms);
         }
         if (st == VACO_AV1PF_OK) {
             best.chain = chain_ms > 0 && chain_ms <= sep_ms ? 1 : 0;
@@ -2305,12 +2333,13 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
                            ctx->stream);
     if (e == hipSuccess && n_units)
         e = hipMemcpyAsync(du, units, n_units * sizeof(LrUnit), hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess && !list.empty())
+    if (e == hipSuccess && !list.empty() && tune.lr_order == 0)
         e = hipMemcpyAsync(dlist, list.data(), list.size() * 8, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(dst, &zero, 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-        e = launch_lr(dl, dlist, nlist[0] + nlist[1] + nlist[2], tune.lr_order != 0, tune.lr_tile_rows,
+        e = launch_lr(dl, 1, tune.lr_order == 0 ? dlist : nullptr, nlist[0] + nlist[1] + nlist[2],
+                      tune.lr_order != 0, tune.lr_tile_rows,
                       ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpy2DAsync(out->data, out->stride * 2, dout, cfg->width * 2, cfg->width * 2,

@@ -25,45 +25,48 @@
 
 namespace av1pf {
 
-// One launch over every tile of the plan, one workgroup per list entry.
-// XCD: workgroup b takes entry xcd_remap(b) (for lists in plane order).
-// ROWS: no Wiener or self-guided tile is taller (copy tiles, which use no
-// LDS, may be up to 64 rows).
+// One launch over every tile of the plan, one workgroup per tile: list
+// entry b, or without a list tile b of the n_items plane items in plane
+// order (fetch_tile). XCD: workgroup b takes tile xcd_remap(b) (tiles in
+// plane order). ROWS: no Wiener or self-guided tile is taller (copy tiles,
+// which use no LDS, may be up to 64 rows).
 template <bool XCD, int ROWS>
 __global__ __launch_bounds__(LrShape<ROWS>::kThreads) void k_lr(const LrPlane *__restrict__ planes,
+                                                               uint32_t n_items,
                                                                const uint64_t *__restrict__ list)
 {
     __shared__ __align__(16) uint8_t lds[LrShape<ROWS>::kLds];
     const uint32_t wg = XCD ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-    lr_tile<ROWS>(fetch_tile(planes, list, wg), lds);
+    lr_tile<ROWS>(fetch_tile(planes, n_items, list, wg), lds);
 }
 
 template <int ROWS>
-static void launch_lr_rows(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd, hipStream_t s,
-                           hipEvent_t ev0, hipEvent_t ev1)
+static void launch_lr_rows(const LrPlane *planes, uint32_t n_items, const uint64_t *list, uint32_t n, bool xcd,
+                           hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     const dim3 g(n), b(LrShape<ROWS>::kThreads);
     if (xcd)
-        hipExtLaunchKernelGGL((k_lr<true, ROWS>), g, b, 0, s, ev0, ev1, 0, planes, list);
+        hipExtLaunchKernelGGL((k_lr<true, ROWS>), g, b, 0, s, ev0, ev1, 0, planes, n_items, list);
     else
-        hipExtLaunchKernelGGL((k_lr<false, ROWS>), g, b, 0, s, ev0, ev1, 0, planes, list);
+        hipExtLaunchKernelGGL((k_lr<false, ROWS>), g, b, 0, s, ev0, ev1, 0, planes, n_items, list);
 }
 
-// n tiles; xcd: list in plane order, numbered XCD-contiguously. rows: the
-// tallest Wiener or self-guided tile (64, 32 or 16; the workgroup takes 8
-// threads per row). ev0/ev1 (may be null) receive the dispatch's start and
-// end.
-hipError_t launch_lr(const LrPlane *planes, const uint64_t *list, uint32_t n, bool xcd, int rows,
-                     hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+// n tiles of n_items plane items; list: the tiles in launch order, or null
+// for plane order; xcd: tiles in plane order, numbered XCD-contiguously.
+// rows: the tallest Wiener or self-guided tile (64, 32 or 16; the workgroup
+// takes 8 threads per row). ev0/ev1 (may be null) receive the dispatch's
+// start and end.
+hipError_t launch_lr(const LrPlane *planes, uint32_t n_items, const uint64_t *list, uint32_t n, bool xcd,
+                     int rows, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     if (n == 0)
         return hipSuccess;
     if (rows <= 16)
-        launch_lr_rows<16>(planes, list, n, xcd, s, ev0, ev1);
+        launch_lr_rows<16>(planes, n_items, list, n, xcd, s, ev0, ev1);
     else if (rows <= 32)
-        launch_lr_rows<32>(planes, list, n, xcd, s, ev0, ev1);
+        launch_lr_rows<32>(planes, n_items, list, n, xcd, s, ev0, ev1);
     else
-        launch_lr_rows<64>(planes, list, n, xcd, s, ev0, ev1);
+        launch_lr_rows<64>(planes, n_items, list, n, xcd, s, ev0, ev1);
     return hipGetLastError();
 }
 

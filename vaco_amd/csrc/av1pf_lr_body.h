@@ -69,19 +69,34 @@ static_assert(offsetof(LrPlane, bit_depth) % 4 == 0 &&
               offsetof(LrPlane, out_vec) + 4 <= sizeof(LrPlane),
               "LrPlane byte fields are read as two words");
 
-__device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, const uint64_t *list, uint32_t wg)
+// The tile of workgroup wg: list entry wg, or, without a list (tiles in
+// plane order), tile wg of the n_items plane items: the item whose
+// block_begin <= wg, its tile wg - block_begin in row-major order of its row
+// and column segments, with the kind of the tile's unit.
+__device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, uint32_t n_items, const uint64_t *list,
+                                           uint32_t wg)
 {
-    const uint64_t e = list[wg];
     Tile t;
-    t.kind = (int)(e & 3u);
-    t.P = &planes[(uint32_t)e >> 2];
+    uint32_t local;
+    if (list) {
+        const uint64_t e = list[wg];
+        t.kind = (int)(e & 3u);
+        t.P = &planes[(uint32_t)e >> 2];
+        local = (uint32_t)(e >> 32);
+    } else {
+        t.P = &find_item(planes, (int)n_items, wg);
+        local = wg - t.P->block_begin;
+    }
     __builtin_memcpy(&t.b0, &t.P->bit_depth, 4);
     __builtin_memcpy(&t.b1, &t.P->out_vec, 4);
-    const uint32_t local = (uint32_t)(e >> 32);
     const uint32_t rsi = local / t.P->n_cols;
     t.rs = glb(t.P->rows)[rsi];
     t.cs = glb(t.P->cols)[local - rsi * t.P->n_cols];
     t.unit_ix = t.rs.unit_row * t.P->unit_cols + t.cs.unit_col;
+    if (!list) {
+        const uint8_t k = t.P->units ? glb(t.P->units)[t.unit_ix].kind : kUnitNone;
+        t.kind = k == kUnitWiener ? kUnitWiener : k == kUnitSelfGuided ? kUnitSelfGuided : kUnitNone;
+    }
     return t;
 }
 
