@@ -170,7 +170,8 @@ constexpr int sr_stage_lds() { return 2 * (ROWS + 6) * kSrWinStride * 2 + 64 * 8
 // (superres.rs:173-214): eight taps at (src_x >> 14) - 3 .., edge-clamped to
 // the padded width, weighted by the phase's filter, rounded by 7 bits and
 // clipped. The workgroup stages, per row, the union of its columns' taps
-// (the window lo .. hi, at most 78 samples) and a copy shifted by one sample
+// (the window lo .. hi, at most 78 samples, from at most 3 samples before
+// lo) and a copy shifted by one sample
 // in `scratch` (sr_stage_lds bytes), so that a column's eight taps are four
 // aligned sample pairs summed with v_dot2; then each thread filters one
 // column over several rows (position, phase and filter depend on the column
@@ -197,31 +198,72 @@ __device__ __forceinline__ bool stage_source_sr(uint16_t *S, const Tile &t, int 
     };
     // -(1 << 14) + initial_subpel_x + x * step of staged column c
     auto pos = [&](int c) { return -(1 << 14) + start + min(max(x0 - 3 + c, 0), W - 1) * step; };
-    const int lo = (pos(0) >> 14) - 3;
-    const int wn = (pos(scols - 1) >> 14) + 4 - lo + 1;  // <= 78
+    const int lo = (pos(0) >> 14) - 3, hi = (pos(scols - 1) >> 14) + 4;  // hi - lo <= 77
     const int tid = (int)threadIdx.x;
     for (int i = tid; i < 64 * 8; i += THREADS)
         tab[i] = kUpscale[i >> 3][i & 7];
-    // Window samples, every load issued before any is stored.
-    constexpr int kWin = 80, N = (ROWS + 6) * kWin, K = (N + THREADS - 1) / THREADS;
-    uint16_t q[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        const int e = min(tid + k * THREADS, N - 1);
-        const int r = min(e / kWin, srows - 1), i = min(e - (e / kWin) * kWin, wn - 1);
-        q[k] = row_of(r)[min(max(lo + i, 0), last)];
-    }
+    // Window samples lo .. hi of every staged row (window index = sample -
+    // org), every load of a thread issued before any is stored. Inside the
+    // padded width with aligned rows, 8-byte loads of four samples from org =
+    // lo rounded down to a multiple of 4 (at most 21 per row); else one
+    // edge-clamped sample per load from org = lo.
     bool bad = false;
+    const int org4 = lo & ~3, groups = (hi - org4) / 4 + 1;
+    const bool vec = t.vec_ok() && lo >= 0 && org4 + 4 * groups - 1 <= last;
+    const int org = vec ? org4 : lo;
+    if (vec) {
+        constexpr int kG4 = 21, N = (ROWS + 6) * kG4, K = (N + THREADS - 1) / THREADS;
+        uint2 q[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        const int e = tid + k * THREADS;
-        const int r = e / kWin, i = e - r * kWin;
-        if (e >= N || r >= srows)
-            break;
-        if (i < wn) {
-            bad |= q[k] > maxv;
-            W0[r * kSrWinStride + i] = q[k];
-            W1[r * kSrWinStride + i + 1] = q[k];
+        for (int k = 0; k < K; k++) {
+            const int e = min(tid + k * THREADS, N - 1);
+            const int r = min(e / kG4, srows - 1), g = min(e - (e / kG4) * kG4, groups - 1);
+            const uint64_t w = *reinterpret_cast<gptr<const uint64_t>>(row_of(r) + org + 4 * g);
+            q[k] = {(uint32_t)w, (uint32_t)(w >> 32)};
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int e = tid + k * THREADS;
+            const int r = e / kG4, g = e - r * kG4;
+            if (e >= N || r >= srows)
+                break;
+            if (g < groups) {
+                const uint2 w = q[k];
+                const int s0 = org + 4 * g;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int v = (int)(((u < 2 ? w.x : w.y) >> (16 * (u & 1))) & 0xffffu);
+                    bad |= s0 + u >= lo && s0 + u <= hi && v > maxv;
+                }
+                uint16_t *d0 = static_cast<uint16_t *>(__builtin_assume_aligned(W0 + r * kSrWinStride + 4 * g, 8));
+                *reinterpret_cast<uint2 *>(d0) = w;
+                uint16_t *d1 = W1 + r * kSrWinStride + 4 * g + 1;  // window index i at i + 1
+                d1[0] = (uint16_t)w.x;
+                *reinterpret_cast<uint32_t *>(d1 + 1) = (w.x >> 16) | (w.y << 16);
+                d1[3] = (uint16_t)(w.y >> 16);
+            }
+        }
+    } else {
+        const int wn = hi - lo + 1;  // <= 78
+        constexpr int kWin = 80, N = (ROWS + 6) * kWin, K = (N + THREADS - 1) / THREADS;
+        uint16_t q[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int e = min(tid + k * THREADS, N - 1);
+            const int r = min(e / kWin, srows - 1), i = min(e - (e / kWin) * kWin, wn - 1);
+            q[k] = row_of(r)[min(max(lo + i, 0), last)];
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int e = tid + k * THREADS;
+            const int r = e / kWin, i = e - r * kWin;
+            if (e >= N || r >= srows)
+                break;
+            if (i < wn) {
+                bad |= q[k] > maxv;
+                W0[r * kSrWinStride + i] = q[k];
+                W1[r * kSrWinStride + i + 1] = q[k];
+            }
         }
     }
     __syncthreads();
@@ -229,7 +271,7 @@ __device__ __forceinline__ bool stage_source_sr(uint16_t *S, const Tile &t, int 
     const int c = tid % kC, rl = tid / kC;
     if (rl < kR && c < scols) {
         const int p = pos(c);
-        const int b = (p >> 14) - 3 - lo;  // window index of the first tap: 0 <= b, b + 7 < wn
+        const int b = (p >> 14) - 3 - org;  // window index of the first tap: 0 <= b, b + 7 <= hi - org
         const uint4 fq = *reinterpret_cast<const uint4 *>(&tab[((p & 0x3fff) >> 8) * 8]);
         const s16x2 f0 = __builtin_bit_cast(s16x2, fq.x), f1 = __builtin_bit_cast(s16x2, fq.y);
         const s16x2 f2 = __builtin_bit_cast(s16x2, fq.z), f3 = __builtin_bit_cast(s16x2, fq.w);
