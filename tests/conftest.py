@@ -1,10 +1,11 @@
 import os
 import sys
 
-# The serving loop overlaps its batches on the device only when each of the
-# process's streams has a hardware queue of its own; the runtime's pool holds
-# GPU_MAX_HW_QUEUES of them (4 by default), read when the runtime starts.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# Calls on several streams (per-frame calls, the serving loop's batches)
+# overlap on the device only when each of the process's streams has a
+# hardware queue of its own; the runtime's pool holds GPU_MAX_HW_QUEUES of
+# them (4 by default), read when the runtime starts.
+os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:

@@ -293,9 +293,10 @@ def test_gpu_frame_device_entry_matches_host_entry(ctx):
 
 
 def test_gpu_calls_alternating_between_two_caller_streams_match_oracle(ctx):
-    """Per-frame and batched calls share the context's intermediate images;
-    calls alternating between two caller streams must still each equal the
-    oracle (a call on a new stream waits for the previous call's work)."""
+    """Calls alternating between two caller streams overlap on the device
+    (each stream's per-frame calls get an intermediate set of their own, and a
+    set or the batch workspace last used on the other stream is waited for);
+    each must still equal the oracle."""
     from vaco_amd import hip
     say("two caller streams: per-frame and batched calls alternating")
     frames = [av1pf.generate(workloads.cfg1(), k) for k in range(2)]
@@ -318,6 +319,37 @@ def test_gpu_calls_alternating_between_two_caller_streams_match_oracle(ctx):
     for rep in range(3):
         for k in range(len(frames)):
             assert_frame_equal([t.download() for t in outs[rep][k]], want[k], f"stream call {rep}/{k}")
+    for st in streams:
+        st.destroy()
+
+
+def test_gpu_calls_on_three_streams_with_chained_launches_match_oracle(ctx):
+    """Three caller streams over two intermediate sets (so sets change hands
+    between streams) with the chained launch on (chained launches order
+    themselves across streams) and superres frames among them."""
+    from vaco_amd import hip
+    say("three caller streams, chained launches")
+    frames = [av1pf.generate(workloads.cfg0(), k) for k in range(3)]
+    frames += [F.random_frame(**c) for c in (dict(seed=14, coded_w=200, coded_h=130, bd=10, unit=128, uv_unit=64),
+                                             dict(seed=15, coded_w=93, coded_h=75, upscaled_w=160),
+                                             dict(seed=16, coded_w=136, coded_h=72, ssx=1, ssy=0))]
+    want = [O.frame(f.desc, f.planes) for f in frames]
+    streams = [hip.Stream() for _ in range(3)]
+    dev_in = [[hip.DeviceArray.from_host(p) for p in f.planes] for f in frames]
+    outs = [[[hip.DeviceArray(g) for g in f.output_geometry()] for f in frames] for _ in range(4)]
+    ptr = lambda arrs: [t.ptr for t in arrs]
+    saved = ctx.tuning
+    ctx.tuning = {"chain": 1}
+    try:
+        for rep in range(4):
+            for k, f in enumerate(frames):
+                ctx.frame_device(f, ptr(dev_in[k]), ptr(outs[rep][k]), stream=streams[(k + rep) % 3].handle)
+        ctx.finish()
+    finally:
+        ctx.tuning = saved
+    for rep in range(4):
+        for k in range(len(frames)):
+            assert_frame_equal([t.download() for t in outs[rep][k]], want[k], f"3 streams {rep}/{k}")
     for st in streams:
         st.destroy()
 

@@ -84,7 +84,6 @@ struct Arena {
     // outputs visible to any agent). `done_ev` is the one last recorded.
     hipEvent_t done = nullptr, done_sys = nullptr, done_ev = nullptr;
     hipStream_t stream = nullptr;  // the stream done_ev was recorded on
-    hipEvent_t up_done = nullptr;  // its upload (on the context's upload stream) has finished
     bool pending = false;
     // h is ordinary (pageable) host memory: the upload is a runtime copy, not
     // k_upload's direct read of coherent pinned memory
@@ -138,14 +137,36 @@ struct Arena {
 
 }  // namespace
 
+namespace {
+
+// A buffer that calls of a context share (a set of per-frame intermediates,
+// the batch workspace, the chained launch's counters): the stream of its last
+// user and the event that ends that use. A call on another stream waits for
+// that event on the device before using the buffer; calls on one stream are
+// ordered by the stream.
+struct Use {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+};
+
+constexpr int kSets = 2;                 // per-frame intermediate sets
+constexpr int kSetPlanes = 6;            // CDEF image + upscaled CDEF image, 3 planes each
+constexpr int kPoolPlanes = 6 + kSets * kSetPlanes;
+
+}  // namespace
+
 struct vaco_av1pf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t up_stream = nullptr;  // side-information uploads, ahead of the work
     uint32_t max_w = 0, max_h = 0;
     size_t plane_cap = 0;          // samples per plane buffer
-    uint16_t *pool = nullptr;      // 5 images x 3 planes
-    uint16_t *img[5][3] = {};      // 0 input, 1 cdef, 2 upscaled, 3 upscaled pre-CDEF, 4 output
+    uint16_t *pool = nullptr;      // kPoolPlanes planes
+    uint16_t *img_in[3] = {}, *img_out[3] = {};  // the host entry's input and output
+    // Intermediates of per-frame calls (CDEF image, upscaled CDEF image):
+    // two sets, so that calls on two streams overlap on the device.
+    uint16_t *set_img[kSets] = {};
+    Use set_use[kSets];
+    int next_set = 0;
     Arena slot[kSlots];
     int next_slot = 0;
     uint32_t *h_status = nullptr;  // coherent pinned status words, one per slot
@@ -156,12 +177,9 @@ struct vaco_av1pf_ctx {
     // device intermediates of batched calls (grown on demand)
     uint16_t *ws = nullptr;
     size_t ws_cap = 0;  // samples
-    // The last call that used the context's images or batch workspace: the
-    // stream it ran on and the event that ends it. A call on another stream
-    // waits for it first, so calls on different streams never overlap on
-    // those buffers.
-    hipStream_t last_stream = nullptr;
-    hipEvent_t last_done = nullptr;
+    // The batch workspace's and the chained launches' last use (a chained
+    // launch's counters assume the previous one has completed).
+    Use ws_use, chain_use;
     // Chained launches (tuning.chain): per-superblock completion flags and
     // the staged-bytes copy counter (device memory, zeroed at creation,
     // never reset); the counter's value once every chained launch so far has
@@ -286,15 +304,12 @@ template <class F>
 static hipError_t timed_launch(vaco_av1pf_ctx *ctx, int kernel, uint64_t bytes, hipStream_t s,
                                F launch);
 
-// Copy the slot's staged bytes to its device mirror. Up to kInlineUpload
-// bytes (one frame's side information) the copy runs on stream s itself,
-// ahead of the call's kernels: a few microseconds of GPU time, against the
-// host cost of an event record and a cross-stream wait per call. Larger
-// uploads (batches) run on the upload stream, which is otherwise idle, so
-// they overlap the previous call's kernels; stream s waits for them. The
-// slot's previous frame has finished (take_slot), so its mirror is free.
-constexpr size_t kInlineUpload = 512 * 1024;
-
+// Copy the slot's staged bytes (descriptors; the side information the
+// kernels read in place stays in host memory) to its device mirror, on the
+// call's stream s ahead of its kernels: tens of microseconds at most for a
+// large batch, against a cross-stream event hand-off, which cost as much and
+// needed a hardware queue of its own. The slot's previous frame has finished
+// (take_slot), so its mirror is free.
 static vaco_av1pf_status upload_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t s)
 {
     if (a->used == 0)
@@ -303,48 +318,54 @@ static vaco_av1pf_status upload_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t 
         HIPCHECK(hipMemcpyAsync(a->d, a->h, a->used, hipMemcpyHostToDevice, s));
         return VACO_AV1PF_OK;
     }
-    if (a->used <= kInlineUpload) {
-        HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_UPLOAD, (uint64_t)a->used * 2, s,
-                              [&](hipEvent_t e0, hipEvent_t e1) {
-                                  return launch_upload(a->d, a->hd, a->used, s, e0, e1);
-                              }));
-        return VACO_AV1PF_OK;
-    }
-    hipStream_t u = ctx->up_stream;
-    HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_UPLOAD, (uint64_t)a->used * 2, u,
-                          [&](hipEvent_t e0, hipEvent_t e1) {
-                              return launch_upload(a->d, a->hd, a->used, u, e0, e1);
-                          }));
-    HIPCHECK(hipEventRecord(a->up_done, u));
-    HIPCHECK(hipStreamWaitEvent(s, a->up_done, 0));
+    HIPCHECK(timed_launch(ctx, VACO_AV1PF_KERNEL_UPLOAD, (uint64_t)a->used * 2, s,
+                          [&](hipEvent_t e0, hipEvent_t e1) { return launch_upload(a->d, a->hd, a->used, s, e0, e1); }));
     return VACO_AV1PF_OK;
 }
 
 // Marks the end of the slot's frames on stream s; host_visible: the event is
 // handed on as the frames' completion (the server), so it carries the
-// system-scope fence. shared: the call used the context's own images or
-// batch workspace, so later calls on other streams must order after it.
-static vaco_av1pf_status close_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t s,
-                                    bool host_visible = false, bool shared = true)
+// system-scope fence. u0, u1 (may be null): the shared buffers the call used,
+// whose last use it now is.
+static vaco_av1pf_status close_slot(vaco_av1pf_ctx *ctx, Arena *a, hipStream_t s, bool host_visible = false,
+                                    Use *u0 = nullptr, Use *u1 = nullptr)
 {
     a->done_ev = host_visible ? a->done_sys : a->done;
     a->stream = s;
     HIPCHECK(hipEventRecord(a->done_ev, s));
     a->pending = true;
-    if (shared) {
-        ctx->last_stream = s;
-        ctx->last_done = a->done_ev;
-    }
+    for (Use *u : {u0, u1})
+        if (u) {
+            u->stream = s;
+            u->done = a->done_ev;
+        }
     return VACO_AV1PF_OK;
 }
 
-// A call about to use the context's images or batch workspace on stream s:
-// if the previous such call ran on another stream, s waits for its end.
-static vaco_av1pf_status order_after_previous(vaco_av1pf_ctx *ctx, hipStream_t s)
+// A call about to use shared buffer u on stream s: if its last use was on
+// another stream, s waits for that use's end. (A slot's event is recorded
+// again only once its previous work has completed, take_slot, so a newer
+// record never hides an older use.)
+static vaco_av1pf_status acquire(vaco_av1pf_ctx *ctx, const Use &u, hipStream_t s)
 {
-    if (ctx->last_done && s != ctx->last_stream)
-        HIPCHECK(hipStreamWaitEvent(s, ctx->last_done, 0));
+    if (u.done && s != u.stream)
+        HIPCHECK(hipStreamWaitEvent(s, u.done, 0));
     return VACO_AV1PF_OK;
+}
+
+// The intermediate set for a per-frame call on stream s: one last used on s
+// (no wait), else an unused one, else the sets in turn.
+static int pick_set(vaco_av1pf_ctx *ctx, hipStream_t s)
+{
+    for (int k = 0; k < kSets; k++)
+        if (ctx->set_use[k].stream == s)
+            return k;
+    for (int k = 0; k < kSets; k++)
+        if (!ctx->set_use[k].done)
+            return k;
+    const int k = ctx->next_set;
+    ctx->next_set = (k + 1) % kSets;
+    return k;
 }
 
 // ------------------------------------------------------------ create --
@@ -375,20 +396,21 @@ extern "C" vaco_av1pf_status vaco_av1pf_create(int device, uint32_t max_width, u
     if (e != hipSuccess)
         return bail(e);
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e == hipSuccess)
-        e = hipStreamCreateWithFlags(&ctx->up_stream, hipStreamNonBlocking);
     if (e != hipSuccess)
         return bail(e);
     // Mi-padded (multiple of 8) maximum geometry.
     ctx->max_w = (max_width + 7) & ~7u;
     ctx->max_h = (max_height + 7) & ~7u;
     ctx->plane_cap = (size_t)ctx->max_w * ctx->max_h;
-    e = hipMalloc(&ctx->pool, ctx->plane_cap * 15 * sizeof(uint16_t));
+    e = hipMalloc(&ctx->pool, ctx->plane_cap * kPoolPlanes * sizeof(uint16_t));
     if (e != hipSuccess)
         return bail(e);
-    for (int i = 0; i < 5; i++)
-        for (int p = 0; p < 3; p++)
-            ctx->img[i][p] = ctx->pool + ctx->plane_cap * (size_t)(i * 3 + p);
+    for (int p = 0; p < 3; p++) {
+        ctx->img_in[p] = ctx->pool + ctx->plane_cap * (size_t)p;
+        ctx->img_out[p] = ctx->pool + ctx->plane_cap * (size_t)(3 + p);
+    }
+    for (int k = 0; k < kSets; k++)
+        ctx->set_img[k] = ctx->pool + ctx->plane_cap * (size_t)(6 + k * kSetPlanes);
     const size_t blocks8 = (size_t)(ctx->max_w / 8 + 2) * (ctx->max_h / 8 + 2);
     const size_t units = (size_t)(ctx->max_w / 32 + 2) * (ctx->max_h / 32 + 2) * 3;
     const size_t cap =
@@ -403,8 +425,6 @@ extern "C" vaco_av1pf_status vaco_av1pf_create(int device, uint32_t max_width, u
         e = hipEventCreateWithFlags(&a.done, hipEventDisableTiming | hipEventDisableSystemFence);
         if (e == hipSuccess)
             e = hipEventCreateWithFlags(&a.done_sys, hipEventDisableTiming);
-        if (e == hipSuccess)
-            e = hipEventCreateWithFlags(&a.up_done, hipEventDisableTiming | hipEventDisableSystemFence);
         if (e != hipSuccess)
             return bail(e);
     }
@@ -442,8 +462,6 @@ extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
             hipEventSynchronize(ctx->slot[i].done_ev);
     if (ctx->stream)
         hipStreamSynchronize(ctx->stream);
-    if (ctx->up_stream)
-        hipStreamSynchronize(ctx->up_stream);
     for (int i = 0; i < kSlots; i++) {
         if (ctx->slot[i].h)
             hipHostFree(ctx->slot[i].h);
@@ -453,8 +471,6 @@ extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
             hipEventDestroy(ctx->slot[i].done);
         if (ctx->slot[i].done_sys)
             hipEventDestroy(ctx->slot[i].done_sys);
-        if (ctx->slot[i].up_done)
-            hipEventDestroy(ctx->slot[i].up_done);
         for (auto &gr : ctx->slot[i].graphs)
             hipGraphExecDestroy(gr.exec);
     }
@@ -470,8 +486,6 @@ extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
         hipFree(ctx->chain_flags);
     if (ctx->stream)
         hipStreamDestroy(ctx->stream);
-    if (ctx->up_stream)
-        hipStreamDestroy(ctx->up_stream);
     delete ctx;
 }
 
@@ -1192,11 +1206,11 @@ static vaco_av1pf_status launch_plan(vaco_av1pf_ctx *ctx, const Plan &plan, Aren
 static vaco_av1pf_status enqueue_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_desc &d,
                                        const FrameGeo &g, const DevPlane in[3],
                                        const DevPlane out[3], Arena *a, uint32_t *status,
-                                       hipStream_t s)
+                                       hipStream_t s, int set)
 {
     Scratch ws;
-    ws.base = ctx->img[1][0];
-    ws.cap = ctx->plane_cap * 9;
+    ws.base = ctx->set_img[set];
+    ws.cap = ctx->plane_cap * kSetPlanes;
     Plan plan;
     plan.tune = ctx->tune;
     vaco_av1pf_status st = plan_frame(ctx, d, g, in, out, a, status, ws, plan);
@@ -1366,24 +1380,27 @@ extern "C" vaco_av1pf_status vaco_av1pf_frame(vaco_av1pf_ctx *ctx, const vaco_av
         return st;
     if ((st = grow_slot(ctx, a, arena_need(*desc, g))) != VACO_AV1PF_OK)
         return st;
-    if ((st = order_after_previous(ctx, ctx->stream)) != VACO_AV1PF_OK)
+    const int set = pick_set(ctx, ctx->stream);
+    Use *chain = ctx->tune.chain ? &ctx->chain_use : nullptr;
+    if ((st = acquire(ctx, ctx->set_use[set], ctx->stream)) != VACO_AV1PF_OK ||
+        (chain && (st = acquire(ctx, *chain, ctx->stream)) != VACO_AV1PF_OK))
         return st;
     DevPlane din[3], dout[3];
     for (int p = 0; p < g.nplanes; p++) {
-        din[p] = DevPlane{ctx->img[0][p], g.in_w[p]};
-        dout[p] = DevPlane{ctx->img[4][p], g.out_w[p]};
+        din[p] = DevPlane{ctx->img_in[p], g.in_w[p]};
+        dout[p] = DevPlane{ctx->img_out[p], g.out_w[p]};
         HIPCHECK(hipMemcpy2DAsync(din[p].data, din[p].stride * 2, in->plane[p].data,
                                   in->plane[p].stride * 2, g.in_w[p] * 2, g.in_h[p],
                                   hipMemcpyHostToDevice, ctx->stream));
     }
-    st = enqueue_frame(ctx, *desc, g, din, dout, a, a->status, ctx->stream);
+    st = enqueue_frame(ctx, *desc, g, din, dout, a, a->status, ctx->stream, set);
     if (st != VACO_AV1PF_OK)
         return st;
     for (int p = 0; p < g.nplanes; p++)
         HIPCHECK(hipMemcpy2DAsync(out->plane[p].data, out->plane[p].stride * 2, dout[p].data,
                                   dout[p].stride * 2, g.out_w[p] * 2, g.out_h[p],
                                   hipMemcpyDeviceToHost, ctx->stream));
-    if ((st = close_slot(ctx, a, ctx->stream)) != VACO_AV1PF_OK)
+    if ((st = close_slot(ctx, a, ctx->stream, false, &ctx->set_use[set], chain)) != VACO_AV1PF_OK)
         return st;
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     if (*a->status & kFlagChainStall)
@@ -1432,15 +1449,22 @@ static vaco_av1pf_status frame_device_to(vaco_av1pf_ctx *ctx, const vaco_av1pf_f
         return st;
     if ((st = grow_slot(ctx, a, arena_need(*desc, g))) != VACO_AV1PF_OK)
         return st;
-    if ((st = order_after_previous(ctx, s)) != VACO_AV1PF_OK)
+    // Intermediates from a set last used on this stream where there is one;
+    // the chained launch's counters are ordered across streams (a chain
+    // assumes the previous chain has completed).
+    const int set = pick_set(ctx, s);
+    if ((st = acquire(ctx, ctx->set_use[set], s)) != VACO_AV1PF_OK)
+        return st;
+    Use *chain = ctx->tune.chain ? &ctx->chain_use : nullptr;
+    if (chain && (st = acquire(ctx, *chain, s)) != VACO_AV1PF_OK)
         return st;
     DevPlane din[3];
     for (int p = 0; p < g.nplanes; p++)
         din[p] = DevPlane{in->plane[p].data, in->plane[p].stride};
-    st = enqueue_frame(ctx, *desc, g, din, dout, a, a->status, s);
+    st = enqueue_frame(ctx, *desc, g, din, dout, a, a->status, s, set);
     if (st != VACO_AV1PF_OK)
         return st;
-    if ((st = close_slot(ctx, a, s)) != VACO_AV1PF_OK)
+    if ((st = close_slot(ctx, a, s, false, &ctx->set_use[set], chain)) != VACO_AV1PF_OK)
         return st;
     // The status is settled when the slot is reused or at finish().
     a->descs.assign(1, *desc);
@@ -1786,8 +1810,8 @@ vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
         // needs more than any batch before it.
         HIPCHECK(hipStreamSynchronize(s));
         HIPCHECK(hipStreamSynchronize(ctx->stream));
-        if (ctx->last_done)
-            HIPCHECK(hipEventSynchronize(ctx->last_done));
+        if (ctx->ws_use.done)
+            HIPCHECK(hipEventSynchronize(ctx->ws_use.done));
         if (ctx->ws)
             HIPCHECK(hipFree(ctx->ws));
         ctx->ws = nullptr;
@@ -1809,7 +1833,7 @@ vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
         return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "batch exceeds its staging slot");
     if ((st = grow_slot(ctx, a, arena)) != VACO_AV1PF_OK)
         return st;
-    if (!own_ws && (st = order_after_previous(ctx, s)) != VACO_AV1PF_OK)
+    if (!own_ws && (st = acquire(ctx, ctx->ws_use, s)) != VACO_AV1PF_OK)
         return st;
     Plan plan;
     plan.tune = ctx->tune;
@@ -1826,7 +1850,7 @@ vaco_av1pf_status av1pf_enqueue_batch(vaco_av1pf_ctx *ctx, uint32_t n,
     }
     if ((st = launch_plan(ctx, plan, a, s)) != VACO_AV1PF_OK)
         return st;
-    if ((st = close_slot(ctx, a, s, done != nullptr, own_ws == nullptr)) != VACO_AV1PF_OK)
+    if ((st = close_slot(ctx, a, s, done != nullptr, own_ws ? nullptr : &ctx->ws_use)) != VACO_AV1PF_OK)
         return st;
     if (statuses) {
         *statuses = a->status;
