@@ -480,8 +480,8 @@ vaco_av1pf_status vaco_av1pf_set_chain_spin_limit(vaco_av1pf_ctx *ctx, uint32_t 
  *    The HIP runtime maps streams onto a pool of GPU_MAX_HW_QUEUES hardware
  *    queues per process (4 by default) and streams sharing a queue run in
  *    order, so overlap needs a queue per stream: set GPU_MAX_HW_QUEUES to
- *    at least the process's stream count (each context has 2, the server
- *    max_inflight more) before the runtime starts.
+ *    at least the process's stream count (each context has 1, the server
+ *    max_inflight more, plus the caller's own) before the runtime starts.
  * Every device and pinned buffer the server uses is allocated by
  * vaco_av1pf_server_create (one workspace per stream, up to 2 GiB each but
  * at least the largest frame's; staging slots of up to 16 MiB); a batch is

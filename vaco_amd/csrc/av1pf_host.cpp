@@ -1868,25 +1868,34 @@ extern "C" vaco_av1pf_status vaco_av1pf_finish(vaco_av1pf_ctx *ctx)
     if (!ctx)
         return VACO_AV1PF_INVALID_ARGUMENT;
     HIPCHECK(hipSetDevice(ctx->device));
-    // The streams the pending frames ran on are synchronised at the end: a
-    // stream synchronisation makes their outputs visible to the host.
+    // The streams the pending frames ran on are synchronised (which also
+    // makes their outputs visible to the host); every pending slot's work
+    // has then completed, and the slots' deferred checks are settled.
     hipStream_t used[kSlots];
     int n_used = 0;
     for (int i = 0; i < kSlots; i++) {
-        Arena &a = ctx->slot[i];
-        if (a.pending) {
-            HIPCHECK(hipEventSynchronize(a.done_ev));
-            a.pending = false;
-            if (a.stream != ctx->stream && std::find(used, used + n_used, a.stream) == used + n_used)
-                used[n_used++] = a.stream;
-        }
-        vaco_av1pf_status st = resolve_slot(ctx, a);
-        if (st != VACO_AV1PF_OK)
-            return st;
+        const Arena &a = ctx->slot[i];
+        if (a.pending && a.stream && a.stream != ctx->stream &&
+            std::find(used, used + n_used, a.stream) == used + n_used)
+            used[n_used++] = a.stream;
     }
     for (int i = 0; i < n_used; i++)
         HIPCHECK(hipStreamSynchronize(used[i]));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < kSlots; i++) {
+        Arena &a = ctx->slot[i];
+        if (a.pending && !a.stream)  // settled by its owner; its stream may be gone
+            HIPCHECK(hipEventSynchronize(a.done_ev));
+        a.pending = false;
+        vaco_av1pf_status st = resolve_slot(ctx, a);
+        if (st != VACO_AV1PF_OK)
+            return st;
+    }
+    // Nothing is in flight: no later call need wait for an earlier one (and
+    // the callers' streams may now go away).
+    for (Use &u : ctx->set_use)
+        u = Use{};
+    ctx->ws_use = ctx->chain_use = Use{};
     const vaco_av1pf_status r = ctx->deferred;
     if (r != VACO_AV1PF_OK)
         fail(ctx, r, ctx->deferred_msg);
