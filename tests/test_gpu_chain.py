@@ -100,7 +100,7 @@ def test_gpu_chain_matches_oracle_under_every_variant(ctx, frames, cdef_xcd, lr_
     try:
         run_frames(ctx, frames, label)
     finally:
-        ctx.tuning = {"chain": 0, "cdef_xcd_order": 1, "lr_order": 1, "lr_tile_rows": 64}
+        ctx.tuning = {"chain": 0, "cdef_xcd_order": 1, "lr_order": 0, "lr_tile_rows": 64}
 
 
 def test_gpu_chain_off_runs_separate_launches(ctx, frames):

@@ -20,7 +20,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-DEFAULT = {"superres_rows": 8, "cdef_xcd_order": 1, "lr_order": 1, "lr_tile_rows": 64, "cdef_waves": 16, "chain": 0}
+DEFAULT = {"superres_rows": 8, "cdef_xcd_order": 1, "lr_order": 0, "lr_tile_rows": 64, "cdef_waves": 16, "chain": 0}
 
 
 @pytest.fixture(scope="module")

@@ -58,7 +58,7 @@ constexpr int kSlots = 8;
 constexpr uint32_t kMaxBatch = 256;  // frames per batched call
 constexpr uint32_t kMaxSuperresWidth = 65536;  // k_superres's 32-bit source positions
 // Launch variants of a new context (vaco_av1pf_autotune may change them).
-constexpr vaco_av1pf_tuning kDefaultTuning = {8, 1, 1, 64, 16, 0};
+constexpr vaco_av1pf_tuning kDefaultTuning = {8, 1, 0, 64, 16, 0};
 constexpr const char *kChainStallMsg =
     "a restoration tile of a chained launch gave up waiting for its CDEF superblocks";
 // Polls (each ~0.1-0.3 us) a chained launch's restoration tile makes before
