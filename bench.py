@@ -160,6 +160,7 @@ def measure_mixed_batch(av1pf, hip, workloads, reps: int, n: int = 64):
     prof = ctx.profile_read()
     ctx.profile(False)
     kernel_ms = {k: round(v[0] / reps, 4) for k, v in prof.items() if v[1]}
+    kernel_bytes = {k: v[2] // reps for k, v in prof.items() if v[1]}
     exact = all(np.array_equal(a.download(), b.download())
                 for fa, fb in zip(out_pf, out_b) for a, b in zip(fa, fb))
     mpix = sum(f.output_luma_pixels for f in frames) / 1e6
@@ -178,6 +179,7 @@ def measure_mixed_batch(av1pf, hip, workloads, reps: int, n: int = 64):
         "batched_graph_replay": {"ms_per_batch": round(t_bg * 1e3, 3), "mpix_s": round(mpix / t_bg, 2),
                                  "bit_exact_vs_per_frame": bool(exact_graph)},
         "batched_kernel_ms_per_batch": kernel_ms,
+        "batched_kernel_algorithmic_bytes_per_batch": kernel_bytes,
         "batched_kernel_sum_ms": round(sum(v for k, v in kernel_ms.items() if k != "upload"), 3),
         "batched_host_enqueue_ms": round(sorted(host)[len(host) // 2] * 1e3, 3),
         "bit_exact_vs_per_frame": bool(exact),
@@ -306,15 +308,16 @@ def read_traffic(kernel, tuning):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE / WRITE_SIZE, scaled by the copy calibration), for the
     launch variant `tuning` selects (work items numbered XCD-contiguously or
-    not; then the CDEF workgroup size or the restoration tile height), or
-    None."""
+    not; then the CDEF workgroup size or the restoration tile height; for the
+    chained launch both orders and the tile height), or None."""
     path = os.path.join(ROOT, "profiles", "pmc_hbm_traffic_1080p_per_frame.md")
-    xcd = {"cdef": tuning.get("cdef_xcd_order"), "restoration": tuning.get("lr_order")}.get(kernel)
-    order = "xcd" if xcd else "plain"
+    order = lambda field: "xcd" if tuning.get(field) else "plain"
     if kernel == "cdef":
-        key = f"{order}-{64 * tuning.get('cdef_waves', 16)}"
-    else:
-        key = f"{order}-{tuning.get('lr_tile_rows', 64)}"
+        key = f"{order('cdef_xcd_order')}-{64 * tuning.get('cdef_waves', 16)}"
+    elif kernel == "restoration":
+        key = f"{order('lr_order')}-{tuning.get('lr_tile_rows', 64)}"
+    else:  # the chained launch: both orders, then the tile height
+        key = f"{order('cdef_xcd_order')}-{order('lr_order')}-{tuning.get('lr_tile_rows', 64)}"
     try:
         for line in open(path):
             if line.startswith("traffic-json:"):

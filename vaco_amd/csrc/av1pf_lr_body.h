@@ -34,12 +34,47 @@ constexpr int kS = 72;   // staged row stride (samples), 70 used
 constexpr int kG = 66;   // A/B grid row stride, 66 used
 
 // SGR_PARAMS, restoration.rs:286-303: {r0, e0, r1, e1}.
-static __constant__ int kSgr[16][4] = {
+constexpr int kSgr[16][4] = {
     {2, 12, 1, 4},  {2, 15, 1, 6},  {2, 18, 1, 8},  {2, 21, 1, 9},
     {2, 24, 1, 10}, {2, 29, 1, 11}, {2, 36, 1, 12}, {2, 45, 1, 13},
     {2, 56, 1, 14}, {2, 68, 1, 15}, {0, 0, 1, 5},   {0, 0, 1, 8},
     {0, 0, 1, 11},  {0, 0, 1, 14},  {2, 30, 0, 0},  {2, 75, 0, 0},
 };
+
+// One guided pass of a parameter set: radius (0: pass off), the scale of
+// restoration.rs:316-317 for n = (2r+1)^2 and its eps, and the clamp
+// threshold of ab_of (the smallest p whose z reaches 255).
+struct SgrPass {
+    uint32_t r, scale, thr, pad;
+};
+
+constexpr SgrPass sgr_pass(int r, int eps)
+{
+    if (r == 0)
+        return SgrPass{0, 0, 0, 0};
+    const uint32_t n = (uint32_t)((2 * r + 1) * (2 * r + 1));
+    const uint32_t n2e = n * n * (uint32_t)eps;
+    const uint32_t scale = ((1u << 20) + n2e / 2) / n2e;
+    const uint32_t thr = ((255u << 20) - (1u << 19) + scale - 1) / scale;
+    return SgrPass{(uint32_t)r, scale, thr, 0};
+}
+
+struct SgrPasses {
+    SgrPass p[16][2];
+};
+
+constexpr SgrPasses sgr_passes()
+{
+    SgrPasses t{};
+    for (int s = 0; s < 16; s++) {
+        t.p[s][0] = sgr_pass(kSgr[s][0], kSgr[s][1]);
+        t.p[s][1] = sgr_pass(kSgr[s][2], kSgr[s][3]);
+    }
+    return t;
+}
+
+// Both passes of each set, indexed by a wave-uniform set (scalar loads).
+static __constant__ SgrPasses kSgrPasses = sgr_passes();
 
 // One tile of a list entry: tile index within its item << 32 | item
 // (LrPlane index) << 2 | unit type (kUnitNone, kUnitWiener, kUnitSelfGuided;
@@ -577,62 +612,100 @@ __device__ __forceinline__ uint32_t ab_of(int sum, int sumsq, uint32_t scale, ui
 // x0-1+cc), cr < th+2, cc < tw+2, the packed A/B, written over VS/VQ as
 // grid AB (row stride kG). Pass 0 (R = 2) needs odd rows y only
 // (restoration.rs:347-356), so it walks only those centre rows.
+//
+// Each thread takes runs of 4 columns of one row, so that one 8-byte read
+// brings 4 samples: the column sums read 2R+1 such groups of S; the box
+// sums of 4 centres read VS columns 4j .. 4j+7 (two 8-byte reads) and VQ
+// columns 4j .. 4j+7 (two 16-byte reads) and slide over them. Columns past the tile's
+// (up to the row stride) are computed from stale LDS and never stored to
+// the grid.
 template <int R, int BD, int ROWS>
 __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th, int tw, int y0,
-                                         int eps, const uint16_t *xbx1)
+                                         uint32_t scale, uint32_t thr, const uint16_t *xbx1)
 {
     constexpr int N = (2 * R + 1) * (2 * R + 1);
     constexpr int STEP = R == 2 ? 2 : 1;  // centre rows walked
     constexpr int T = LrShape<ROWS>::kThreads;
-    constexpr int KC = ((R == 2 ? ROWS / 2 + 2 : ROWS + 2) * kS + T - 1) / T;
-    const uint32_t n2e = (uint32_t)(N * N * eps);
-    const uint32_t scale = ((1u << 20) + n2e / 2) / n2e;
-    const uint32_t thr = ((255u << 20) - (1u << 19) + scale - 1) / scale;
+    constexpr int MAXR = R == 2 ? ROWS / 2 + 2 : ROWS + 2;  // centre rows walked, at most
+    constexpr int Q1 = kS / 4;                               // 4-column groups of a VS row
+    constexpr int Q2 = (kG + 2) / 4;                         // 4-centre runs of a grid row
+    constexpr int KC = (MAXR * Q2 + T - 1) / T;
+    static_assert(kS % 8 == 0 && 4 * Q2 + 4 <= kS, "runs read VS / VQ columns 4j .. 4j+7 of one row");
     const int crows = th + 2, ccols = tw + 2, scols = tw + 6;
     const int cr0 = R == 2 ? (y0 & 1) : 0;  // first centre row with y0-1+cr odd
     const int nr = (crows - cr0 + STEP - 1) / STEP;
+    const int ng = (scols + 3) >> 2, nq = (ccols + 3) >> 2;
     const int tid = threadIdx.x;
     uint16_t *VS = reinterpret_cast<uint16_t *>(X);
     int *VQ = reinterpret_cast<int *>(X) + (ROWS + 2) * kS / 2;
 #pragma unroll 2
-    for (int e = tid; e < nr * kS; e += T) {
-        const int k = e / kS, c = e - k * kS, cr = cr0 + STEP * k;
-        if (c >= scols)
+    for (int e = tid; e < nr * Q1; e += T) {
+        const int k = e / Q1, g = e - k * Q1, cr = cr0 + STEP * k;
+        if (g >= ng)
             continue;
-        int s = 0, q = 0;
+        int s[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int dy = -R; dy <= R; dy++) {
-            const int v = S[(cr + 2 + dy) * kS + c];
-            s += v;
-            q += v * v;
+            const uint2 w = *reinterpret_cast<const uint2 *>(&S[(cr + 2 + dy) * kS + 4 * g]);
+            const int v[4] = {(int)(w.x & 0xffffu), (int)(w.x >> 16), (int)(w.y & 0xffffu), (int)(w.y >> 16)};
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                s[m] += v[m];
+                q[m] += v[m] * v[m];
+            }
         }
-        VS[cr * kS + c] = (uint16_t)s;
-        VQ[cr * kS + c] = q;
+        *reinterpret_cast<uint2 *>(&VS[cr * kS + 4 * g]) =
+            make_uint2((uint32_t)s[0] | ((uint32_t)s[1] << 16), (uint32_t)s[2] | ((uint32_t)s[3] << 16));
+        *reinterpret_cast<int4 *>(&VQ[cr * kS + 4 * g]) = make_int4(q[0], q[1], q[2], q[3]);
     }
     __syncthreads();
-    uint32_t ab[KC];
+    uint32_t ab[KC][4];
 #pragma unroll
     for (int i = 0; i < KC; i++) {
         const int e = tid + i * T;
-        const int k = e / kS, cc = e - k * kS, cr = cr0 + STEP * k;
-        ab[i] = 0;
-        if (k >= nr || cc >= ccols)
+        const int k = e / Q2, j = e - k * Q2, cr = cr0 + STEP * k;
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+            ab[i][m] = 0;
+        if (k >= nr || j >= nq)
             continue;
-        int s = 0, q = 0;
+        // VS rows are 8-byte aligned at column 4j, VQ rows 16-byte aligned
+        const uint2 va = *reinterpret_cast<const uint2 *>(&VS[cr * kS + 4 * j]);
+        const uint2 vb = *reinterpret_cast<const uint2 *>(&VS[cr * kS + 4 * j + 4]);
+        const int4 q0 = *reinterpret_cast<const int4 *>(&VQ[cr * kS + 4 * j]);
+        const int4 q1 = *reinterpret_cast<const int4 *>(&VQ[cr * kS + 4 * j + 4]);
+        const int sv[8] = {(int)(va.x & 0xffffu), (int)(va.x >> 16), (int)(va.y & 0xffffu), (int)(va.y >> 16),
+                           (int)(vb.x & 0xffffu), (int)(vb.x >> 16), (int)(vb.y & 0xffffu), (int)(vb.y >> 16)};
+        const int qv[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
 #pragma unroll
-        for (int dx = -R; dx <= R; dx++) {
-            s += VS[cr * kS + cc + 2 + dx];
-            q += VQ[cr * kS + cc + 2 + dx];
+        for (int m = 0; m < 4; m++) {
+            // centre column 4j+m is staged column 4j+m+2
+            int s = 0, q = 0;
+#pragma unroll
+            for (int dx = -R; dx <= R; dx++) {
+                s += sv[m + 2 + dx];
+                q += qv[m + 2 + dx];
+            }
+            ab[i][m] = ab_of<N, BD>(s, q, scale, thr, xbx1);
         }
-        ab[i] = ab_of<N, BD>(s, q, scale, thr, xbx1);
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < KC; i++) {
         const int e = tid + i * T;
-        const int k = e / kS, cc = e - k * kS, cr = cr0 + STEP * k;
-        if (k < nr && cc < ccols)
-            X[cr * kG + cc] = ab[i];
+        const int k = e / Q2, j = e - k * Q2, cr = cr0 + STEP * k;
+        if (k >= nr || j >= nq)
+            continue;
+        uint32_t *dst = &X[cr * kG + 4 * j];
+        if (4 * j + 4 <= ccols) {
+            *reinterpret_cast<uint2 *>(dst) = make_uint2(ab[i][0], ab[i][1]);
+            *reinterpret_cast<uint2 *>(dst + 2) = make_uint2(ab[i][2], ab[i][3]);
+        } else {
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+                if (4 * j + m < ccols)
+                    dst[m] = ab[i][m];
+        }
     }
     __syncthreads();
 }
@@ -720,9 +793,11 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
     for (int z = tid; z < 256; z += T)
         xbx1[z] = z == 0 ? 1 : (uint16_t)(((z << 8) + z / 2) / (z + 1));
 
-    // self_guided, restoration.rs:373-384, with guided_pass 341-371.
-    const int set = u.set & 15;
-    const int r0 = kSgr[set][0], e0 = kSgr[set][1], r1 = kSgr[set][2], e1 = kSgr[set][3];
+    // self_guided, restoration.rs:373-384, with guided_pass 341-371. The set
+    // is the tile's (one unit), so its parameters come through scalar loads.
+    const int set = __builtin_amdgcn_readfirstlane(u.set & 15);
+    const SgrPass g0 = kSgrPasses.p[set][0], g1 = kSgrPasses.p[set][1];
+    const int r0 = (int)g0.r, r1 = (int)g1.r;
     // Thread = column c and a strip of 8 output rows rs .. rs + 7 (its wave),
     // so each A/B grid row the strip needs is read once for all 8 rows.
     const int c = tid & 63, rs = (tid >> 6) * 8;
@@ -734,11 +809,11 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
     }
     if (r0) {
         if (bd == 8)
-            sgr_grid<2, 8, ROWS>(S, X, th, tw, y0, e0, xbx1);
+            sgr_grid<2, 8, ROWS>(S, X, th, tw, y0, g0.scale, g0.thr, xbx1);
         else if (bd == 10)
-            sgr_grid<2, 10, ROWS>(S, X, th, tw, y0, e0, xbx1);
+            sgr_grid<2, 10, ROWS>(S, X, th, tw, y0, g0.scale, g0.thr, xbx1);
         else
-            sgr_grid<2, 12, ROWS>(S, X, th, tw, y0, e0, xbx1);
+            sgr_grid<2, 12, ROWS>(S, X, th, tw, y0, g0.scale, g0.thr, xbx1);
         if ((y0 + rs) & 1)
             sgr_pass0_strip<1>(X, c, rs, uu, f0);
         else
@@ -748,11 +823,11 @@ __device__ __forceinline__ void lr_sgr(const Tile &t, uint8_t *lds)
         if (r0)
             __syncthreads();  // the pass-0 grid is fully consumed
         if (bd == 8)
-            sgr_grid<1, 8, ROWS>(S, X, th, tw, y0, e1, xbx1);
+            sgr_grid<1, 8, ROWS>(S, X, th, tw, y0, g1.scale, g1.thr, xbx1);
         else if (bd == 10)
-            sgr_grid<1, 10, ROWS>(S, X, th, tw, y0, e1, xbx1);
+            sgr_grid<1, 10, ROWS>(S, X, th, tw, y0, g1.scale, g1.thr, xbx1);
         else
-            sgr_grid<1, 12, ROWS>(S, X, th, tw, y0, e1, xbx1);
+            sgr_grid<1, 12, ROWS>(S, X, th, tw, y0, g1.scale, g1.thr, xbx1);
         sgr_pass1_strip(X, c, rs, uu, f1);
     }
     const int w0 = u.xqd[0], w1 = u.xqd[1];
