@@ -654,6 +654,15 @@ void lr_cut_rows(const std::vector<LrRowSeg> &segs, int rows, std::vector<LrRowS
         }
 }
 
+// One k_lr list entry: the tile's row segment ri and column segment ci of
+// its item (LrPlane index), and its kind (0 copy, 1 Wiener, 2 self-guided).
+// Segment counts stay below 2^16 (a plane of at most 65536 samples a side
+// has at most 2050 row and 1025 column segments).
+static uint64_t lr_entry(uint32_t ri, uint32_t ci, uint32_t item, int kind)
+{
+    return (uint64_t)ri << 48 | (uint64_t)ci << 32 | (uint64_t)item << 2 | (uint64_t)kind;
+}
+
 // k_lr's tile list: self-guided tiles, then Wiener, then copies (by_kind), or
 // the one plane-order list (remapped in the kernel, xcd_remap), written to
 // `out`. In a by-kind list, each kind's tiles (in plane order) are dealt to
@@ -753,8 +762,7 @@ struct Plan {
     uint32_t sr_blocks = 0;
     std::vector<LrPlane> lr;
     uint32_t lr_blocks = 0;
-    // restoration tiles by kernel: 0 copy, 1 Wiener, 2 self-guided;
-    // entry = tile index within the plane << 32 | LrPlane index << 2 | kind
+    // restoration tiles by kernel: 0 copy, 1 Wiener, 2 self-guided (lr_entry)
     std::vector<uint64_t> lr_tiles[3];
     struct Copy {
         DevPlane src, dst;
@@ -1011,8 +1019,7 @@ static vaco_av1pf_status plan_frame(vaco_av1pf_ctx *ctx, const vaco_av1pf_frame_
                                 d.lr_units[p][(*rows)[ri].unit_row * L.unit_cols + (*cols)[ci].unit_col].kind;
                             kind = k == VACO_AV1PF_UNIT_WIENER ? 1 : k == VACO_AV1PF_UNIT_SELF_GUIDED ? 2 : 0;
                         }
-                        plan.lr_tiles[kind].push_back((uint64_t)(ri * L.n_cols + ci) << 32 | (uint64_t)item << 2 |
-                                                      (uint64_t)kind);
+                        plan.lr_tiles[kind].push_back(lr_entry(ri, ci, item, kind));
                     }
             plan.lr.push_back(L);
         }
@@ -2283,6 +2290,8 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
         return fail(ctx, VACO_AV1PF_INVALID_DATA, msg);
     if (out->width != cfg->width || out->height != cfg->height || out->stride < out->width)
         return fail(ctx, VACO_AV1PF_INVALID_ARGUMENT, "plane geometry");
+    if (cfg->width > 65536 || cfg->height > 65536)  // tile list entries hold 16-bit segment indices
+        return fail(ctx, VACO_AV1PF_LIMIT_EXCEEDED, "restoration plane larger than 65536 samples a side");
     HIPCHECK(hipSetDevice(ctx->device));
     std::vector<LrRowSeg> segs;
     std::vector<LrColSeg> cols;
@@ -2302,8 +2311,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
         for (uint32_t ci = 0; ci < cols.size(); ci++) {
             const uint8_t k = units[rows[ri].unit_row * ucols + cols[ci].unit_col].kind;
             const int kind = k == VACO_AV1PF_UNIT_WIENER ? 1 : k == VACO_AV1PF_UNIT_SELF_GUIDED ? 2 : 0;
-            tiles[tune.lr_order ? 0 : kind].push_back((uint64_t)(ri * (uint32_t)cols.size() + ci) << 32 |
-                                                      (uint64_t)kind);
+            tiles[tune.lr_order ? 0 : kind].push_back(lr_entry(ri, ci, 0, kind));
         }
     const size_t ntiles = rows.size() * cols.size();
     const size_t bytes_tail = rows.size() * sizeof(LrRowSeg) + cols.size() * sizeof(LrColSeg) +

@@ -76,7 +76,7 @@ constexpr SgrPasses sgr_passes()
 // Both passes of each set, indexed by a wave-uniform set (scalar loads).
 static __constant__ SgrPasses kSgrPasses = sgr_passes();
 
-// One tile of a list entry: tile index within its item << 32 | item
+// One tile of a list entry: row segment << 48 | column segment << 32 | item
 // (LrPlane index) << 2 | unit type (kUnitNone, kUnitWiener, kUnitSelfGuided;
 // kUnitNone also for planes without restoration). The type in the entry
 // settles the workgroup's path before any descriptor is read; the unit's
@@ -108,26 +108,57 @@ static_assert(offsetof(LrPlane, bit_depth) % 4 == 0 &&
 // plane order), tile wg of the n_items plane items: the item whose
 // block_begin <= wg, its tile wg - block_begin in row-major order of its row
 // and column segments, with the kind of the tile's unit.
+// The descriptor words a tile needs before it can stage: its segment
+// tables, the unit grid width and the byte fields.
+struct PlaneHead {
+    const LrRowSeg *rows;
+    const LrColSeg *cols;
+    uint32_t unit_cols, b0, b1;
+    __device__ __forceinline__ explicit PlaneHead(const LrPlane &P)
+        : rows(P.rows), cols(P.cols), unit_cols(P.unit_cols)
+    {
+        __builtin_memcpy(&b0, &P.bit_depth, 4);
+        __builtin_memcpy(&b1, &P.out_vec, 4);
+    }
+};
+
+// With a list, the entry names the segments (no division by the segment
+// count before their loads).
 __device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, uint32_t n_items, const uint64_t *list,
                                            uint32_t wg)
 {
     Tile t;
-    uint32_t local;
+    uint32_t ri, ci;
+    const LrRowSeg *rows;
+    const LrColSeg *cols;
+    uint32_t unit_cols;
     if (list) {
         const uint64_t e = list[wg];
+        const PlaneHead h(planes[(uint32_t)e >> 2]);
+        rows = h.rows;
+        cols = h.cols;
+        unit_cols = h.unit_cols;
+        t.b0 = h.b0;
+        t.b1 = h.b1;
         t.kind = (int)(e & 3u);
         t.P = &planes[(uint32_t)e >> 2];
-        local = (uint32_t)(e >> 32);
+        ri = (uint32_t)(e >> 48);
+        ci = (uint32_t)(e >> 32) & 0xffffu;
     } else {
         t.P = &find_item(planes, (int)n_items, wg);
-        local = wg - t.P->block_begin;
+        const uint32_t local = wg - t.P->block_begin;
+        ri = local / t.P->n_cols;
+        ci = local - ri * t.P->n_cols;
+        const PlaneHead h(*t.P);
+        rows = h.rows;
+        cols = h.cols;
+        unit_cols = h.unit_cols;
+        t.b0 = h.b0;
+        t.b1 = h.b1;
     }
-    __builtin_memcpy(&t.b0, &t.P->bit_depth, 4);
-    __builtin_memcpy(&t.b1, &t.P->out_vec, 4);
-    const uint32_t rsi = local / t.P->n_cols;
-    t.rs = glb(t.P->rows)[rsi];
-    t.cs = glb(t.P->cols)[local - rsi * t.P->n_cols];
-    t.unit_ix = t.rs.unit_row * t.P->unit_cols + t.cs.unit_col;
+    t.rs = glb(rows)[ri];
+    t.cs = glb(cols)[ci];
+    t.unit_ix = t.rs.unit_row * unit_cols + t.cs.unit_col;
     if (!list) {
         const uint8_t k = t.P->units ? glb(t.P->units)[t.unit_ix].kind : kUnitNone;
         t.kind = k == kUnitWiener ? kUnitWiener : k == kUnitSelfGuided ? kUnitSelfGuided : kUnitNone;
