@@ -186,15 +186,19 @@ __device__ __forceinline__ uint32_t dir_cost_any(int d, const uint16_t *p, int s
 }
 
 // Filter parameters of one 8x8 block (one plane class), cdef.rs:35-54.
-// off[k] is the LDS offset (in samples) of tap k (dir d1, d2; dir+2 d1, d2;
-// dir+6 d1, d2). The packed path reads sample pairs starting at ci + off and
-// ci - off as one aligned 32-bit word: for an odd offset that pair starts at
-// an odd index, so it is read from the shifted copy of the tile (index i
-// stored at i + 1). offp / offm are those two reads' byte offsets from the
-// pair's own address, resolved to the copy that makes them aligned.
+// Tap k's LDS offset (in samples) is that of direction dir, d1 / d2 (k = 0,
+// 1), dir+2 (k = 2, 3) or dir+6 (k = 4, 5). The packed path reads sample
+// pairs starting at ci + off and ci - off as one aligned 32-bit word: for an
+// odd offset that pair starts at an odd index, so it is read from the
+// shifted copy of the tile (index i stored at i + 1). offp / offm are those
+// two reads' byte offsets from the pair's own address, resolved to the copy
+// that makes them aligned.
+struct DirOffs {
+    int32_t offp[6], offm[6];
+};
+
 struct __align__(16) Blk {
     int32_t offp[6], offm[6];
-    int16_t off[6];
     uint8_t pri, sec;    // strengths after scaling / variance adjustment
     uint8_t shifts;      // damping - ilog2(strength), primary | secondary << 4
     uint8_t flags;       // 1: filter this block, 2: primary taps are (3, 3)
@@ -202,8 +206,30 @@ struct __align__(16) Blk {
 
 // ROW, SHIFT: the row stride and shifted-copy distance of the block's tile.
 template <int ROW, int SHIFT>
-__device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int damping,
-                                        int shift)
+__device__ __forceinline__ DirOffs dir_offs(int dir)
+{
+    DirOffs D;
+    const int d2 = (dir + 2) & 7, d6 = (dir + 6) & 7;
+    int off[6];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        off[k] = tap_offset<ROW>(dir, k);
+        off[2 + k] = tap_offset<ROW>(d2, k);
+        off[4 + k] = tap_offset<ROW>(d6, k);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int o = off[k];
+        D.offp[k] = 2 * ((o & 1) ? SHIFT + 1 + o : o);
+        D.offm[k] = 2 * ((o & 1) ? SHIFT + 1 - o : -o);
+    }
+    return D;
+}
+
+// A block's parameters from its direction's offsets D (dir_offs of the
+// direction the block filters along: 0 when the primary strength is 0).
+__device__ __forceinline__ Blk make_blk_with(const DirOffs &D, uint32_t pri, uint32_t sec, int damping,
+                                             int shift)
 {
     Blk B;
     B.pri = (uint8_t)min(pri, 255u);
@@ -212,20 +238,18 @@ __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int
     const int ss = sec ? max(0, damping - ilog2u(sec)) : 0;
     B.shifts = (uint8_t)(min(ps, 15) | (min(ss, 15) << 4));
     B.flags = 1 | (((pri >> shift) & 1) ? 2 : 0);  // cdef.rs:156-160
-    const int d2 = (dir + 2) & 7, d6 = (dir + 6) & 7;
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        B.off[k] = (int16_t)tap_offset<ROW>(dir, k);
-        B.off[2 + k] = (int16_t)tap_offset<ROW>(d2, k);
-        B.off[4 + k] = (int16_t)tap_offset<ROW>(d6, k);
-    }
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-        const int o = B.off[k];
-        B.offp[k] = 2 * ((o & 1) ? SHIFT + 1 + o : o);
-        B.offm[k] = 2 * ((o & 1) ? SHIFT + 1 - o : -o);
+        B.offp[k] = D.offp[k];
+        B.offm[k] = D.offm[k];
     }
     return B;
+}
+
+template <int ROW, int SHIFT>
+__device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int damping, int shift)
+{
+    return make_blk_with(dir_offs<ROW, SHIFT>(dir), pri, sec, damping, shift);
 }
 
 // filter_block (cdef.rs:142-205), computed for two horizontally adjacent
@@ -680,11 +704,12 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
 // V, each with its copy (V starts 8 samples further on: U and V pairs of one
 // half-wave then fall in different banks; a model of the tap addresses gives
 // 2.3 accesses per bank per read without the gap, 1.3 with it); the block
-// parameters, direction costs and block list.
+// parameters, the directions' tap offsets, direction costs and block list.
 template <int SX, int SY>
 struct __align__(16) CdefLds {
     uint16_t lt[2 * kTile + 4 * chroma_tile<SX, SY>() + 8];
     Blk blk[2][64];
+    DirOffs dirs[2][8];  // the 8 directions' tap offsets, luma then chroma tiles
     uint32_t cost[8][64];
     uint8_t bact[64];
     uint8_t blist[64];  // blocks to filter (ascending), then the rest
@@ -813,6 +838,15 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
     // Wave 0 stores its share of the staged-bytes copy and turns the loaded
     // skip flags and cdef_idx into the block list.
     auto side_info = [&]() {
+        // Wave 1 tabulates the 8 directions' tap offsets for the block
+        // parameters (their arithmetic then leaves the parameter phase).
+        if (tid >= 64 && tid < 80) {
+            const int d = tid & 7;
+            if (tid < 72)
+                S.dirs[0][d] = dir_offs<kRow, kTile>(d);
+            else
+                S.dirs[1][d] = dir_offs<CROW, CTILE>(d);
+        }
         if (tid < 64) {
             if (ARG && up_first) {
                 gptr<uint4> dst = glb(up_dst);
@@ -878,8 +912,8 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
                         const uint32_t lg = sc ? min(ilog2u(sc), 12) : 0;
                         adj = (pri * (4 + lg) + 8) >> 4;
                     }
-                    B = make_blk<kRow, kTile>(pri ? best : 0, adj, sf.strength(1, ix) << shift,
-                                              (uint8_t)(sf.damping + shift), shift);
+                    B = make_blk_with(S.dirs[0][pri ? best : 0], adj, sf.strength(1, ix) << shift,
+                                      (uint8_t)(sf.damping + shift), shift);
                 } else {
                     const uint32_t pri = sf.strength(2, ix) << shift;
                     // chroma_direction, cdef.rs:222-230: nibble `best` of
@@ -889,8 +923,8 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
                         cdir = (0x66654207u >> (4 * best)) & 7;
                     else if (!SX && SY)
                         cdir = (0x06432221u >> (4 * best)) & 7;
-                    B = make_blk<CROW, CTILE>(pri ? cdir : 0, pri, sf.strength(3, ix) << shift,
-                                              (uint8_t)(sf.damping + shift - 1), shift);
+                    B = make_blk_with(S.dirs[1][pri ? cdir : 0], pri, sf.strength(3, ix) << shift,
+                                      (uint8_t)(sf.damping + shift - 1), shift);
                 }
             }
             blk[cls][b] = B;
