@@ -31,7 +31,7 @@
 namespace av1pf {
 
 constexpr int kS = 72;   // staged row stride (samples), 70 used
-constexpr int kG = 66;   // A/B grid row stride, 66 used
+constexpr int kG = 68;   // A/B grid row stride (66 used; rows 16-byte aligned)
 
 // SGR_PARAMS, restoration.rs:286-303: {r0, e0, r1, e1}.
 constexpr int kSgr[16][4] = {
@@ -659,9 +659,11 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
     constexpr int T = LrShape<ROWS>::kThreads;
     constexpr int MAXR = R == 2 ? ROWS / 2 + 2 : ROWS + 2;  // centre rows walked, at most
     constexpr int Q1 = kS / 4;                               // 4-column groups of a VS row
-    constexpr int Q2 = (kG + 2) / 4;                         // 4-centre runs of a grid row
+    constexpr int Q2 = 17;                                   // 4-centre runs of a grid row (66 centres)
     constexpr int KC = (MAXR * Q2 + T - 1) / T;
     static_assert(kS % 8 == 0 && 4 * Q2 + 4 <= kS, "runs read VS / VQ columns 4j .. 4j+7 of one row");
+    static_assert(kG % 4 == 0 && 4 * Q2 <= kG && (ROWS + 2) * kG * 4 <= LrShape<ROWS>::kXBytes,
+                  "grid rows hold whole 16-byte runs inside the grid's LDS");
     const int crows = th + 2, ccols = tw + 2, scols = tw + 6;
     const int cr0 = R == 2 ? (y0 & 1) : 0;  // first centre row with y0-1+cr odd
     const int nr = (crows - cr0 + STEP - 1) / STEP;
@@ -727,10 +729,9 @@ __device__ __forceinline__ void sgr_grid(const uint16_t *S, uint32_t *X, int th,
         const int k = e / Q2, j = e - k * Q2, cr = cr0 + STEP * k;
         if (k >= nr || j >= nq)
             continue;
-        uint32_t *dst = &X[cr * kG + 4 * j];
+        uint32_t *dst = &X[cr * kG + 4 * j];  // 16-byte aligned
         if (4 * j + 4 <= ccols) {
-            *reinterpret_cast<uint2 *>(dst) = make_uint2(ab[i][0], ab[i][1]);
-            *reinterpret_cast<uint2 *>(dst + 2) = make_uint2(ab[i][2], ab[i][3]);
+            *reinterpret_cast<uint4 *>(dst) = make_uint4(ab[i][0], ab[i][1], ab[i][2], ab[i][3]);
         } else {
 #pragma unroll
             for (int m = 0; m < 4; m++)
