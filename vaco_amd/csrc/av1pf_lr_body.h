@@ -73,6 +73,26 @@ constexpr SgrPasses sgr_passes()
     return t;
 }
 
+// What ab_of relies on, for every pass in use: z = (p * scale + 2^19) >> 20
+// reaches 255 first at p = thr (so clamping p to thr keeps every z that
+// selects a table entry, and a2 = 256 beyond it), and thr * scale + 2^19
+// fits 32 bits.
+constexpr bool sgr_passes_exact()
+{
+    const SgrPasses t = sgr_passes();
+    for (int s = 0; s < 16; s++)
+        for (int i = 0; i < 2; i++) {
+            const SgrPass g = t.p[s][i];
+            if (g.r == 0)
+                continue;
+            const uint64_t top = (uint64_t)g.thr * g.scale + (1u << 19), below = top - g.scale;
+            if (top >= (1ull << 32) || (top >> 20) < 255 || (below >> 20) >= 255)
+                return false;
+        }
+    return true;
+}
+static_assert(sgr_passes_exact(), "self-guided clamp thresholds are exact in 32 bits");
+
 // Both passes of each set, indexed by a wave-uniform set (scalar loads).
 static __constant__ SgrPasses kSgrPasses = sgr_passes();
 
