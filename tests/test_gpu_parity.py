@@ -96,6 +96,20 @@ def test_gpu_superres_plane_wider_than_65536_is_refused(ctx):
     assert e.value.status == abi.LIMIT_EXCEEDED
 
 
+def test_gpu_restore_plane_rejects_planes_wider_than_65536(ctx):
+    """The tile list names row and column segments in 16 bits each; a wider
+    plane is refused before anything runs (the frame path's context is
+    capped at 65536 a side already)."""
+    say("restoration plane 65600 wide")
+    cfg = abi.LrPlaneConfig(65600, 8, 64, 8, 0)
+    cols, rows = av1pf.lr_unit_counts(cfg)
+    units = [abi.make_unit(abi.UNIT_NONE)] * (cols * rows)
+    plane = np.zeros((8, 65600), np.uint16)
+    with pytest.raises(av1pf.Av1Error) as e:
+        ctx.restore_plane(plane, plane, cfg, units)
+    assert e.value.status == abi.LIMIT_EXCEEDED
+
+
 def test_gpu_restoration_constant_planes_match_dav1d(ctx):
     say("restoration constant planes")
     for bd, s, plane, exp in G.constant_cases():

@@ -124,10 +124,6 @@ static_assert(offsetof(LrPlane, bit_depth) % 4 == 0 &&
               offsetof(LrPlane, out_vec) + 4 <= sizeof(LrPlane),
               "LrPlane byte fields are read as two words");
 
-// The tile of workgroup wg: list entry wg, or, without a list (tiles in
-// plane order), tile wg of the n_items plane items: the item whose
-// block_begin <= wg, its tile wg - block_begin in row-major order of its row
-// and column segments, with the kind of the tile's unit.
 // The descriptor words a tile needs before it can stage: its segment
 // tables, the unit grid width and the byte fields.
 struct PlaneHead {
@@ -142,8 +138,12 @@ struct PlaneHead {
     }
 };
 
-// With a list, the entry names the segments (no division by the segment
-// count before their loads).
+// The tile of workgroup wg: list entry wg, or, without a list (tiles in
+// plane order), tile wg of the n_items plane items: the item whose
+// block_begin <= wg, its tile wg - block_begin in row-major order of its row
+// and column segments, with the kind of the tile's unit. With a list, the
+// entry names the segments (no division by the segment count before their
+// loads).
 __device__ __forceinline__ Tile fetch_tile(const LrPlane *planes, uint32_t n_items, const uint64_t *list,
                                            uint32_t wg)
 {
