@@ -671,6 +671,7 @@ static uint64_t lr_entry(uint32_t ri, uint32_t ci, uint32_t item, int kind)
 // halo lines in one L2, while every XCD still gets a share of each kind's
 // long tiles.
 constexpr size_t kLrChunk = 4;
+constexpr size_t kCus = 256;  // compute units of an MI355X (8 XCDs of 32)
 static void lr_tile_list(const std::vector<uint64_t> (&tiles)[3], bool by_kind, uint64_t *out)
 {
     size_t a = 0;
@@ -696,6 +697,15 @@ static void lr_tile_list(const std::vector<uint64_t> (&tiles)[3], bool by_kind, 
         }
         a = b;
     }
+    // Workgroup i of a launch lands on CU i % kCus, so CU c takes entries c,
+    // c + kCus, c + 2 kCus ...: reversing every other block of kCus entries
+    // pairs the CUs that took the longest tiles of one block with the
+    // shortest of the next (kind order is roughly longest first). A block
+    // reversed keeps its XCDs (kCus is a multiple of 8), so chunks stay
+    // together.
+    if (by_kind)
+        for (size_t r = kCus; r < a; r += 2 * kCus)
+            std::reverse(out + r, out + std::min(a, r + kCus));
 }
 
 }  // namespace
