@@ -70,7 +70,8 @@ constexpr int bin_count_rows(int d, int b, int i0)
 // groups of 4 banks, so a half-wave filtering one block along one direction
 // reads its taps without bank conflicts. Chroma tiles are 80 samples wide
 // (48 when subsampled horizontally) and 68 rows high (36 when subsampled
-// vertically), so that two workgroups' tiles fit a CU's LDS in every layout.
+// vertically), so that two workgroups' tiles fit a CU's LDS in every layout
+// and four in 4:2:0.
 constexpr int kRow = 88;
 constexpr int kTile = 68 * kRow;
 template <int SX>
@@ -226,18 +227,35 @@ __device__ __forceinline__ DirOffs dir_offs(int dir)
     return D;
 }
 
-// A block's parameters from its direction's offsets D (dir_offs of the
-// direction the block filters along: 0 when the primary strength is 0).
-__device__ __forceinline__ Blk make_blk_with(const DirOffs &D, uint32_t pri, uint32_t sec, int damping,
-                                             int shift)
+// A block's parameters without its tap offsets: the direction it filters
+// along (0 when the primary strength is 0), strengths, shifts and flags. The
+// frame kernel keeps these 8 bytes per block in LDS and takes the offsets
+// from its per-workgroup table of the 8 directions when filtering.
+struct BlkP {
+    uint8_t dir, pri, sec, shifts, flags, pad[3];
+};
+
+__device__ __forceinline__ BlkP make_blkp(int dir, uint32_t pri, uint32_t sec, int damping, int shift)
 {
-    Blk B;
-    B.pri = (uint8_t)min(pri, 255u);
-    B.sec = (uint8_t)min(sec, 255u);
+    BlkP p = {};
+    p.dir = (uint8_t)dir;
+    p.pri = (uint8_t)min(pri, 255u);
+    p.sec = (uint8_t)min(sec, 255u);
     const int ps = pri ? max(0, damping - ilog2u(pri)) : 0;  // constrain, cdef.rs:130
     const int ss = sec ? max(0, damping - ilog2u(sec)) : 0;
-    B.shifts = (uint8_t)(min(ps, 15) | (min(ss, 15) << 4));
-    B.flags = 1 | (((pri >> shift) & 1) ? 2 : 0);  // cdef.rs:156-160
+    p.shifts = (uint8_t)(min(ps, 15) | (min(ss, 15) << 4));
+    p.flags = 1 | (((pri >> shift) & 1) ? 2 : 0);  // cdef.rs:156-160
+    return p;
+}
+
+// The block's filter parameters: p with the offsets D of its direction.
+__device__ __forceinline__ Blk blk_from(const BlkP &p, const DirOffs &D)
+{
+    Blk B;
+    B.pri = p.pri;
+    B.sec = p.sec;
+    B.shifts = p.shifts;
+    B.flags = p.flags;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         B.offp[k] = D.offp[k];
@@ -249,7 +267,7 @@ __device__ __forceinline__ Blk make_blk_with(const DirOffs &D, uint32_t pri, uin
 template <int ROW, int SHIFT>
 __device__ __forceinline__ Blk make_blk(int dir, uint32_t pri, uint32_t sec, int damping, int shift)
 {
-    return make_blk_with(dir_offs<ROW, SHIFT>(dir), pri, sec, damping, shift);
+    return blk_from(make_blkp(dir, pri, sec, damping, shift), dir_offs<ROW, SHIFT>(dir));
 }
 
 // filter_block (cdef.rs:142-205), computed for two horizontally adjacent
@@ -632,7 +650,8 @@ __device__ __forceinline__ bool stage_planes(const CdefFrame &f, uint16_t *L, ui
 // dropped). L, U, V: the planes' tiles.
 template <bool BORDER, int SX, int SY, int T, bool COH = false>
 __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, const uint16_t *U,
-                                          const uint16_t *V, const Blk (*blk)[64], const uint8_t *blist,
+                                          const uint16_t *V, const BlkP (*bp)[64], const DirOffs (*dirs)[8],
+                                          const uint8_t *blist,
                                           int nact, int x0, int y0, int ix, bool prio)
 {
     constexpr int CROW = chroma_row<SX>();
@@ -654,7 +673,8 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
             const int ci = (2 + bi * 8 + i) * kRow + 8 + bj * 8 + j;
             s16x2 res;
             if (act) {
-                const Blk B = blk[0][b];
+                const BlkP p = bp[0][b];
+                const Blk B = blk_from(p, dirs[0][p.dir]);
                 filter_pairs<BORDER, 1, 0>(L, ci, B, &res, ypri, ysec);
             } else {
                 res = ld2(L, ci);
@@ -675,8 +695,10 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
         const int bi = b >> 3, bj = b & 7;
         const bool act = e < nact;
         Blk B = {};
-        if (act)
-            B = blk[1][b];
+        if (act) {
+            const BlkP p = bp[1][b];
+            B = blk_from(p, dirs[1][p.dir]);
+        }
         const int cx0 = x0 >> SX, cy0 = y0 >> SY, W1 = f.crop_w[1], H1 = f.crop_h[1];
 #pragma unroll
         for (int k = 0; k < CPX; k++) {
@@ -708,14 +730,21 @@ __device__ __forceinline__ void filter_sb(const SbFields &f, const uint16_t *L, 
 template <int SX, int SY>
 struct __align__(16) CdefLds {
     uint16_t lt[2 * kTile + 4 * chroma_tile<SX, SY>() + 8];
-    Blk blk[2][64];
     DirOffs dirs[2][8];  // the 8 directions' tap offsets, luma then chroma tiles
-    uint32_t cost[8][64];
-    uint8_t bact[64];
-    uint8_t blist[64];  // blocks to filter (ascending), then the rest
+    // The direction costs are dead once the parameter phase has read them;
+    // the block parameters take their place.
+    union {
+        uint32_t cost[8][64];
+        BlkP bp[2][64];  // block parameters, luma then chroma
+    };
+    unsigned long long bact;  // bit b: block b is filtered
+    uint8_t blist[64];        // blocks to filter (ascending), then the rest
     int snact;
     int sidx;
 };
+// Four 4:2:0 workgroups share a CU's 160 KiB of LDS (4K CDEF 36.2 to 34.1 us,
+// 64-frame batch 446 to 424 us against three).
+static_assert(sizeof(CdefLds<1, 1>) <= 160 * 1024 / 4, "four 4:2:0 workgroups per CU");
 
 // One 64x64 luma superblock of one frame, all planes (decode.rs:772-856).
 // SX, SY: chroma subsampling (monochrome frames use <0, 0> with 1 plane).
@@ -746,9 +775,8 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
     constexpr int CW = 64 >> SX, CH = 64 >> SY;
     constexpr int CROW = chroma_row<SX>(), CTILE = chroma_tile<SX, SY>();
     uint16_t *const L = S.lt, *const U = S.lt + 2 * kTile, *const V = U + 2 * CTILE + 8;
-    auto &blk = S.blk;
+    auto &bp = S.bp;
     auto &cost = S.cost;
-    auto &bact = S.bact;
     auto &blist = S.blist;
     int &snact = S.snact;
     int &sidx = S.sidx;
@@ -838,8 +866,9 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
     // Wave 0 stores its share of the staged-bytes copy and turns the loaded
     // skip flags and cdef_idx into the block list.
     auto side_info = [&]() {
-        // Wave 1 tabulates the 8 directions' tap offsets for the block
-        // parameters (their arithmetic then leaves the parameter phase).
+        // Wave 1 tabulates the 8 directions' tap offsets, which the
+        // filtering reads by each block's direction (their arithmetic then
+        // leaves the parameter phase, and a block keeps 8 bytes of LDS).
         if (tid >= 64 && tid < 80) {
             const int d = tid & 7;
             if (tid < 72)
@@ -859,8 +888,9 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
             if (CHAIN && ARG && b * 64u < up_n16 && tid == 0)
                 __hip_atomic_fetch_add(cs.copied, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             const bool a = vidx >= 0 && skip_raw == 0 && !outside;
-            bact[tid] = a;
             const uint64_t m = __ballot(a);
+            if (tid == 0)
+                S.bact = m;
             const int below = __popcll(m & ((1ull << tid) - 1)), nact = __popcll(m);
             blist[a ? below : nact + (tid - below)] = (uint8_t)tid;
             if (tid == 0) {
@@ -873,7 +903,7 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
     auto search = [&]() {
         if (tid < 512) {
             const int b = tid & 63;
-            if (bact[b])
+            if ((S.bact >> b) & 1)
                 cost[tid >> 6][b] =
                     dir_cost_any(tid >> 6, &L[(2 + (b >> 3) * 8) * kRow + 8 + (b & 7) * 8], shift);
         }
@@ -888,21 +918,34 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
     if (idx >= 0) {
         search();
         __syncthreads();
+        // Waves 0 and 1 take their block's 8 costs into registers; after the
+        // barrier the costs' LDS holds the block parameters, which they write.
+        const int b = tid & 63, cls = tid >> 6;
+        const bool act = (S.bact >> b) & 1;
+        uint32_t c[8];
+        if (tid < 128 && act) {
+#pragma unroll
+            for (int d = 0; d < 8; d++)
+                c[d] = cost[d][b];
+        }
+        __syncthreads();
         if (tid < 128) {
-            const int b = tid & 63, cls = tid >> 6;
-            Blk B = {};
-            if (bact[b]) {
+            BlkP B = {};
+            if (act) {
                 int best = 0;
-                uint32_t bc = cost[0][b];
+                uint32_t bc = c[0];
 #pragma unroll
                 for (int d = 1; d < 8; d++) {
-                    const uint32_t c = cost[d][b];
-                    if (c > bc) {  // first maximum wins, cdef.rs:114-119
-                        bc = c;
+                    if (c[d] > bc) {  // first maximum wins, cdef.rs:114-119
+                        bc = c[d];
                         best = d;
                     }
                 }
-                const uint32_t var = (bc - cost[(best + 4) & 7][b]) >> 10;
+                uint32_t opp = c[0];  // cost of the orthogonal direction
+#pragma unroll
+                for (int d = 1; d < 8; d++)
+                    opp = ((best + 4) & 7) == d ? c[d] : opp;
+                const uint32_t var = (bc - opp) >> 10;
                 const int ix = idx & 7;
                 if (cls == 0) {  // decode.rs:806-818
                     const uint32_t pri = sf.strength(0, ix) << shift;
@@ -912,8 +955,8 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
                         const uint32_t lg = sc ? min(ilog2u(sc), 12) : 0;
                         adj = (pri * (4 + lg) + 8) >> 4;
                     }
-                    B = make_blk_with(S.dirs[0][pri ? best : 0], adj, sf.strength(1, ix) << shift,
-                                      (uint8_t)(sf.damping + shift), shift);
+                    B = make_blkp(pri ? best : 0, adj, sf.strength(1, ix) << shift, (uint8_t)(sf.damping + shift),
+                                  shift);
                 } else {
                     const uint32_t pri = sf.strength(2, ix) << shift;
                     // chroma_direction, cdef.rs:222-230: nibble `best` of
@@ -923,11 +966,11 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
                         cdir = (0x66654207u >> (4 * best)) & 7;
                     else if (!SX && SY)
                         cdir = (0x06432221u >> (4 * best)) & 7;
-                    B = make_blk_with(S.dirs[1][pri ? cdir : 0], pri, sf.strength(3, ix) << shift,
-                                      (uint8_t)(sf.damping + shift - 1), shift);
+                    B = make_blkp(pri ? cdir : 0, pri, sf.strength(3, ix) << shift,
+                                  (uint8_t)(sf.damping + shift - 1), shift);
                 }
             }
-            blk[cls][b] = B;
+            bp[cls][b] = B;
         }
         __syncthreads();
     }
@@ -936,9 +979,9 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
     if (prio)
         __builtin_amdgcn_s_setprio(1);
     if (border)
-        filter_sb<true, SX, SY, T, CHAIN>(sf, L, U, V, blk, blist, nact, x0, y0, idx & 7, prio);
+        filter_sb<true, SX, SY, T, CHAIN>(sf, L, U, V, bp, S.dirs, blist, nact, x0, y0, idx & 7, prio);
     else
-        filter_sb<false, SX, SY, T, CHAIN>(sf, L, U, V, blk, blist, nact, x0, y0, idx & 7, prio);
+        filter_sb<false, SX, SY, T, CHAIN>(sf, L, U, V, bp, S.dirs, blist, nact, x0, y0, idx & 7, prio);
     if (CHAIN) {
         // Every wave's output stores have completed before the barrier; then
         // the superblock is published to the restoration tiles that read it.
