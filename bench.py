@@ -99,7 +99,7 @@ def measure_mixed_batch(av1pf, hip, workloads, reps: int, n: int = 64):
     batch = ctx.batch_call(frames, ptr(dev_in), ptr(out_b))
     k1080 = next(k for k, f in enumerate(frames) if f.desc.coded_width == 1920)
     tuning_pf = tune(ctx_pf, frames[k1080:k1080 + 1], ptr(dev_in)[k1080:k1080 + 1], ptr(out_pf)[k1080:k1080 + 1],
-                     reps=3)
+                     reps=15)
     tuning = tune(ctx, frames, ptr(dev_in), ptr(out_b), reps=4)
     for c in per_frame:
         c()
@@ -339,7 +339,7 @@ def measure_per_frame(av1pf, hip, cfg, n_frames, steps, warmup):
     hip.synchronize()
     calls = [ctx.device_call(f, [t.ptr for t in dev_in[k]], [t.ptr for t in dev_out[k]])
              for k, f in enumerate(frames)]
-    tuning = tune(ctx, frames[:1], [[t.ptr for t in dev_in[0]]], [[t.ptr for t in dev_out[0]]], reps=3)
+    tuning = tune(ctx, frames[:1], [[t.ptr for t in dev_in[0]]], [[t.ptr for t in dev_out[0]]], reps=15)
     for i in range(warmup):
         calls[i % len(calls)]()
     ctx.finish()
@@ -470,7 +470,7 @@ def measure_4k10(av1pf, hip, workloads, reps: int, n: int = 16):
     ptr = lambda arrs: [[t.ptr for t in ts] for ts in arrs]
     calls = [ctx_pf.device_call(f, ptr(dev_in)[k], ptr(out_pf)[k]) for k, f in enumerate(frames)]
     batch = ctx.batch_call(frames, ptr(dev_in), ptr(out_b))
-    tuning_pf = tune(ctx_pf, frames[:1], ptr(dev_in)[:1], ptr(out_pf)[:1], reps=3)
+    tuning_pf = tune(ctx_pf, frames[:1], ptr(dev_in)[:1], ptr(out_pf)[:1], reps=15)
     tuning = tune(ctx, frames, ptr(dev_in), ptr(out_b), reps=4)
     for c in calls:
         c()

@@ -439,7 +439,7 @@ vaco_av1pf_status vaco_av1pf_set_tuning(vaco_av1pf_ctx *ctx, const vaco_av1pf_tu
  * several as one vaco_av1pf_batch_device call, so that each path is tuned on
  * its own kernels) `reps` times
  * with each variant of each kernel (after one untimed run), times the
- * kernels by their dispatch events (the fastest of the `reps` runs counts),
+ * kernels by their dispatch events (the median of the `reps` runs counts),
  * keeps the fastest variant per kernel as the context's tuning and returns
  * it in *out. A field leaves its current value only for a variant at least
  * 2% faster, and for the CDEF order that moves more HBM bytes

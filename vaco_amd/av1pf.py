@@ -406,8 +406,9 @@ class Context:
         self._check(self._lib.vaco_av1pf_set_chain_spin_limit(self.handle, polls))
 
     def autotune(self, frames, in_ptrs, out_ptrs, reps: int = 3) -> dict:
-        """Time every launch variant on these frames (device planes) and keep
-        the fastest per kernel; returns the chosen tuning."""
+        """Time every launch variant on these frames (device planes; the
+        median of `reps` runs per variant) and keep the fastest per kernel;
+        returns the chosen tuning."""
         n = len(frames)
         descs = (abi.FrameDesc * n)()
         pins, pouts = (abi.Picture * n)(), (abi.Picture * n)()

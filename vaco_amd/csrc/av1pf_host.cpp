@@ -1973,9 +1973,10 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
         return st;
     double ms[VACO_AV1PF_KERNEL_COUNT];
     uint64_t launches[VACO_AV1PF_KERNEL_COUNT];
-    // Mean time of `kernel` per batch with tuning t (one untimed batch first).
-    // Fastest of `reps` batches (the minimum is robust to a batch slowed by
-    // something else on the device), after one untimed batch.
+    // Time of `kernel` per call with tuning t: the median of `reps` calls
+    // (robust both to a call slowed by something else on the device and to
+    // the wide spread of one frame's short launches, where the fastest of a
+    // few calls picks noise), after one untimed call.
     // One frame is timed through the per-frame path (whose CDEF launch reads
     // its side information in place and copies the staged bytes), several
     // through one batched call: each path's own kernels.
@@ -1984,9 +1985,11 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
             return vaco_av1pf_frame_device(ctx, descs, ins, outs, nullptr);
         return av1pf_enqueue_batch(ctx, n, descs, ins, outs, nullptr, nullptr, nullptr, nullptr, nullptr);
     };
+    std::vector<double> samples;
     auto measure = [&](const vaco_av1pf_tuning &t, int kernel, double *res) {
         ctx->tune = t;
         *res = 0.0;
+        samples.clear();
         vaco_av1pf_status r = run();
         if (r == VACO_AV1PF_OK)
             r = vaco_av1pf_finish(ctx);
@@ -2006,8 +2009,13 @@ extern "C" vaco_av1pf_status vaco_av1pf_autotune(vaco_av1pf_ctx *ctx, uint32_t n
                     v += ms[k];
                     nl += launches[k];
                 }
-            if (r == VACO_AV1PF_OK && nl && (i == 0 || v < *res))
-                *res = v;
+            if (r == VACO_AV1PF_OK && nl)
+                samples.push_back(v);
+        }
+        if (r == VACO_AV1PF_OK && !samples.empty()) {
+            std::sort(samples.begin(), samples.end());
+            const size_t h = samples.size() / 2;
+            *res = samples.size() % 2 ? samples[h] : 0.5 * (samples[h - 1] + samples[h]);
         }
         return r;
     };
