@@ -918,20 +918,19 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
     if (idx >= 0) {
         search();
         __syncthreads();
-        // Waves 0 and 1 take their block's 8 costs into registers; after the
-        // barrier the costs' LDS holds the block parameters, which they write.
-        const int b = tid & 63, cls = tid >> 6;
-        const bool act = (S.bact >> b) & 1;
-        uint32_t c[8];
-        if (tid < 128 && act) {
+        // Wave 0, lane = block: the block's 8 costs into registers, then
+        // its luma and chroma parameters written over the costs' LDS. One
+        // wave's LDS accesses complete in order, so its writes cannot
+        // overtake any lane's reads of the costs, and no other wave reads
+        // them.
+        if (tid < 64) {
+            const int b = tid;
+            BlkP BY = {}, BC = {};
+            if ((S.bact >> b) & 1) {
+                uint32_t c[8];
 #pragma unroll
-            for (int d = 0; d < 8; d++)
-                c[d] = cost[d][b];
-        }
-        __syncthreads();
-        if (tid < 128) {
-            BlkP B = {};
-            if (act) {
+                for (int d = 0; d < 8; d++)
+                    c[d] = cost[d][b];
                 int best = 0;
                 uint32_t bc = c[0];
 #pragma unroll
@@ -947,7 +946,7 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
                     opp = ((best + 4) & 7) == d ? c[d] : opp;
                 const uint32_t var = (bc - opp) >> 10;
                 const int ix = idx & 7;
-                if (cls == 0) {  // decode.rs:806-818
+                {  // luma, decode.rs:806-818
                     const uint32_t pri = sf.strength(0, ix) << shift;
                     uint32_t adj = 0;
                     if (var != 0) {  // adjust_strength, cdef.rs:208-220
@@ -955,9 +954,10 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
                         const uint32_t lg = sc ? min(ilog2u(sc), 12) : 0;
                         adj = (pri * (4 + lg) + 8) >> 4;
                     }
-                    B = make_blkp(pri ? best : 0, adj, sf.strength(1, ix) << shift, (uint8_t)(sf.damping + shift),
-                                  shift);
-                } else {
+                    BY = make_blkp(pri ? best : 0, adj, sf.strength(1, ix) << shift, (uint8_t)(sf.damping + shift),
+                                   shift);
+                }
+                {  // chroma
                     const uint32_t pri = sf.strength(2, ix) << shift;
                     // chroma_direction, cdef.rs:222-230: nibble `best` of
                     // {7,0,2,4,5,6,6,6} (4:2:2) or {1,2,2,2,3,4,6,0} (4:4:0).
@@ -966,11 +966,12 @@ __device__ __forceinline__ void cdef_sb(CdefLds<SX, SY> &S, const CdefFrame *__r
                         cdir = (0x66654207u >> (4 * best)) & 7;
                     else if (!SX && SY)
                         cdir = (0x06432221u >> (4 * best)) & 7;
-                    B = make_blkp(pri ? cdir : 0, pri, sf.strength(3, ix) << shift,
-                                  (uint8_t)(sf.damping + shift - 1), shift);
+                    BC = make_blkp(pri ? cdir : 0, pri, sf.strength(3, ix) << shift,
+                                   (uint8_t)(sf.damping + shift - 1), shift);
                 }
             }
-            bp[cls][b] = B;
+            bp[0][b] = BY;
+            bp[1][b] = BC;
         }
         __syncthreads();
     }
