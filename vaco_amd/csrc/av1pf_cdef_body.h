@@ -7,14 +7,15 @@
 // adjust_strength 208-220, chroma_direction 222-230, frame walk
 // decode.rs:739-861 with cdef_block_skipped 892-896.
 //
-// One workgroup of 1024 threads (16 wave64) per 64x64 luma superblock and
-// its co-located chroma, in XCD-contiguous order. The superblock plus a
-// 2-sample border is staged in LDS once (8-byte loads inside the plane),
-// with a copy shifted by one sample so that every pair of samples is one
-// aligned 32-bit LDS read. The direction search keeps each block's 15 line
+// One workgroup of 1024 or 512 threads (16 or 8 wave64) per 64x64 luma
+// superblock and its co-located chroma, in XCD-contiguous order. The
+// superblock plus a 2-sample border is staged in LDS once (8-byte loads
+// inside the plane), with a copy shifted by one sample so that every pair of
+// samples is one aligned 32-bit LDS read. The direction search keeps each block's 15 line
 // sums in registers (one wave per direction, one lane per 8x8 block,
-// 16-byte LDS row loads). Then 16 lanes filter each 8x8 luma block, two
-// samples at a time in packed 16-bit arithmetic, with the block's
+// 16-byte LDS row loads); one wave turns each block's 8 costs into its
+// luma and chroma parameters. Then a half-wave filters each 8x8 luma block,
+// two samples per lane in packed 16-bit arithmetic, with the block's
 // parameters held in registers. Superblocks that touch a plane edge add one
 // bit select per tap for unavailable taps.
 #pragma once
