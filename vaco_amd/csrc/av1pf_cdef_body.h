@@ -303,6 +303,24 @@ __device__ __forceinline__ s16x2 ldb(const char *a)
 // sample in its place has exactly that effect (constrain(0) = 0, and x is
 // already in [mn, mx]), so BORDER superblocks swap it in with one bit
 // select per tap.
+// The largest and the smallest of three sample pairs, lane by lane, in one
+// packed instruction each: samples are at most 4095, and as f16 bit patterns
+// those are +0, subnormals and small normals, which f16 orders as the
+// integers (the kernels keep f16 subnormals, they do not flush them).
+__device__ __forceinline__ u16x2 max3_u16(u16x2 a, u16x2 b, u16x2 c)
+{
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 m = __builtin_elementwise_maximum(__builtin_bit_cast(h2, a), __builtin_bit_cast(h2, b));
+    return __builtin_bit_cast(u16x2, __builtin_elementwise_maximum(m, __builtin_bit_cast(h2, c)));
+}
+
+__device__ __forceinline__ u16x2 min3_u16(u16x2 a, u16x2 b, u16x2 c)
+{
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 m = __builtin_elementwise_minimum(__builtin_bit_cast(h2, a), __builtin_bit_cast(h2, b));
+    return __builtin_bit_cast(u16x2, __builtin_elementwise_minimum(m, __builtin_bit_cast(h2, c)));
+}
+
 // Taps k0 .. k1-1 (both signs) of filter_pairs. CLAMP = false: no clamp
 // range is kept (see filter_pairs).
 template <bool BORDER, bool CLAMP, int N, int STRIDE, int K0, int K1>
@@ -314,10 +332,11 @@ __device__ __forceinline__ void apply_taps(const char *a, const Blk &B, const u1
     for (int k = K0; k < K1; k++) {
         const s16x2 w = ((k - K0) & 1) ? w1 : w0;
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-            const int o = s ? B.offp[k] : B.offm[k];
+        for (int n = 0; n < N; n++) {
+            u16x2 ps[2];
 #pragma unroll
-            for (int n = 0; n < N; n++) {
+            for (int s = 0; s < 2; s++) {
+                const int o = s ? B.offp[k] : B.offm[k];
                 u16x2 p = __builtin_bit_cast(u16x2, ldb(a + 2 * STRIDE * n + o));
                 if (BORDER) {
                     const uint32_t m =
@@ -332,10 +351,11 @@ __device__ __forceinline__ void apply_taps(const char *a, const Blk &B, const u1
                 const s16x2 c = __builtin_bit_cast(s16x2, __builtin_elementwise_min(up, T)) -
                                 __builtin_bit_cast(s16x2, __builtin_elementwise_min(dn, T));
                 sum[n] += w * c;
-                if (CLAMP) {
-                    mn[n] = __builtin_elementwise_min(mn[n], p);
-                    mx[n] = __builtin_elementwise_max(mx[n], p);
-                }
+                ps[s] = p;
+            }
+            if (CLAMP) {  // both signs of the tap at once
+                mn[n] = min3_u16(mn[n], ps[0], ps[1]);
+                mx[n] = max3_u16(mx[n], ps[0], ps[1]);
             }
         }
     }
