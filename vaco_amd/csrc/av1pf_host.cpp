@@ -456,36 +456,38 @@ extern "C" void vaco_av1pf_destroy(vaco_av1pf_ctx *ctx)
 {
     if (!ctx)
         return;
-    hipSetDevice(ctx->device);
+    // Teardown has no caller to report to: every step runs whatever the
+    // previous one returned.
+    (void)hipSetDevice(ctx->device);
     for (int i = 0; i < kSlots; i++)
         if (ctx->slot[i].pending)
-            hipEventSynchronize(ctx->slot[i].done_ev);
+            (void)hipEventSynchronize(ctx->slot[i].done_ev);
     if (ctx->stream)
-        hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
     for (int i = 0; i < kSlots; i++) {
         if (ctx->slot[i].h)
-            hipHostFree(ctx->slot[i].h);
+            (void)hipHostFree(ctx->slot[i].h);
         if (ctx->slot[i].d)
-            hipFree(ctx->slot[i].d);
+            (void)hipFree(ctx->slot[i].d);
         if (ctx->slot[i].done)
-            hipEventDestroy(ctx->slot[i].done);
+            (void)hipEventDestroy(ctx->slot[i].done);
         if (ctx->slot[i].done_sys)
-            hipEventDestroy(ctx->slot[i].done_sys);
+            (void)hipEventDestroy(ctx->slot[i].done_sys);
         for (auto &gr : ctx->slot[i].graphs)
-            hipGraphExecDestroy(gr.exec);
+            (void)hipGraphExecDestroy(gr.exec);
     }
     if (ctx->h_status)
-        hipHostFree(ctx->h_status);
+        (void)hipHostFree(ctx->h_status);
     for (auto ev : ctx->prof_pool)
-        hipEventDestroy(ev);
+        (void)hipEventDestroy(ev);
     if (ctx->pool)
-        hipFree(ctx->pool);
+        (void)hipFree(ctx->pool);
     if (ctx->ws)
-        hipFree(ctx->ws);
+        (void)hipFree(ctx->ws);
     if (ctx->chain_flags)
-        hipFree(ctx->chain_flags);
+        (void)hipFree(ctx->chain_flags);
     if (ctx->stream)
-        hipStreamDestroy(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -2216,7 +2218,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_cdef_blocks(vaco_av1pf_ctx *ctx, uint32_
         e = hipMemcpyAsync(dirs, ddir, n, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(ctx->stream);
-    hipFree(buf);
+    (void)hipFree(buf);
     if (e != hipSuccess)
         return hip_fail(ctx, e);
     return VACO_AV1PF_OK;
@@ -2282,7 +2284,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_upscale_plane(vaco_av1pf_ctx *ctx, const
         e = hipMemcpyAsync(&flag, dst, 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(ctx->stream);
-    hipFree(buf);
+    (void)hipFree(buf);
     if (e != hipSuccess)
         return hip_fail(ctx, e);
     if (flag & kFlagSampleRange)  // superres.rs:201-203
@@ -2405,7 +2407,7 @@ extern "C" vaco_av1pf_status vaco_av1pf_restore_plane(vaco_av1pf_ctx *ctx,
                              cfg->height, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(ctx->stream);
-    hipFree(buf);
+    (void)hipFree(buf);
     if (e != hipSuccess)
         return hip_fail(ctx, e);
     return VACO_AV1PF_OK;

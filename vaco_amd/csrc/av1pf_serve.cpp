@@ -233,7 +233,7 @@ static void launch(vaco_av1pf_server *s)
 
 static void work(vaco_av1pf_server *s)
 {
-    hipSetDevice(av1pf_ctx_device(s->ctx));
+    (void)hipSetDevice(av1pf_ctx_device(s->ctx));  // a failure shows in the calls after it
     const uint64_t window = (uint64_t)s->cfg.window_us * 1000;
     for (;;) {
         {
@@ -306,15 +306,15 @@ extern "C" vaco_av1pf_status vaco_av1pf_server_create(vaco_av1pf_ctx *ctx,
     if (st != VACO_AV1PF_OK) {
         for (int i = 0; i < kMaxStreams; i++) {
             if (s->streams[i])
-                hipStreamDestroy(s->streams[i]);
+                (void)hipStreamDestroy(s->streams[i]);
             for (auto &e : s->mark[i])
                 if (e)
-                    hipEventDestroy(e);
+                    (void)hipEventDestroy(e);
         }
         if (s->t0)
-            hipEventDestroy(s->t0);
+            (void)hipEventDestroy(s->t0);
         if (s->ws_mem)
-            hipFree(s->ws_mem);
+            (void)hipFree(s->ws_mem);
         delete s;
         return st;
     }
@@ -399,16 +399,16 @@ extern "C" void vaco_av1pf_server_destroy(vaco_av1pf_server *s)
         s->worker.join();
     for (int i = 0; i < kMaxStreams; i++) {
         if (s->streams[i]) {
-            hipStreamSynchronize(s->streams[i]);
-            hipStreamDestroy(s->streams[i]);
+            (void)hipStreamSynchronize(s->streams[i]);
+            (void)hipStreamDestroy(s->streams[i]);
         }
         for (auto &e : s->mark[i])
             if (e)
-                hipEventDestroy(e);
+                (void)hipEventDestroy(e);
     }
     if (s->t0)
-        hipEventDestroy(s->t0);
+        (void)hipEventDestroy(s->t0);
     if (s->ws_mem)
-        hipFree(s->ws_mem);
+        (void)hipFree(s->ws_mem);
     delete s;
 }
